@@ -1,0 +1,209 @@
+"""Headline benchmark: KeySwitch/s at N=2^16, L=24, K=6, dnum=4 (alpha=6) on one
+MI355X, batched (64 components per step, captured HIP graph, resident key)
+versus plain (one component per submission, per-stage launches, a sync per
+request), with p50/p99 latency, the NTT's HBM roofline fraction and the CPU
+oracle timed on this host beside it.
+
+A "step" is one batched key switch of `--batch` components already resident
+in HBM. Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ntt-gpu_amd"))
+
+from nttgpu import hip, params  # noqa: E402
+
+N = 65536
+L, K, ALPHA = 24, 6, 6
+HBM_PEAK_GBS = 8000.0
+
+
+def pct(values, q):
+    v = sorted(values)
+    pos = (len(v) - 1) * q
+    lo = int(pos)
+    hi = min(lo + 1, len(v) - 1)
+    return v[lo] + (v[hi] - v[lo]) * (pos - lo)
+
+
+def seeded(n, moduli, seed):
+    rng = np.random.default_rng(seed)
+    return np.concatenate([rng.integers(0, q, n, dtype=np.uint64) for q in moduli])
+
+
+def ntt_roofline(towers, reps, stream):
+    """Forward NTT over `towers` distinct 2^16-point towers (one ~2^59-bit prime):
+    the key switch's dominant kernel pair, measured with HIP events on its stream."""
+    q = params.last_prime(59, 2 * N)
+    ctx = hip.Context(N, [q], [params.min_root(q, N)])
+    data = hip.DeviceBuffer(towers * N)
+    chunk = 256
+    for t0 in range(0, towers, chunk):
+        data.upload(seeded(N, [q] * min(chunk, towers - t0), 17 + t0), offset=t0 * N)
+    e0, e1 = hip.Event(), hip.Event()
+    ctx.transform_batch(data.ptr, towers, N, 0, 1, False, stream=stream)  # warm
+    hip.Stream(0, stream).synchronize()
+    times = []
+    for r in range(reps):
+        e0.record(stream)
+        ctx.transform_batch(data.ptr, towers, N, 0, 1, r % 2 == 1, stream=stream)
+        e1.record(stream)
+        e1.synchronize()
+        times.append(e0.elapsed_ms(e1))
+    fwd = [t for i, t in enumerate(times) if i % 2 == 0]
+    inv = [t for i, t in enumerate(times) if i % 2 == 1]
+    alg_bytes = 16 * N * towers  # read + write residues once per direction
+    ms = statistics.mean(fwd)
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    data.free()
+    ctx.close()
+    return {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "kernel": "forward NTT (ntt_pass_a256 + ntt_pass_b), N=65536",
+        "towers_per_launch": towers,
+        "launch_ms": round(ms, 4),
+        "inverse_launch_ms": round(statistics.mean(inv), 4),
+        "alg_bytes_per_launch": alg_bytes,
+    }
+
+
+def cpu_baseline(hp, x0, kb, ka, gpu_b, gpu_a, samples):
+    """The oracle's key switch (reference algorithm, 1 thread, towers in sequence)
+    on component 0 of the same batch; also checks the GPU output bit-exactly."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as o
+    o.set_threads(1)
+    times, exact = [], True
+    for _ in range(samples):
+        t0 = time.perf_counter()
+        wb, wa = o.key_switch(x0, N, hp.q, hp.q_psi, hp.M, hp.p, hp.p_psi, hp.alpha, kb, ka)
+        times.append(time.perf_counter() - t0)
+        exact = exact and np.array_equal(wb, gpu_b) and np.array_equal(wa, gpu_a)
+    return {
+        "value": round(1.0 / statistics.median(times), 4),
+        "unit": "KeySwitch/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{samples} key switch(es) of one component (N=2^16, L=24, K=6, dnum=4), C oracle, "
+                  f"1 thread, towers in sequence; median {statistics.median(times):.2f} s",
+    }, exact
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--plain-requests", type=int, default=64)
+    ap.add_argument("--ntt-towers", type=int, default=2048)
+    ap.add_argument("--cpu-samples", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
+                                   "(data-parallel over ciphertexts), so --gpus must be 1"}))
+        sys.exit(2)
+
+    hp = params.openfhe_like(N, L, K, ALPHA)
+    B = args.batch
+    ctx = hip.Context(N, hp.q + hp.p, hp.q_psi + hp.p_psi)
+    ks = hip.KeySwitch(ctx, L, K, ALPHA, max_batch=B)
+    full = hp.q + hp.p
+    beta = hp.beta()
+    kb = np.concatenate([seeded(N, full, 1000 + d) for d in range(beta)])
+    ka = np.concatenate([seeded(N, full, 2000 + d) for d in range(beta)])
+    ks.upload_key(kb, ka, hp.M, hp.q)
+    x = np.concatenate([seeded(N, hp.q, 5000 + c) for c in range(B)])
+    comp = L * N
+    d_in = hip.DeviceBuffer(B * comp).upload(x)
+    d_b, d_a = hip.DeviceBuffer(B * comp), hip.DeviceBuffer(B * comp)
+    ws = hip.DeviceBuffer(ks.workspace_words(B))
+    stream = hip.Stream(0)
+    s = stream.handle
+
+    # ---- batched: captured graph, one launch per step
+    graph = ks.capture(d_in.ptr, B, d_b.ptr, d_a.ptr, ws.ptr, s)
+    for _ in range(args.warmup):
+        graph.launch(s)
+    stream.synchronize()
+    events = [hip.Event() for _ in range(args.steps + 1)]
+    t0 = time.perf_counter()
+    events[0].record(s)
+    for i in range(args.steps):
+        graph.launch(s)
+        events[i + 1].record(s)
+    stream.synchronize()
+    wall = time.perf_counter() - t0
+    step_ms = [events[i].elapsed_ms(events[i + 1]) for i in range(args.steps)]
+    batched_kss = B * args.steps / wall
+    gpu_b = d_b.download(comp, 0)
+    gpu_a = d_a.download(comp, 0)
+
+    # ---- plain: one component per submission, per-stage launches, sync per request
+    ws1 = hip.DeviceBuffer(ks.workspace_words(1))
+    for _ in range(2):
+        ks.run(d_in.ptr, 1, d_b.ptr, d_a.ptr, ws1.ptr, s)
+    stream.synchronize()
+    lat = []
+    t1 = time.perf_counter()
+    for r in range(args.plain_requests):
+        c = r % B
+        r0 = time.perf_counter()
+        ks.run(d_in.at(c * comp), 1, d_b.at(c * comp), d_a.at(c * comp), ws1.ptr, s)
+        stream.synchronize()
+        lat.append((time.perf_counter() - r0) * 1e3)
+    plain_wall = time.perf_counter() - t1
+    plain_kss = args.plain_requests / plain_wall
+
+    roof = ntt_roofline(args.ntt_towers, 6, s)
+    cpu, exact = (None, None)
+    if not args.no_cpu:
+        cpu, exact = cpu_baseline(hp, x[:comp], kb, ka, gpu_b, gpu_a, args.cpu_samples)
+
+    out = {
+        "metric": "KeySwitch/s N=2^16 L=24 dnum=4 batched vs plain; p50/p99 ms; NTT HBM GB/s",
+        "value": round(batched_kss, 2),
+        "unit": "KeySwitch/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic: seeded uniform residues per prime (inputs and evaluation key)",
+        "config": {"workload": "batched Hybrid key switch, N=65536, L=24 Q (60+50-bit), K=6 P (60-bit), "
+                               "dnum=4 (alpha=6), OpenFHE-shaped primes and minimum roots",
+                   "batch": B, "graph": True},
+        "batched_p50_ms": round(pct(step_ms, 0.5), 3),
+        "batched_p99_ms": round(pct(step_ms, 0.99), 3),
+        "plain_keyswitch_per_s": round(plain_kss, 2),
+        "plain_p50_ms": round(pct(lat, 0.5), 3),
+        "plain_p99_ms": round(pct(lat, 0.99), 3),
+        "batched_over_plain": round(batched_kss / plain_kss, 3),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "bit_exact_vs_oracle": exact,
+        "device": hip.device_name(0),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,150 @@
+/*
+ * ntt_hip.h — C ABI of the gfx950 CKKS hot-path library (libntt_hip.so).
+ *
+ * Plain pointers and sizes only. Device buffers are raw device pointers to
+ * tower-major uint64 residues: element (tower t, index i) at t * n + i
+ * (reference docs/openfhe-representation-contract.md:385-404). Every call
+ * returns NTT_HIP_OK or a negative status; ntt_hip_last_error() gives the
+ * message of the calling thread's last failure. NTT_HIP_EINVAL is a contract
+ * violation detected before any device work (the reference's ContractError,
+ * include/ntt/backend_contract.hpp:20-26); NTT_HIP_ERUNTIME is a HIP runtime
+ * failure (the reference's std::runtime_error("CUDA: ..."),
+ * src/cuda/ntt_kernels.cu:40-46).
+ *
+ * Streams and events are opaque handles (hipStream_t / hipEvent_t). A NULL
+ * stream argument means the context's own ordered stream
+ * (src/cuda/resident_adapter.inc:91).
+ */
+#ifndef NTT_HIP_H
+#define NTT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+#pragma GCC visibility push(default)
+
+#define NTT_HIP_OK 0
+#define NTT_HIP_NOT_READY 1
+#define NTT_HIP_EINVAL (-1)
+#define NTT_HIP_ERUNTIME (-2)
+
+#define NTT_HIP_H2D 1
+#define NTT_HIP_D2H 2
+#define NTT_HIP_D2D 3
+
+typedef struct ntt_hip_context ntt_hip_context;
+typedef struct ntt_hip_keyswitch ntt_hip_keyswitch;
+
+/* ---- status ---- */
+const char *ntt_hip_last_error(void);
+int ntt_hip_abi_version(void);
+
+/* ---- device, memory, streams, events: the plumbing of the reference CUDA
+ * Driver (src/cuda/resident_adapter.inc:131-156 Buffer/Event,
+ * :178-197 upload/copy/copy_range, :281-301 download/record/ready/wait/quiesce,
+ * src/cuda/ntt_kernels.cu:57-75 DeviceGuard). ---- */
+int ntt_hip_device_count(int *count);
+int ntt_hip_device_name(int device, char *name, size_t capacity);
+int ntt_hip_malloc(int device, void **ptr, size_t bytes);
+int ntt_hip_free(int device, void *ptr);
+int ntt_hip_host_alloc(void **ptr, size_t bytes); /* pinned host staging */
+int ntt_hip_host_free(void *ptr);
+/* kind: NTT_HIP_H2D / D2H / D2D; asynchronous on `stream` */
+int ntt_hip_memcpy(int device, void *dst, const void *src, size_t bytes, int kind, void *stream);
+int ntt_hip_stream_create(int device, void **stream);
+int ntt_hip_stream_destroy(int device, void *stream);
+int ntt_hip_stream_synchronize(int device, void *stream);
+int ntt_hip_event_create(int device, void **event, int timing);
+int ntt_hip_event_destroy(int device, void *event);
+int ntt_hip_event_record(int device, void *event, void *stream);
+int ntt_hip_event_query(int device, void *event); /* NTT_HIP_OK ready, NTT_HIP_NOT_READY pending */
+int ntt_hip_event_synchronize(int device, void *event);
+int ntt_hip_event_elapsed(float *ms, void *start, void *stop);
+
+/* ---- context: one fixed basis of towers (q_t, psi_t) of ring dimension n
+ * with its device twiddle tables (reference resident_adapter.inc:60-130
+ * Context; CudaNttTransform ctor src/cuda/ntt_kernels.cu:336-386;
+ * preflight include/ntt/cuda_validation.hpp:17-37: power-of-two n in
+ * [2, 65536], prime q < 2^63 with 2n | q-1, psi^n = -1). ---- */
+int ntt_hip_context_create(ntt_hip_context **ctx, int device, uint64_t n, const uint64_t *q, const uint64_t *psi,
+                           size_t towers);
+int ntt_hip_context_destroy(ntt_hip_context *ctx);
+void *ntt_hip_context_stream(ntt_hip_context *ctx);
+int ntt_hip_context_device(ntt_hip_context *ctx);
+/* bytes of device tables held by the context (reference Driver::constant_bytes,
+ * resident_driver.hpp:28 / resident_adapter.inc:171-174) */
+size_t ntt_hip_context_table_bytes(ntt_hip_context *ctx);
+
+/* ---- Driver seam operations (include/ntt/resident_driver.hpp:13-57) on
+ * contiguous towers [first, first + towers) of the context basis ---- */
+/* Driver::transform (resident_adapter.inc:198-203): in place on `data`;
+ * `src` != NULL reads the input from src (out of place). Forward: natural
+ * coefficients -> bit-reversed evaluations; inverse: the exact inverse. */
+int ntt_hip_transform(ntt_hip_context *ctx, uint64_t *data, const uint64_t *src, size_t first, size_t towers,
+                      int inverse, void *stream);
+/* Batched form: `polys` polynomials of `towers` towers each, poly p at
+ * data + p * poly_stride (elements). One launch for all of them. */
+int ntt_hip_transform_batch(ntt_hip_context *ctx, uint64_t *data, const uint64_t *src, size_t polys,
+                            size_t poly_stride, size_t src_poly_stride, size_t first, size_t towers, int inverse,
+                            void *stream);
+/* Driver::pointwise (resident_adapter.inc:204-213): out[i] = out[i] * right[i] mod q */
+int ntt_hip_pointwise(ntt_hip_context *ctx, uint64_t *out, const uint64_t *right, size_t first, size_t towers,
+                      void *stream);
+/* Driver::switch_basis (resident_adapter.inc:254-266, include/ntt/basis_conversion.hpp:13-114):
+ * coefficient-domain unreduced CRT from source towers src[0..s) (indices into
+ * the context basis, input tower-major s*n) to target towers dst[0..t)
+ * (output t*n). 1..40 each, distinct and disjoint. */
+int ntt_hip_switch_basis(ntt_hip_context *ctx, uint64_t *out, const uint64_t *in, const size_t *src, size_t s,
+                         const size_t *dst, size_t t, void *stream);
+/* Driver::correct (resident_adapter.inc:267-280, src/resident.cpp:404-430):
+ * out = (left - right) * P^-1 mod q on towers [first, first+towers); P is the
+ * product of context towers p[0..k). */
+int ntt_hip_correct(ntt_hip_context *ctx, uint64_t *out, const uint64_t *left, const uint64_t *right, size_t first,
+                    size_t towers, const size_t *p, size_t k, void *stream);
+/* Driver::hybrid_mac (resident_adapter.inc:214-231): b,a = (first ? 0 : b,a) + digit * key_{b,a} */
+int ntt_hip_hybrid_mac(ntt_hip_context *ctx, uint64_t *b, uint64_t *a, const uint64_t *digit, const uint64_t *key_b,
+                       const uint64_t *key_a, size_t first, size_t towers, int first_digit, void *stream);
+/* Driver::hybrid_rotate (resident_adapter.inc:232-253): sigma_k(B + P*c0), sigma_k(A)
+ * on evaluation QP towers [first, first+towers) whose first active_q are Q;
+ * P = product of context towers p[0..k). */
+int ntt_hip_hybrid_rotate(ntt_hip_context *ctx, uint64_t *out_b, uint64_t *out_a, const uint64_t *ext_b,
+                          const uint64_t *ext_a, const uint64_t *c0, size_t first, size_t towers, size_t active_q,
+                          const size_t *p, size_t k, uint32_t automorphism, void *stream);
+
+/* ---- fused Hybrid key switch (KeySwitchHybrid core:
+ * include/ntt/openfhe_hybrid_adapter.hpp:163-181,212-219; graph
+ * src/resident.cpp:458-570 + 448-457). The context basis must be Q towers
+ * [0, L) followed by P towers [L, L + K); alpha = towers per digit. ---- */
+int ntt_hip_keyswitch_create(ntt_hip_keyswitch **ks, ntt_hip_context *ctx, size_t L, size_t K, size_t alpha,
+                             size_t max_batch);
+int ntt_hip_keyswitch_destroy(ntt_hip_keyswitch *ks);
+/* Resident evaluation key (Session::upload_hybrid_key, src/resident.cpp:492-529):
+ * host key_b/key_a hold ceil(M/alpha) digits of the FULL context (M Q towers,
+ * then the K P towers), digit-major, each (M + K) * n; q_full/psi_full are the
+ * M full-context Q towers (their first L must equal the context's Q towers).
+ * Active rows use key index i (< L) and i + (M - L) (P towers). */
+int ntt_hip_keyswitch_upload_key(ntt_hip_keyswitch *ks, const uint64_t *key_b, const uint64_t *key_a, size_t M,
+                                 const uint64_t *q_full);
+/* Device workspace bytes for `batch` components. */
+int ntt_hip_keyswitch_workspace_bytes(ntt_hip_keyswitch *ks, size_t batch, size_t *bytes);
+/* Key-switch `batch` components: in = batch * L * n evaluation residues,
+ * out_b / out_a = batch * L * n each (B then A contributions, evaluation Q).
+ * All pointers are device pointers; workspace of the size above. */
+int ntt_hip_keyswitch_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batch, uint64_t *out_b, uint64_t *out_a,
+                          void *workspace, void *stream);
+/* The same batch captured once as a HIP graph on `stream` (fixed buffer
+ * addresses; no memset nodes) and replayed by ntt_hip_graph_launch. No
+ * reference counterpart: the reference launches every stage per call. */
+int ntt_hip_keyswitch_capture(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batch, uint64_t *out_b,
+                              uint64_t *out_a, void *workspace, void *stream, void **graph);
+int ntt_hip_graph_launch(void *graph, void *stream);
+int ntt_hip_graph_destroy(void *graph);
+
+#pragma GCC visibility pop
+#ifdef __cplusplus
+}
+#endif
+#endif /* NTT_HIP_H */

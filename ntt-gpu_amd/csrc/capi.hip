@@ -1,0 +1,994 @@
+// C ABI of the library: contexts with device tables, cached conversion plans,
+// the Driver-seam operations and the fused batched Hybrid key switch.
+// Declarations and reference citations: include/ntt_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/ntt_hip.h"
+#include "conv_device.hpp"
+
+using namespace nttamd;
+using u128 = unsigned __int128;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct ContractError : std::invalid_argument {
+  using std::invalid_argument::invalid_argument;
+};
+struct RuntimeError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+void require(bool ok, const char *why) {
+  if (!ok) throw ContractError(why);
+}
+
+#define HIP_CHECK(call)                                                                          \
+  do {                                                                                           \
+    const hipError_t hip_status_ = (call);                                                       \
+    if (hip_status_ != hipSuccess)                                                               \
+      throw RuntimeError(std::string("HIP: ") + #call + ": " + hipGetErrorString(hip_status_)); \
+  } while (0)
+
+template <class F>
+int guarded(F &&f) {
+  try {
+    return f();
+  } catch (const ContractError &e) {
+    g_last_error = e.what();
+    return NTT_HIP_EINVAL;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return NTT_HIP_ERUNTIME;
+  } catch (...) {
+    g_last_error = "unknown failure";
+    return NTT_HIP_ERUNTIME;
+  }
+}
+
+// Restores the caller's current device (reference src/cuda/ntt_kernels.cu:57-75).
+class DeviceGuard {
+  int previous_ = -1;
+  bool changed_ = false;
+
+ public:
+  explicit DeviceGuard(int device) {
+    HIP_CHECK(hipGetDevice(&previous_));
+    if (previous_ != device) {
+      HIP_CHECK(hipSetDevice(device));
+      changed_ = true;
+    }
+  }
+  ~DeviceGuard() {
+    if (changed_) (void)hipSetDevice(previous_);
+  }
+  DeviceGuard(const DeviceGuard &) = delete;
+  DeviceGuard &operator=(const DeviceGuard &) = delete;
+};
+
+// ---------------------------------------------------------------- host arithmetic
+uint64_t mulmod(uint64_t a, uint64_t b, uint64_t q) { return (uint64_t)((u128)a * b % q); }
+uint64_t powmod(uint64_t a, uint64_t e, uint64_t q) {
+  uint64_t r = 1 % q;
+  a %= q;
+  while (e) {
+    if (e & 1) r = mulmod(r, a, q);
+    a = mulmod(a, a, q);
+    e >>= 1;
+  }
+  return r;
+}
+uint64_t invmod(uint64_t x, uint64_t q) { return powmod(x % q, q - 2, q); }
+uint64_t shoup_of(uint64_t w, uint64_t q) { return (uint64_t)(((u128)w << 64) / q); }
+uint64_t neg_inv64(uint64_t q) {  // -q^-1 mod 2^64, q odd (Newton)
+  uint64_t x = q;  // correct to 3 bits
+  for (int i = 0; i < 6; ++i) x *= 2 - q * x;
+  return ~x + 1;
+}
+uint64_t mont_of(uint64_t x, uint64_t q) { return (uint64_t)(((u128)(x % q) << 64) % q); }
+uint32_t bitrev(uint32_t x, uint32_t lg) {
+  uint32_t r = 0;
+  for (uint32_t i = 0; i < lg; ++i) {
+    r = (r << 1) | (x & 1);
+    x >>= 1;
+  }
+  return r;
+}
+bool is_prime(uint64_t n) {
+  if (n < 2) return false;
+  static const uint64_t bases[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+  for (uint64_t p : bases)
+    if (n % p == 0) return n == p;
+  uint64_t d = n - 1, s = 0;
+  while ((d & 1) == 0) {
+    d >>= 1;
+    ++s;
+  }
+  for (uint64_t a : bases) {
+    uint64_t x = powmod(a, d, n);
+    if (x == 1 || x == n - 1) continue;
+    bool witness = true;
+    for (uint64_t r = 0; r + 1 < s; ++r) {
+      x = mulmod(x, x, n);
+      if (x == n - 1) {
+        witness = false;
+        break;
+      }
+    }
+    if (witness) return false;
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------- device blobs
+struct DeviceBlob {
+  int device = 0;
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  DeviceBlob() = default;
+  DeviceBlob(int dev, size_t n) : device(dev), bytes(n) {
+    DeviceGuard g(dev);
+    HIP_CHECK(hipMalloc(&ptr, n ? n : 1));
+  }
+  ~DeviceBlob() {
+    if (ptr) {
+      try {
+        DeviceGuard g(device);
+        (void)hipFree(ptr);
+      } catch (...) {
+      }
+    }
+  }
+  DeviceBlob(const DeviceBlob &) = delete;
+  DeviceBlob &operator=(const DeviceBlob &) = delete;
+};
+
+struct BasisPlanDev {
+  std::unique_ptr<DeviceBlob> blob;
+  const BasisConst *dev = nullptr;
+  uint32_t s = 0, t = 0;
+  bool per_term = false;
+};
+
+}  // namespace
+
+struct ntt_hip_context {
+  int device = 0;
+  uint32_t n = 0, log_n = 0;
+  std::vector<uint64_t> q, psi;
+  bool lazy = true;
+  hipStream_t stream = nullptr;
+  std::unique_ptr<DeviceBlob> tables;  // twiddles of every tower
+  std::unique_ptr<DeviceBlob> primes;  // PrimeConst[towers]
+  std::vector<PrimeConst> host_primes;
+  std::mutex mu;
+  std::map<std::pair<std::vector<size_t>, std::vector<size_t>>, std::shared_ptr<BasisPlanDev>> basis_cache;
+  std::map<std::pair<std::vector<size_t>, std::vector<size_t>>, std::unique_ptr<DeviceBlob>> scale_cache;
+
+  const PrimeConst *dprimes() const { return (const PrimeConst *)primes->ptr; }
+  hipStream_t pick(void *s) const { return s ? (hipStream_t)s : stream; }
+  void check_range(size_t first, size_t towers) const {
+    require(towers > 0 && first < q.size() && towers <= q.size() - first, "tower range outside the context basis");
+  }
+  ~ntt_hip_context() {
+    try {
+      DeviceGuard g(device);
+      if (stream) {
+        (void)hipStreamSynchronize(stream);
+        (void)hipStreamDestroy(stream);
+      }
+    } catch (...) {
+    }
+  }
+};
+
+namespace {
+
+// Approximate CRT plan for context towers src -> dst (reference
+// include/ntt/basis_conversion.hpp:32-114): Bhat_i^-1 mod b_i with Shoup
+// companion, and (Bhat_i mod m) * 2^64 mod m for Montgomery accumulation.
+std::shared_ptr<BasisPlanDev> basis_plan(ntt_hip_context &c, const std::vector<size_t> &src,
+                                         const std::vector<size_t> &dst) {
+  std::lock_guard<std::mutex> lock(c.mu);
+  auto key = std::make_pair(src, dst);
+  auto it = c.basis_cache.find(key);
+  if (it != c.basis_cache.end()) return it->second;
+  const size_t s = src.size(), t = dst.size();
+  require(s >= 1 && s <= 40 && t >= 1 && t <= 40, "basis switch requires 1..40 source and target primes");
+  std::vector<uint64_t> all;
+  for (size_t i : src) {
+    require(i < c.q.size(), "source tower outside the context basis");
+    all.push_back(c.q[i]);
+  }
+  for (size_t i : dst) {
+    require(i < c.q.size(), "target tower outside the context basis");
+    all.push_back(c.q[i]);
+  }
+  std::sort(all.begin(), all.end());
+  require(std::adjacent_find(all.begin(), all.end()) == all.end(), "basis switch requires distinct disjoint primes");
+  std::vector<SourceConst> sc(s);
+  std::vector<TargetConst> tc(t);
+  std::vector<uint64_t> hats(s * t);
+  u128 sum_src = 0;
+  for (size_t k = 0; k < s; ++k) {
+    const uint64_t b = c.q[src[k]];
+    sum_src += b;
+    uint64_t hat = 1;
+    for (size_t o = 0; o < s; ++o)
+      if (o != k) hat = mulmod(hat, c.q[src[o]] % b, b);
+    sc[k].q = b;
+    sc[k].inv = invmod(hat, b);
+    sc[k].inv_shoup = shoup_of(sc[k].inv, b);
+    for (size_t m = 0; m < t; ++m) {
+      const uint64_t qm = c.q[dst[m]];
+      uint64_t h = 1;
+      for (size_t o = 0; o < s; ++o)
+        if (o != k) h = mulmod(h, c.q[src[o]] % qm, qm);
+      hats[k * t + m] = mont_of(h, qm);
+    }
+  }
+  for (size_t m = 0; m < t; ++m) {
+    tc[m].q = c.q[dst[m]];
+    tc[m].qinv_neg = neg_inv64(tc[m].q);
+  }
+  auto plan = std::make_shared<BasisPlanDev>();
+  plan->s = (uint32_t)s;
+  plan->t = (uint32_t)t;
+  // exact single reduction needs sum_i y_i * hat < m * 2^64, i.e. sum b_i < 2^64
+  plan->per_term = sum_src >= ((u128)1 << 64);
+  const size_t off_src = 64, off_dst = off_src + s * sizeof(SourceConst), off_hat = off_dst + t * sizeof(TargetConst);
+  const size_t bytes = off_hat + s * t * sizeof(uint64_t);
+  plan->blob = std::make_unique<DeviceBlob>(c.device, bytes);
+  char *base = (char *)plan->blob->ptr;
+  BasisConst head{(uint32_t)s, (uint32_t)t, (const SourceConst *)(base + off_src),
+                  (const TargetConst *)(base + off_dst), (const uint64_t *)(base + off_hat)};
+  std::vector<char> host(bytes);
+  std::memcpy(host.data(), &head, sizeof(head));
+  std::memcpy(host.data() + off_src, sc.data(), s * sizeof(SourceConst));
+  std::memcpy(host.data() + off_dst, tc.data(), t * sizeof(TargetConst));
+  std::memcpy(host.data() + off_hat, hats.data(), s * t * sizeof(uint64_t));
+  {
+    DeviceGuard g(c.device);
+    HIP_CHECK(hipMemcpy(base, host.data(), bytes, hipMemcpyHostToDevice));
+  }
+  plan->dev = (const BasisConst *)base;
+  c.basis_cache.emplace(key, plan);
+  return plan;
+}
+
+// Per-tower {c, Shoup(c)} constants cached by (towers, P towers); mode 0:
+// P^-1 mod q (correction), mode 1: P mod q (rotation).
+const ulonglong2 *scale_table(ntt_hip_context &c, size_t first, size_t towers, const std::vector<size_t> &p,
+                              int mode) {
+  std::lock_guard<std::mutex> lock(c.mu);
+  std::vector<size_t> key_t = {first, towers, (size_t)mode};
+  auto key = std::make_pair(key_t, p);
+  auto it = c.scale_cache.find(key);
+  if (it != c.scale_cache.end()) return (const ulonglong2 *)it->second->ptr;
+  std::vector<ulonglong2> host(towers);
+  for (size_t t = 0; t < towers; ++t) {
+    const uint64_t q = c.q[first + t];
+    uint64_t prod = 1;
+    for (size_t j : p) prod = mulmod(prod, c.q[j] % q, q);
+    const uint64_t v = mode == 0 ? invmod(prod, q) : prod;
+    host[t].x = v;
+    host[t].y = shoup_of(v, q);
+  }
+  auto blob = std::make_unique<DeviceBlob>(c.device, towers * sizeof(ulonglong2));
+  {
+    DeviceGuard g(c.device);
+    HIP_CHECK(hipMemcpy(blob->ptr, host.data(), towers * sizeof(ulonglong2), hipMemcpyHostToDevice));
+  }
+  const ulonglong2 *out = (const ulonglong2 *)blob->ptr;
+  c.scale_cache.emplace(key, std::move(blob));
+  return out;
+}
+
+void validate_towers(uint64_t n, const uint64_t *q, const uint64_t *psi, size_t towers) {
+  require(n >= 2 && n <= 65536 && (n & (n - 1)) == 0, "requires power-of-two N in [2,65536]");
+  require(q && psi && towers > 0 && towers <= 65535, "one (q, psi) pair per tower; 1..65535 towers");
+  for (size_t k = 0; k < towers; ++k) {
+    require(q[k] > n && q[k] < (uint64_t{1} << 63) && q[k] % (2 * n) == 1 && is_prime(q[k]),
+            "requires prime q < 2^63 with 2N dividing q-1");
+    require(psi[k] != 0 && psi[k] < q[k] && powmod(psi[k], n, q[k]) == q[k] - 1,
+            "requires a canonical primitive 2N-th root");
+  }
+}
+
+TowerMap affine(uint64_t *base, const uint64_t *src, uint64_t n, size_t towers, size_t first, size_t poly_stride,
+                size_t src_poly_stride) {
+  TowerMap m{};
+  m.base = base;
+  m.src = src;
+  m.jobs = nullptr;
+  m.poly_stride = poly_stride;
+  m.src_poly_stride = src_poly_stride;
+  m.tower_stride = n;
+  m.towers_per_poly = (uint32_t)towers;
+  m.prime_offset = (uint32_t)first;
+  return m;
+}
+
+void hip_ok(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw RuntimeError(std::string("HIP: ") + what + ": " + hipGetErrorString(e));
+}
+
+}  // namespace
+
+// ================================================================ key switch plan
+struct ntt_hip_keyswitch {
+  ntt_hip_context *ctx = nullptr;
+  uint32_t L = 0, K = 0, alpha = 0, beta = 0;
+  size_t max_batch = 0;
+  std::vector<std::shared_ptr<BasisPlanDev>> up;  // per digit: digit towers -> complement
+  std::shared_ptr<BasisPlanDev> down;             // P -> Q
+  const ulonglong2 *pinv = nullptr;               // P^-1 mod q_i, i < L
+  std::unique_ptr<DeviceBlob> tower_prime;        // identity map 0..L+K
+  std::unique_ptr<DeviceBlob> jobs;               // complement NTT jobs for max_batch
+  size_t jobs_per_poly = 0;
+  std::unique_ptr<DeviceBlob> key;                // beta*(L+K)*n * 2 (b then a), Montgomery form
+  bool key_loaded = false;
+  bool mac_per_term = false;
+
+  size_t n() const { return ctx->n; }
+  // workspace layout (elements per component)
+  size_t coeff_words() const { return (size_t)L * n(); }
+  size_t digit_words() const { return (size_t)beta * (L + K) * n(); }
+  size_t z_words() const { return (size_t)(L + K) * n(); }
+  size_t pc_words() const { return (size_t)K * n(); }
+  size_t corr_words() const { return (size_t)L * n(); }
+  size_t per_component() const {
+    return coeff_words() + digit_words() + 2 * z_words() + 2 * pc_words() + 2 * corr_words();
+  }
+};
+
+extern "C" {
+
+const char *ntt_hip_last_error(void) { return g_last_error.c_str(); }
+int ntt_hip_abi_version(void) { return 1; }
+
+int ntt_hip_device_count(int *count) {
+  return guarded([&] {
+    require(count != nullptr, "null count");
+    HIP_CHECK(hipGetDeviceCount(count));
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_device_name(int device, char *name, size_t capacity) {
+  return guarded([&] {
+    require(name && capacity > 0, "null name buffer");
+    hipDeviceProp_t p;
+    HIP_CHECK(hipGetDeviceProperties(&p, device));
+    std::snprintf(name, capacity, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_malloc(int device, void **ptr, size_t bytes) {
+  return guarded([&] {
+    require(ptr != nullptr && bytes > 0, "invalid allocation request");
+    DeviceGuard g(device);
+    HIP_CHECK(hipMalloc(ptr, bytes));
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_free(int device, void *ptr) {
+  return guarded([&] {
+    DeviceGuard g(device);
+    HIP_CHECK(hipFree(ptr));
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_host_alloc(void **ptr, size_t bytes) {
+  return guarded([&] {
+    require(ptr != nullptr && bytes > 0, "invalid allocation request");
+    HIP_CHECK(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_host_free(void *ptr) {
+  return guarded([&] {
+    HIP_CHECK(hipHostFree(ptr));
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_memcpy(int device, void *dst, const void *src, size_t bytes, int kind, void *stream) {
+  return guarded([&] {
+    require(dst && src, "null copy pointer");
+    require(kind == NTT_HIP_H2D || kind == NTT_HIP_D2H || kind == NTT_HIP_D2D, "invalid copy kind");
+    if (bytes == 0) return NTT_HIP_OK;
+    const hipMemcpyKind k = kind == NTT_HIP_H2D   ? hipMemcpyHostToDevice
+                            : kind == NTT_HIP_D2H ? hipMemcpyDeviceToHost
+                                                  : hipMemcpyDeviceToDevice;
+    DeviceGuard g(device);
+    HIP_CHECK(hipMemcpyAsync(dst, src, bytes, k, (hipStream_t)stream));
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_stream_create(int device, void **stream) {
+  return guarded([&] {
+    require(stream != nullptr, "null stream out");
+    DeviceGuard g(device);
+    hipStream_t s;
+    HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_stream_destroy(int device, void *stream) {
+  return guarded([&] {
+    DeviceGuard g(device);
+    HIP_CHECK(hipStreamDestroy((hipStream_t)stream));
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_stream_synchronize(int device, void *stream) {
+  return guarded([&] {
+    DeviceGuard g(device);
+    HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_event_create(int device, void **event, int timing) {
+  return guarded([&] {
+    require(event != nullptr, "null event out");
+    DeviceGuard g(device);
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreateWithFlags(&e, timing ? hipEventDefault : hipEventDisableTiming));
+    *event = e;
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_event_destroy(int device, void *event) {
+  return guarded([&] {
+    DeviceGuard g(device);
+    HIP_CHECK(hipEventDestroy((hipEvent_t)event));
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_event_record(int device, void *event, void *stream) {
+  return guarded([&] {
+    DeviceGuard g(device);
+    HIP_CHECK(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_event_query(int device, void *event) {
+  return guarded([&] {
+    DeviceGuard g(device);
+    const hipError_t s = hipEventQuery((hipEvent_t)event);
+    if (s == hipErrorNotReady) return NTT_HIP_NOT_READY;
+    HIP_CHECK(s);
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_event_synchronize(int device, void *event) {
+  return guarded([&] {
+    DeviceGuard g(device);
+    HIP_CHECK(hipEventSynchronize((hipEvent_t)event));
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_event_elapsed(float *ms, void *start, void *stop) {
+  return guarded([&] {
+    require(ms != nullptr, "null elapsed out");
+    HIP_CHECK(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return NTT_HIP_OK;
+  });
+}
+
+// ---------------------------------------------------------------- context
+int ntt_hip_context_create(ntt_hip_context **out, int device, uint64_t n, const uint64_t *q, const uint64_t *psi,
+                           size_t towers) {
+  return guarded([&] {
+    require(out != nullptr, "null context out");
+    require(device >= 0, "invalid device ordinal");
+    validate_towers(n, q, psi, towers);
+    auto c = std::make_unique<ntt_hip_context>();
+    c->device = device;
+    c->n = (uint32_t)n;
+    c->log_n = 0;
+    while ((1u << c->log_n) < n) ++c->log_n;
+    c->q.assign(q, q + towers);
+    c->psi.assign(psi, psi + towers);
+    c->lazy = std::all_of(c->q.begin(), c->q.end(), [](uint64_t v) { return v < (uint64_t{1} << 62); });
+    // Host tables: fwd[k] = {psi^bitrev(k), Shoup}, inv[k] = {psi^-bitrev(k), Shoup}
+    // (reference transform_stage.hpp:19-30), one thread per group of towers.
+    std::vector<ulonglong2> host(towers * 2 * (size_t)n);
+    auto build = [&](size_t t0, size_t t1) {
+      for (size_t t = t0; t < t1; ++t) {
+        const uint64_t qt = c->q[t];
+        for (int dir = 0; dir < 2; ++dir) {
+          const uint64_t root = dir == 0 ? c->psi[t] : invmod(c->psi[t], qt);
+          ulonglong2 *tab = host.data() + (t * 2 + dir) * (size_t)n;
+          uint64_t power = 1;
+          for (uint32_t e = 0; e < n; ++e) {
+            ulonglong2 &w = tab[bitrev(e, c->log_n)];
+            w.x = power;
+            w.y = shoup_of(power, qt);
+            power = mulmod(power, root, qt);
+          }
+        }
+      }
+    };
+    const size_t workers = std::min<size_t>(towers, std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    std::vector<std::thread> pool;
+    for (size_t w = 0; w < workers; ++w)
+      pool.emplace_back(build, towers * w / workers, towers * (w + 1) / workers);
+    for (auto &th : pool) th.join();
+    DeviceGuard g(device);
+    HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->tables = std::make_unique<DeviceBlob>(device, host.size() * sizeof(ulonglong2));
+    HIP_CHECK(hipMemcpy(c->tables->ptr, host.data(), host.size() * sizeof(ulonglong2), hipMemcpyHostToDevice));
+    const ulonglong2 *dt = (const ulonglong2 *)c->tables->ptr;
+    c->host_primes.resize(towers);
+    for (size_t t = 0; t < towers; ++t) {
+      PrimeConst &p = c->host_primes[t];
+      const uint64_t qt = c->q[t];
+      p.q = qt;
+      p.two_q = 2 * qt;
+      p.qinv_neg = neg_inv64(qt);
+      p.ninv = invmod(n % qt, qt);
+      p.ninv_shoup = shoup_of(p.ninv, qt);
+      const uint64_t w1 = host[(t * 2 + 1) * (size_t)n + 1].x;  // psi^-bitrev(1)
+      p.last_w = mulmod(w1, p.ninv, qt);
+      p.last_wp = shoup_of(p.last_w, qt);
+      p.r2 = (uint64_t)(((u128)mont_of(1, qt) << 64) % qt);  // 2^128 mod q
+      p.fwd = dt + (t * 2) * (size_t)n;
+      p.inv = dt + (t * 2 + 1) * (size_t)n;
+    }
+    c->primes = std::make_unique<DeviceBlob>(device, towers * sizeof(PrimeConst));
+    HIP_CHECK(hipMemcpy(c->primes->ptr, c->host_primes.data(), towers * sizeof(PrimeConst), hipMemcpyHostToDevice));
+    *out = c.release();
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_context_destroy(ntt_hip_context *ctx) {
+  return guarded([&] {
+    delete ctx;
+    return NTT_HIP_OK;
+  });
+}
+
+void *ntt_hip_context_stream(ntt_hip_context *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+int ntt_hip_context_device(ntt_hip_context *ctx) { return ctx ? ctx->device : -1; }
+size_t ntt_hip_context_table_bytes(ntt_hip_context *ctx) {
+  return ctx ? ctx->tables->bytes + ctx->primes->bytes : 0;
+}
+
+int ntt_hip_transform(ntt_hip_context *ctx, uint64_t *data, const uint64_t *src, size_t first, size_t towers,
+                      int inverse, void *stream) {
+  return ntt_hip_transform_batch(ctx, data, src, 1, 0, 0, first, towers, inverse, stream);
+}
+
+int ntt_hip_transform_batch(ntt_hip_context *ctx, uint64_t *data, const uint64_t *src, size_t polys,
+                            size_t poly_stride, size_t src_poly_stride, size_t first, size_t towers, int inverse,
+                            void *stream) {
+  return guarded([&] {
+    require(ctx && data, "null context or buffer");
+    ctx->check_range(first, towers);
+    require(polys >= 1 && polys * towers <= 65535, "1..65535 tower transforms per launch");
+    require(polys == 1 || (poly_stride >= towers * ctx->n && (!src || src_poly_stride >= towers * ctx->n)),
+            "polynomial stride shorter than its towers");
+    DeviceGuard g(ctx->device);
+    const TowerMap m = affine(data, src, ctx->n, towers, first, poly_stride, src_poly_stride);
+    hip_ok(launch_ntt(m, ctx->dprimes(), ctx->log_n, (uint32_t)(polys * towers), inverse != 0, ctx->lazy,
+                      ctx->pick(stream)),
+           "ntt launch");
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_pointwise(ntt_hip_context *ctx, uint64_t *out, const uint64_t *right, size_t first, size_t towers,
+                      void *stream) {
+  return guarded([&] {
+    require(ctx && out && right, "null context or buffer");
+    ctx->check_range(first, towers);
+    DeviceGuard g(ctx->device);
+    hip_ok(launch_pointwise(out, right, ctx->dprimes(), (uint32_t)first, ctx->n, (uint32_t)towers, ctx->pick(stream)),
+           "pointwise launch");
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_switch_basis(ntt_hip_context *ctx, uint64_t *out, const uint64_t *in, const size_t *src, size_t s,
+                         const size_t *dst, size_t t, void *stream) {
+  return guarded([&] {
+    require(ctx && out && in && src && dst, "null argument");
+    auto plan = basis_plan(*ctx, std::vector<size_t>(src, src + s), std::vector<size_t>(dst, dst + t));
+    BasisLaunch a{};
+    a.plan = plan->dev;
+    a.in = in;
+    a.out = out;
+    a.n = ctx->n;
+    a.targets_per_block = (uint32_t)std::min<size_t>(t, 8);
+    a.plan_host_s = plan->s;
+    a.plan_host_t = plan->t;
+    a.skip_at = 0xffffffffu;
+    a.skip_len = 0;
+    DeviceGuard g(ctx->device);
+    hip_ok(launch_basis_switch(a, plan->per_term, 1, ctx->pick(stream)), "basis switch launch");
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_correct(ntt_hip_context *ctx, uint64_t *out, const uint64_t *left, const uint64_t *right, size_t first,
+                    size_t towers, const size_t *p, size_t k, void *stream) {
+  return guarded([&] {
+    require(ctx && out && left && right && p && k > 0, "null argument");
+    ctx->check_range(first, towers);
+    std::vector<size_t> pv(p, p + k);
+    for (size_t j : pv) require(j < ctx->q.size() && (j < first || j >= first + towers), "invalid P tower");
+    CorrectLaunch c{};
+    c.out = out;
+    c.left = left;
+    c.right = right;
+    c.primes = ctx->dprimes();
+    c.scale = scale_table(*ctx, first, towers, pv, 0);
+    c.first_prime = (uint32_t)first;
+    c.n = ctx->n;
+    c.towers = (uint32_t)towers;
+    DeviceGuard g(ctx->device);
+    hip_ok(launch_correct(c, 1, ctx->pick(stream)), "correction launch");
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_hybrid_mac(ntt_hip_context *ctx, uint64_t *b, uint64_t *a, const uint64_t *digit, const uint64_t *key_b,
+                       const uint64_t *key_a, size_t first, size_t towers, int first_digit, void *stream) {
+  return guarded([&] {
+    require(ctx && b && a && digit && key_b && key_a, "null argument");
+    ctx->check_range(first, towers);
+    DeviceGuard g(ctx->device);
+    hip_ok(launch_mac_digit(b, a, digit, key_b, key_a, ctx->dprimes(), (uint32_t)first, ctx->n, (uint32_t)towers,
+                            first_digit != 0, ctx->pick(stream)),
+           "hybrid mac launch");
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_hybrid_rotate(ntt_hip_context *ctx, uint64_t *out_b, uint64_t *out_a, const uint64_t *ext_b,
+                          const uint64_t *ext_a, const uint64_t *c0, size_t first, size_t towers, size_t active_q,
+                          const size_t *p, size_t k, uint32_t automorphism, void *stream) {
+  return guarded([&] {
+    require(ctx && out_b && out_a && ext_b && ext_a && c0 && p && k > 0, "null argument");
+    ctx->check_range(first, towers);
+    require(active_q >= 1 && active_q <= towers, "invalid active Q count");
+    require((automorphism & 1) && automorphism < 2 * ctx->n, "invalid negacyclic automorphism index");
+    std::vector<size_t> pv(p, p + k);
+    RotateLaunch r{};
+    r.out_b = out_b;
+    r.out_a = out_a;
+    r.ext_b = ext_b;
+    r.ext_a = ext_a;
+    r.c0 = c0;
+    r.primes = ctx->dprimes();
+    r.scale = scale_table(*ctx, first, active_q, pv, 1);
+    r.first_prime = (uint32_t)first;
+    r.n = ctx->n;
+    r.log_n = ctx->log_n;
+    r.active_q = (uint32_t)active_q;
+    r.automorphism = automorphism;
+    DeviceGuard g(ctx->device);
+    hip_ok(launch_rotate(r, (uint32_t)towers, ctx->pick(stream)), "rotation launch");
+    return NTT_HIP_OK;
+  });
+}
+
+// ---------------------------------------------------------------- fused key switch
+int ntt_hip_keyswitch_create(ntt_hip_keyswitch **out, ntt_hip_context *ctx, size_t L, size_t K, size_t alpha,
+                             size_t max_batch) {
+  return guarded([&] {
+    require(out && ctx, "null argument");
+    require(L >= 1 && K >= 1 && L + K == ctx->q.size(), "context basis must be L Q towers then K P towers");
+    require(alpha >= 1 && alpha <= L, "invalid digit size");
+    require(max_batch >= 1, "invalid batch capacity");
+    auto ks = std::make_unique<ntt_hip_keyswitch>();
+    ks->ctx = ctx;
+    ks->L = (uint32_t)L;
+    ks->K = (uint32_t)K;
+    ks->alpha = (uint32_t)alpha;
+    ks->beta = (uint32_t)((L + alpha - 1) / alpha);
+    ks->max_batch = max_batch;
+    const size_t T = L + K, n = ctx->n;
+    for (uint32_t d = 0; d < ks->beta; ++d) {
+      const size_t first = (size_t)d * alpha, count = std::min(alpha, L - first);
+      std::vector<size_t> src, dst;
+      for (size_t i = 0; i < count; ++i) src.push_back(first + i);
+      for (size_t i = 0; i < T; ++i)
+        if (i < first || i >= first + count) dst.push_back(i);
+      // the 40-prime plan limit applies per target chunk (reference src/resident.cpp:474-484)
+      require(count <= 40, "digit wider than 40 towers");
+      require(dst.size() <= 40, "complement wider than 40 towers");
+      ks->up.push_back(basis_plan(*ctx, src, dst));
+    }
+    std::vector<size_t> p, q;
+    for (size_t i = 0; i < K; ++i) p.push_back(L + i);
+    for (size_t i = 0; i < L; ++i) q.push_back(i);
+    require(K <= 40 && L <= 40, "ModDown limited to 40 towers per basis");
+    ks->down = basis_plan(*ctx, p, q);
+    ks->pinv = scale_table(*ctx, 0, L, p, 0);
+    std::vector<uint32_t> tp(T);
+    for (size_t i = 0; i < T; ++i) tp[i] = (uint32_t)i;
+    ks->tower_prime = std::make_unique<DeviceBlob>(ctx->device, T * sizeof(uint32_t));
+    // complement NTT jobs: for component c, digit d, tower t not owned by d
+    std::vector<TowerJob> jobs;
+    for (size_t c = 0; c < max_batch; ++c)
+      for (uint32_t d = 0; d < ks->beta; ++d) {
+        const size_t first = (size_t)d * alpha, count = std::min(alpha, L - first);
+        for (size_t t = 0; t < T; ++t) {
+          if (t >= first && t < first + count) continue;
+          TowerJob j{};
+          j.offset = (c * ks->digit_words() + ((size_t)d * T + t) * n);
+          j.src_offset = j.offset;
+          j.prime = (uint32_t)t;
+          jobs.push_back(j);
+        }
+      }
+    ks->jobs_per_poly = jobs.size() / max_batch;
+    ks->jobs = std::make_unique<DeviceBlob>(ctx->device, jobs.size() * sizeof(TowerJob));
+    // Montgomery single-reduction bound for the MAC: beta * max q < 2^64
+    u128 bound = 0;
+    for (size_t i = 0; i < T; ++i) bound = std::max<u128>(bound, (u128)ctx->q[i] * ks->beta);
+    ks->mac_per_term = bound >= ((u128)1 << 64);
+    ks->key = std::make_unique<DeviceBlob>(ctx->device, 2 * ks->beta * T * n * sizeof(uint64_t));
+    DeviceGuard g(ctx->device);
+    HIP_CHECK(hipMemcpy(ks->tower_prime->ptr, tp.data(), T * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(ks->jobs->ptr, jobs.data(), jobs.size() * sizeof(TowerJob), hipMemcpyHostToDevice));
+    *out = ks.release();
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_keyswitch_destroy(ntt_hip_keyswitch *ks) {
+  return guarded([&] {
+    delete ks;
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_keyswitch_upload_key(ntt_hip_keyswitch *ks, const uint64_t *key_b, const uint64_t *key_a, size_t M,
+                                 const uint64_t *q_full) {
+  return guarded([&] {
+    require(ks && key_b && key_a && q_full, "null argument");
+    const size_t L = ks->L, K = ks->K, T = L + K, n = ks->n(), full = M + K;
+    require(M >= L, "full context has fewer Q towers than the active level");
+    for (size_t i = 0; i < L; ++i) require(q_full[i] == ks->ctx->q[i], "full-context Q prefix mismatch");
+    const size_t full_digits = (M + ks->alpha - 1) / ks->alpha;
+    require(full_digits >= ks->beta, "key has fewer digits than the active level needs");
+    // canonical check and active row selection (i < L: i; P: i + (M - L))
+    std::vector<uint64_t> host(2 * (size_t)ks->beta * T * n);
+    for (int ch = 0; ch < 2; ++ch) {
+      const uint64_t *src = ch == 0 ? key_b : key_a;
+      for (size_t d = 0; d < ks->beta; ++d)
+        for (size_t t = 0; t < T; ++t) {
+          const size_t kt = t < L ? t : t + (M - L);
+          const uint64_t q = ks->ctx->q[t];
+          const uint64_t *row = src + (d * full + kt) * n;
+          uint64_t *dst = host.data() + ((size_t)ch * ks->beta * T + d * T + t) * n;
+          for (size_t i = 0; i < n; ++i) {
+            require(row[i] < q, "noncanonical key residue");
+            dst[i] = row[i];
+          }
+        }
+    }
+    DeviceGuard g(ks->ctx->device);
+    hipStream_t s = ks->ctx->stream;
+    HIP_CHECK(hipMemcpyAsync(ks->key->ptr, host.data(), host.size() * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    // to Montgomery form: rows are (channel, digit, tower): tower prime = t
+    std::vector<uint32_t> tp(2 * ks->beta * T);
+    for (size_t r = 0; r < tp.size(); ++r) tp[r] = (uint32_t)(r % T);
+    DeviceBlob tpd(ks->ctx->device, tp.size() * sizeof(uint32_t));
+    HIP_CHECK(hipMemcpyAsync(tpd.ptr, tp.data(), tp.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    hip_ok(launch_to_mont((uint64_t *)ks->key->ptr, ks->ctx->dprimes(), (const uint32_t *)tpd.ptr, (uint32_t)n,
+                          (uint32_t)tp.size(), s),
+           "key conversion launch");
+    HIP_CHECK(hipStreamSynchronize(s));
+    ks->key_loaded = true;
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_keyswitch_workspace_bytes(ntt_hip_keyswitch *ks, size_t batch, size_t *bytes) {
+  return guarded([&] {
+    require(ks && bytes, "null argument");
+    require(batch >= 1 && batch <= ks->max_batch, "batch exceeds the plan capacity");
+    *bytes = batch * ks->per_component() * sizeof(uint64_t);
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_keyswitch_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batch, uint64_t *out_b,
+                                     uint64_t *out_a, void *workspace, void *stream) {
+  return guarded([&] {
+    require(ks && in && out_b && out_a && workspace, "null argument");
+    require(ks->key_loaded, "evaluation key not uploaded");
+    require(batch >= 1 && batch <= ks->max_batch, "batch exceeds the plan capacity");
+    ntt_hip_context &c = *ks->ctx;
+    const size_t L = ks->L, K = ks->K, T = L + K, n = c.n, B = batch;
+    const hipStream_t s = c.pick(stream);
+    uint64_t *ws = (uint64_t *)workspace;
+    uint64_t *coeff = ws;                              // B * L * n
+    uint64_t *digits = coeff + B * ks->coeff_words();  // B * beta * T * n
+    uint64_t *z = digits + B * ks->digit_words();      // [2][B] * T * n (b channel, then a)
+    uint64_t *pc = z + 2 * B * ks->z_words();          // [2][B] * K * n
+    uint64_t *corr = pc + 2 * B * ks->pc_words();      // [2][B] * L * n
+    const PrimeConst *pr = c.dprimes();
+    DeviceGuard g(c.device);
+    // 1. ModUp sources: coefficient form of every Q tower (INTT, out of place)
+    hip_ok(launch_ntt(affine(coeff, in, n, L, 0, L * n, L * n), pr, c.log_n, (uint32_t)(B * L), true, c.lazy, s),
+           "intt");
+    // 2. per digit: unreduced CRT into the complement, around the digit's own towers
+    for (uint32_t d = 0; d < ks->beta; ++d) {
+      const auto &plan = *ks->up[d];
+      const uint32_t first = d * ks->alpha, count = std::min<uint32_t>(ks->alpha, ks->L - first);
+      BasisLaunch a{};
+      a.plan = plan.dev;
+      a.in = coeff + (size_t)first * n;
+      a.out = digits + (size_t)d * T * n;
+      a.in_poly_stride = ks->coeff_words();
+      a.out_poly_stride = ks->digit_words();
+      a.n = (uint32_t)n;
+      a.targets_per_block = 8;
+      a.plan_host_s = plan.s;
+      a.plan_host_t = plan.t;
+      a.skip_at = first;
+      a.skip_len = count;
+      hip_ok(launch_basis_switch(a, plan.per_term, (uint32_t)B, s), "modup conversion");
+    }
+    // 3. forward NTT of every complement tower of every digit (one launch)
+    {
+      TowerMap m{};
+      m.base = digits;
+      m.jobs = (const TowerJob *)ks->jobs->ptr;
+      m.tower_stride = n;
+      hip_ok(launch_ntt(m, pr, c.log_n, (uint32_t)(B * ks->jobs_per_poly), false, c.lazy, s), "modup ntt");
+    }
+    // 4. key inner product over all digits, both channels
+    {
+      MacLaunch m{};
+      m.own = in;
+      m.own_poly_stride = L * n;
+      m.alpha = ks->alpha;
+      m.active_q = ks->L;
+      m.digits = digits;
+      m.key_b = (const uint64_t *)ks->key->ptr;
+      m.key_a = m.key_b + (size_t)ks->beta * T * n;
+      m.out_b = z;
+      m.out_a = z + B * ks->z_words();
+      m.primes = pr;
+      m.tower_prime = (const uint32_t *)ks->tower_prime->ptr;
+      m.digit_poly_stride = ks->digit_words();
+      m.out_poly_stride = ks->z_words();
+      m.n = (uint32_t)n;
+      m.towers = (uint32_t)T;
+      m.digits_count = ks->beta;
+      hip_ok(launch_mac_fused(m, ks->mac_per_term, (uint32_t)B, s), "mac");
+    }
+    // 5. ModDown of both channels (2B polynomials): INTT(P) -> CRT to Q -> NTT -> correct
+    hip_ok(launch_ntt(affine(pc, z + L * n, n, K, L, K * n, T * n), pr, c.log_n, (uint32_t)(2 * B * K), true, c.lazy,
+                      s),
+           "moddown intt");
+    {
+      BasisLaunch a{};
+      a.plan = ks->down->dev;
+      a.in = pc;
+      a.out = corr;
+      a.in_poly_stride = K * n;
+      a.out_poly_stride = L * n;
+      a.n = (uint32_t)n;
+      a.targets_per_block = 8;
+      a.plan_host_s = ks->down->s;
+      a.plan_host_t = ks->down->t;
+      a.skip_at = 0xffffffffu;
+      a.skip_len = 0;
+      hip_ok(launch_basis_switch(a, ks->down->per_term, (uint32_t)(2 * B), s), "moddown conversion");
+    }
+    hip_ok(launch_ntt(affine(corr, nullptr, n, L, 0, L * n, 0), pr, c.log_n, (uint32_t)(2 * B * L), false, c.lazy, s),
+           "moddown ntt");
+    for (int ch = 0; ch < 2; ++ch) {
+      CorrectLaunch cl{};
+      cl.out = ch == 0 ? out_b : out_a;
+      cl.left = z + (size_t)ch * B * ks->z_words();
+      cl.right = corr + (size_t)ch * B * L * n;
+      cl.out_poly_stride = L * n;
+      cl.left_poly_stride = ks->z_words();
+      cl.right_poly_stride = L * n;
+      cl.primes = pr;
+      cl.scale = ks->pinv;
+      cl.first_prime = 0;
+      cl.n = (uint32_t)n;
+      cl.towers = (uint32_t)L;
+      hip_ok(launch_correct(cl, (uint32_t)B, s), "correction");
+    }
+    return NTT_HIP_OK;
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- graphs
+namespace {
+struct GraphHandle {
+  int device;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+}  // namespace
+
+extern "C" {
+
+int ntt_hip_keyswitch_capture(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batch, uint64_t *out_b,
+                              uint64_t *out_a, void *workspace, void *stream, void **graph) {
+  return guarded([&] {
+    require(ks && graph && stream, "capture needs a key switch, an explicit stream and a graph out");
+    DeviceGuard g(ks->ctx->device);
+    auto h = std::make_unique<GraphHandle>();
+    h->device = ks->ctx->device;
+    const hipStream_t s = (hipStream_t)stream;
+    HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int status = ntt_hip_keyswitch_run(ks, in, batch, out_b, out_a, workspace, stream);
+    hipGraph_t captured = nullptr;
+    const hipError_t end = hipStreamEndCapture(s, &captured);
+    if (status != NTT_HIP_OK) {
+      if (captured) (void)hipGraphDestroy(captured);
+      const std::string why = g_last_error;
+      throw RuntimeError("capture failed: " + why);
+    }
+    HIP_CHECK(end);
+    h->graph = captured;
+    HIP_CHECK(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
+    *graph = h.release();
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_graph_launch(void *graph, void *stream) {
+  return guarded([&] {
+    require(graph != nullptr, "null graph");
+    auto *h = (GraphHandle *)graph;
+    DeviceGuard g(h->device);
+    HIP_CHECK(hipGraphLaunch(h->exec, (hipStream_t)stream));
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_graph_destroy(void *graph) {
+  return guarded([&] {
+    auto *h = (GraphHandle *)graph;
+    if (!h) return NTT_HIP_OK;
+    DeviceGuard g(h->device);
+    if (h->exec) (void)hipGraphExecDestroy(h->exec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    delete h;
+    return NTT_HIP_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,222 @@
+// Basis conversion, key inner product, ModDown correction, automorphism and
+// pointwise kernels for gfx950.
+//
+// Basis conversion follows the approximate (unreduced) CRT of reference
+// include/ntt/basis_conversion.hpp:13-30: y_i = [x_i * Bhat_i^-1]_{b_i},
+// out_m = [sum_i y_i * Bhat_i]_m with NO reduction modulo B. Instead of one
+// mul_mod + add_mod per term (reference), the sum is accumulated exactly in
+// 128 bits and reduced once per output with Montgomery, the hat residues
+// being stored as (Bhat_i mod m) * 2^64 mod m so the result is the plain
+// residue. A per-term reduction variant covers sums that could exceed
+// m * 2^64 (very wide or very many source primes).
+#include "conv_device.hpp"
+
+namespace nttamd {
+
+// ---------------------------------------------------------------- basis switch
+// blockIdx.x: 256 coefficients; blockIdx.y: (poly, target chunk).
+// Dynamic LDS: s * 256 words for the scaled source residues y_i.
+template <bool PerTerm>
+__global__ __launch_bounds__(256) void basis_switch_kernel(BasisLaunch a) {
+  extern __shared__ uint64_t ys[];
+  const BasisConst &p = *a.plan;
+  const uint32_t s = p.s, t = p.t;
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t chunks = (t + a.targets_per_block - 1) / a.targets_per_block;
+  const uint32_t poly = blockIdx.y / chunks, chunk = blockIdx.y - poly * chunks;
+  const uint64_t *in = a.in + (uint64_t)poly * a.in_poly_stride;
+  uint64_t *out = a.out + (uint64_t)poly * a.out_poly_stride;
+  if (i >= a.n) return;
+  for (uint32_t k = 0; k < s; ++k) {
+    const uint64_t x = in[(uint64_t)k * a.n + i];
+    ys[k * 256 + threadIdx.x] = shoup(x, p.src[k].inv, p.src[k].inv_shoup, p.src[k].q);
+  }
+  const uint32_t m0 = chunk * a.targets_per_block;
+  const uint32_t m1 = min(t, m0 + a.targets_per_block);
+  for (uint32_t m = m0; m < m1; ++m) {
+    const uint64_t q = p.dst[m].q, qn = p.dst[m].qinv_neg;
+    const uint64_t *hat = p.hat_mont + m;  // hat_mont[k * t + m]
+    uint64_t r;
+    if constexpr (!PerTerm) {
+      U128 acc{0, 0};
+      for (uint32_t k = 0; k < s; ++k) add_wide(acc, mul_wide(ys[k * 256 + threadIdx.x], hat[(uint64_t)k * t]));
+      r = mont_reduce(acc, q, qn);
+    } else {
+      r = 0;
+      for (uint32_t k = 0; k < s; ++k)
+        r = add_mod(r, mont_reduce(mul_wide(ys[k * 256 + threadIdx.x], hat[(uint64_t)k * t]), q, qn), q);
+    }
+    const uint32_t ot = m < a.skip_at ? m : m + a.skip_len;
+    out[(uint64_t)ot * a.n + i] = r;
+  }
+}
+
+hipError_t launch_basis_switch(const BasisLaunch &a, bool per_term, uint32_t polys, hipStream_t stream) {
+  const uint32_t chunks = (a.plan_host_t + a.targets_per_block - 1) / a.targets_per_block;
+  dim3 grid((a.n + 255) / 256, chunks * polys);
+  const size_t lds = (size_t)a.plan_host_s * 256 * sizeof(uint64_t);
+  if (lds > 64 * 1024) {
+    // up to 40 sources: 80 KiB of the 160 KiB gfx950 LDS
+    (void)hipFuncSetAttribute((const void *)basis_switch_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    (void)hipFuncSetAttribute((const void *)basis_switch_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+  }
+  if (per_term) hipLaunchKernelGGL(basis_switch_kernel<true>, grid, dim3(256), lds, stream, a);
+  else hipLaunchKernelGGL(basis_switch_kernel<false>, grid, dim3(256), lds, stream, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- key inner product
+// Reference per-digit semantics (resident_adapter.inc:31-43, resident_conversion.hpp:50-54):
+// acc = first ? d*k : acc + d*k (mod q), canonical key rows.
+__global__ __launch_bounds__(256) void mac_digit_kernel(uint64_t *b, uint64_t *a, const uint64_t *digit,
+                                                        const uint64_t *kb, const uint64_t *ka,
+                                                        const PrimeConst *primes, uint32_t first_prime,
+                                                        uint32_t n, uint32_t towers, int first_digit) {
+  const uint64_t idx = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (uint64_t)n * towers) return;
+  const PrimeConst &pc = primes[first_prime + idx / n];
+  const uint64_t d = digit[idx];
+  // mont(mont(d*k) * 2^128) = d*k mod q
+  const uint64_t pb = mont_mul(mont_mul(d, kb[idx], pc.q, pc.qinv_neg), pc.r2, pc.q, pc.qinv_neg);
+  const uint64_t pa = mont_mul(mont_mul(d, ka[idx], pc.q, pc.qinv_neg), pc.r2, pc.q, pc.qinv_neg);
+  b[idx] = first_digit ? pb : add_mod(b[idx], pb, pc.q);
+  a[idx] = first_digit ? pa : add_mod(a[idx], pa, pc.q);
+}
+
+hipError_t launch_mac_digit(uint64_t *b, uint64_t *a, const uint64_t *digit, const uint64_t *kb,
+                            const uint64_t *ka, const PrimeConst *primes, uint32_t first_prime, uint32_t n,
+                            uint32_t towers, bool first_digit, hipStream_t stream) {
+  const uint64_t total = (uint64_t)n * towers;
+  hipLaunchKernelGGL(mac_digit_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, stream, b, a, digit, kb,
+                     ka, primes, first_prime, n, towers, (int)first_digit);
+  return hipGetLastError();
+}
+
+// Fused inner product over all digits with resident Montgomery-form keys:
+// Z[t][i] = sum_d D_d[t][i] * K_d[t][i] for both channels, one reduction per output.
+// digits: poly-major (poly, digit, tower, i); keys: (digit, tower, i) in Montgomery form.
+template <bool PerTerm>
+__global__ __launch_bounds__(256) void mac_fused_kernel(MacLaunch m) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t poly = blockIdx.y / m.towers, tower = blockIdx.y - poly * m.towers;
+  if (i >= m.n) return;
+  const PrimeConst &pc = m.primes[m.tower_prime[tower]];
+  const uint64_t *dg = m.digits + (uint64_t)poly * m.digit_poly_stride + (uint64_t)tower * m.n + i;
+  const uint64_t *kb = m.key_b + (uint64_t)tower * m.n + i;
+  const uint64_t *ka = m.key_a + (uint64_t)tower * m.n + i;
+  const uint64_t step = (uint64_t)m.towers * m.n;
+  const uint32_t own_digit = tower < m.active_q ? tower / m.alpha : 0xffffffffu;
+  const uint64_t own = tower < m.active_q ? m.own[(uint64_t)poly * m.own_poly_stride + (uint64_t)tower * m.n + i] : 0;
+  uint64_t rb, ra;
+  if constexpr (!PerTerm) {
+    U128 sb{0, 0}, sa{0, 0};
+    for (uint32_t d = 0; d < m.digits_count; ++d) {
+      const uint64_t x = d == own_digit ? own : dg[d * step];
+      add_wide(sb, mul_wide(x, kb[d * step]));
+      add_wide(sa, mul_wide(x, ka[d * step]));
+    }
+    rb = mont_reduce(sb, pc.q, pc.qinv_neg);
+    ra = mont_reduce(sa, pc.q, pc.qinv_neg);
+  } else {
+    rb = ra = 0;
+    for (uint32_t d = 0; d < m.digits_count; ++d) {
+      const uint64_t x = d == own_digit ? own : dg[d * step];
+      rb = add_mod(rb, mont_mul(x, kb[d * step], pc.q, pc.qinv_neg), pc.q);
+      ra = add_mod(ra, mont_mul(x, ka[d * step], pc.q, pc.qinv_neg), pc.q);
+    }
+  }
+  const uint64_t o = (uint64_t)poly * m.out_poly_stride + (uint64_t)tower * m.n + i;
+  m.out_b[o] = rb;
+  m.out_a[o] = ra;
+}
+
+hipError_t launch_mac_fused(const MacLaunch &m, bool per_term, uint32_t polys, hipStream_t stream) {
+  dim3 grid((m.n + 255) / 256, m.towers * polys);
+  if (per_term) hipLaunchKernelGGL(mac_fused_kernel<true>, grid, dim3(256), 0, stream, m);
+  else hipLaunchKernelGGL(mac_fused_kernel<false>, grid, dim3(256), 0, stream, m);
+  return hipGetLastError();
+}
+
+// Convert canonical rows to Montgomery form (x * 2^64 mod q) in place.
+__global__ __launch_bounds__(256) void to_mont_kernel(uint64_t *data, const PrimeConst *primes,
+                                                      const uint32_t *tower_prime, uint32_t n, uint32_t towers) {
+  const uint64_t idx = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (uint64_t)n * towers) return;
+  const PrimeConst &pc = primes[tower_prime[idx / n]];
+  data[idx] = mont_mul(data[idx], pc.r2, pc.q, pc.qinv_neg);
+}
+
+hipError_t launch_to_mont(uint64_t *data, const PrimeConst *primes, const uint32_t *tower_prime, uint32_t n,
+                          uint32_t towers, hipStream_t stream) {
+  const uint64_t total = (uint64_t)n * towers;
+  hipLaunchKernelGGL(to_mont_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, stream, data, primes,
+                     tower_prime, n, towers);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- ModDown correction
+// out = (left - right) * P^-1 mod q (reference resident_conversion.hpp:45-49,
+// resident.cpp:404-430). scale[t] = {P^-1 mod q_t, Shoup}.
+__global__ __launch_bounds__(256) void correct_kernel(CorrectLaunch c) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t poly = blockIdx.y / c.towers, tower = blockIdx.y - poly * c.towers;
+  if (i >= c.n) return;
+  const uint64_t q = c.primes[c.first_prime + tower].q;
+  const uint64_t o = (uint64_t)tower * c.n + i;
+  const uint64_t l = c.left[(uint64_t)poly * c.left_poly_stride + o];
+  const uint64_t r = c.right[(uint64_t)poly * c.right_poly_stride + o];
+  c.out[(uint64_t)poly * c.out_poly_stride + o] = shoup(sub_mod(l, r, q), c.scale[tower].x, c.scale[tower].y, q);
+}
+
+hipError_t launch_correct(const CorrectLaunch &c, uint32_t polys, hipStream_t stream) {
+  hipLaunchKernelGGL(correct_kernel, dim3((c.n + 255) / 256, c.towers * polys), dim3(256), 0, stream, c);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- pointwise
+__global__ __launch_bounds__(256) void pointwise_kernel(uint64_t *out, const uint64_t *right, const PrimeConst *primes,
+                                                        uint32_t first_prime, uint32_t n, uint32_t towers) {
+  const uint64_t idx = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (uint64_t)n * towers) return;
+  const PrimeConst &pc = primes[first_prime + idx / n];
+  out[idx] = mont_mul(mont_mul(out[idx], right[idx], pc.q, pc.qinv_neg), pc.r2, pc.q, pc.qinv_neg);
+}
+
+hipError_t launch_pointwise(uint64_t *out, const uint64_t *right, const PrimeConst *primes, uint32_t first_prime,
+                            uint32_t n, uint32_t towers, hipStream_t stream) {
+  const uint64_t total = (uint64_t)n * towers;
+  hipLaunchKernelGGL(pointwise_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, stream, out, right, primes,
+                     first_prime, n, towers);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- automorphism
+// Reference resident_adapter.inc:44-59 / resident_conversion.hpp:15-29:
+// out_b[t][k] = ext_b[t][src] + P*c0[t][src] on Q towers, out_a[t][k] = ext_a[t][src].
+__device__ __forceinline__ uint32_t brev(uint32_t x, uint32_t lg) { return __brev(x) >> (32 - lg); }
+
+__global__ __launch_bounds__(256) void rotate_kernel(RotateLaunch r) {
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t tower = blockIdx.y;
+  if (k >= r.n) return;
+  const uint32_t odd = 2 * brev(k, r.log_n) + 1;
+  const uint32_t src_exp = (uint32_t)(((uint64_t)odd * r.automorphism) & (2ull * r.n - 1));
+  const uint32_t src = brev(src_exp >> 1, r.log_n);
+  const uint64_t at = (uint64_t)tower * r.n;
+  uint64_t b = r.ext_b[at + src];
+  if (tower < r.active_q) {
+    const uint64_t q = r.primes[r.first_prime + tower].q;
+    b = add_mod(b, shoup(r.c0[at + src], r.scale[tower].x, r.scale[tower].y, q), q);
+  }
+  r.out_b[at + k] = b;
+  r.out_a[at + k] = r.ext_a[at + src];
+}
+
+hipError_t launch_rotate(const RotateLaunch &r, uint32_t towers, hipStream_t stream) {
+  hipLaunchKernelGGL(rotate_kernel, dim3((r.n + 255) / 256, towers), dim3(256), 0, stream, r);
+  return hipGetLastError();
+}
+
+}  // namespace nttamd

@@ -1,0 +1,325 @@
+// Negacyclic NTT over 64-bit RNS towers for gfx950.
+//
+// Semantics (reference include/ntt/ntt.hpp:5-35, transform_stage.hpp:19-80):
+// forward maps natural-order coefficients to evaluations at psi^(2 rev(k)+1)
+// in bit-reversed order (Cooley-Tukey with T[m+g] = psi^bitrev(m+g)); inverse
+// is the Gentleman-Sande mirror with psi^-1 and a final n^-1 scale.
+//
+// Schedule (our own, gfx950-first): N = R x 256 with R = N/256.
+//   pass A: the log2(R) far stages, on columns (stride 256). A workgroup
+//           owns 16 adjacent columns so every global access is a 128-byte
+//           segment.
+//   pass B: the 8 near stages, on 256-element rows; a workgroup owns 16 rows.
+// In both passes a thread keeps 16 residues in registers and runs 4 radix-2
+// stages on them, then trades through LDS for the other 4 ("P1" layout:
+// j = u + 16k, "P2" layout: j = 16u + k, u = lane within the sub-transform).
+// Butterflies are Harvey-lazy (values in [0, 4q), q < 2^62) with Shoup
+// twiddles {W, floor(W 2^64/q)} read as one 16-byte load; a canonical variant
+// serves 2^62 <= q < 2^63. Every kernel output is canonical except the
+// internal hand-off between the two passes of one transform.
+//
+// Pass A for R < 256 (N = 2^9..2^15) and whole transforms for N <= 2^12 use
+// a plain LDS-staged kernel (one radix-2 stage per barrier).
+#include "ntt_device.hpp"
+
+namespace nttamd {
+
+constexpr int kRow = 256;          // pass-B sub-transform length
+constexpr int kLdsRow = 256 + 16 + 1;  // padded LDS row (one word per 16, plus one)
+
+__device__ __forceinline__ int lds_idx(int row, int j) { return row * kLdsRow + j + (j >> 4); }
+
+template <bool Lazy>
+__device__ __forceinline__ void ct_bfly(uint64_t &x, uint64_t &y, ulonglong2 w, uint64_t q, uint64_t two_q) {
+  if constexpr (Lazy) {
+    x = x >= two_q ? x - two_q : x;
+    const uint64_t t = shoup_lazy(y, w.x, w.y, q);
+    y = x - t + two_q;
+    x = x + t;
+  } else {
+    const uint64_t t = shoup(y, w.x, w.y, q);
+    y = sub_mod(x, t, q);
+    x = add_mod(x, t, q);
+  }
+}
+
+template <bool Lazy>
+__device__ __forceinline__ void gs_bfly(uint64_t &x, uint64_t &y, ulonglong2 w, uint64_t q, uint64_t two_q) {
+  if constexpr (Lazy) {
+    uint64_t s = x + y;
+    s = s >= two_q ? s - two_q : s;
+    y = shoup_lazy(x - y + two_q, w.x, w.y, q);
+    x = s;
+  } else {
+    const uint64_t s = add_mod(x, y, q);
+    y = shoup(sub_mod(x, y, q), w.x, w.y, q);
+    x = s;
+  }
+}
+
+// Final inverse stage with n^-1 folded in: x' = (x+y) n^-1, y' = (x-y) W n^-1.
+template <bool Lazy>
+__device__ __forceinline__ void gs_bfly_last(uint64_t &x, uint64_t &y, const PrimeConst &pc) {
+  if constexpr (Lazy) {
+    const uint64_t s = x + y;
+    const uint64_t d = x - y + pc.two_q;
+    x = reduce_once(shoup_lazy(s, pc.ninv, pc.ninv_shoup, pc.q), pc.q);
+    y = reduce_once(shoup_lazy(d, pc.last_w, pc.last_wp, pc.q), pc.q);
+  } else {
+    const uint64_t s = add_mod(x, y, pc.q);
+    const uint64_t d = sub_mod(x, y, pc.q);
+    x = shoup(s, pc.ninv, pc.ninv_shoup, pc.q);
+    y = shoup(d, pc.last_w, pc.last_wp, pc.q);
+  }
+}
+
+template <bool Lazy>
+__device__ __forceinline__ uint64_t canon(uint64_t x, uint64_t q, uint64_t two_q) {
+  if constexpr (Lazy) x = x >= two_q ? x - two_q : x;
+  return x >= q ? x - q : x;
+}
+
+// Four register stages on v[0..15]. H runs 8,4,2,1 (forward) or 1,2,4,8
+// (inverse). Twiddle index = ml * row_factor + group, ml = (8/h) * ml_scale,
+// group = g_base * (8/h) + k / (2h).
+template <bool Inverse, bool Lazy>
+__device__ __forceinline__ void reg_stages(uint64_t (&v)[16], const ulonglong2 *__restrict__ tw,
+                                           uint32_t ml_scale, uint32_t row_factor, uint32_t g_base,
+                                           uint64_t q, uint64_t two_q) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int h = Inverse ? (1 << s) : (8 >> s);
+    const uint32_t ml = (8u / h) * ml_scale;
+    const uint32_t gb = g_base * (8u / h);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (k & h) continue;
+      const uint32_t idx = ml * row_factor + gb + (uint32_t)(k / (2 * h));
+      const ulonglong2 w = tw[idx];
+      if constexpr (Inverse) gs_bfly<Lazy>(v[k], v[k + h], w, q, two_q);
+      else ct_bfly<Lazy>(v[k], v[k + h], w, q, two_q);
+    }
+  }
+}
+
+// Resolve tower y of a launch: destination, source (== destination when in
+// place or when this is not the first pass) and prime.
+__device__ __forceinline__ void tower_of(const TowerMap &m, uint32_t y, bool first_pass, uint64_t *&data,
+                                         const uint64_t *&src, uint32_t &prime) {
+  if (m.jobs) {
+    const TowerJob j = m.jobs[y];
+    data = m.base + j.offset;
+    src = (first_pass && m.src) ? m.src + j.src_offset : data;
+    prime = j.prime;
+  } else {
+    const uint32_t poly = y / m.towers_per_poly, t = y - poly * m.towers_per_poly;
+    data = m.base + (uint64_t)poly * m.poly_stride + (uint64_t)t * m.tower_stride;
+    src = (first_pass && m.src) ? m.src + (uint64_t)poly * m.src_poly_stride + (uint64_t)t * m.tower_stride : data;
+    prime = m.prime_offset + t;
+  }
+}
+
+// ---------------------------------------------------------------- pass A, R = 256
+template <bool Inverse, bool Lazy>
+__global__ __launch_bounds__(256) void ntt_pass_a256(TowerMap map, const PrimeConst *__restrict__ primes) {
+  __shared__ uint64_t lds[16 * kLdsRow];
+  uint64_t *data;
+  const uint64_t *src;
+  uint32_t prime;
+  tower_of(map, blockIdx.y, !Inverse, data, src, prime);
+  const PrimeConst pc = primes[prime];
+  const ulonglong2 *tw = Inverse ? pc.inv : pc.fwd;
+  const int t = threadIdx.x, cl = t & 15, u = t >> 4;
+  const uint32_t c = blockIdx.x * 16 + cl;  // column
+  uint64_t v[16];
+  if constexpr (!Inverse) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = src[(uint64_t)(u + 16 * k) * kRow + c];
+    reg_stages<false, Lazy>(v, tw, 1, 1, 0, pc.q, pc.two_q);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) lds[lds_idx(cl, u + 16 * k)] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = lds[lds_idx(cl, 16 * u + k)];
+    reg_stages<false, Lazy>(v, tw, 16, 1, u, pc.q, pc.two_q);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) data[(uint64_t)(16 * u + k) * kRow + c] = v[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = data[(uint64_t)(16 * u + k) * kRow + c];
+    reg_stages<true, Lazy>(v, tw, 16, 1, u, pc.q, pc.two_q);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) lds[lds_idx(cl, 16 * u + k)] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = lds[lds_idx(cl, u + 16 * k)];
+    // stages h = 1, 2, 4 (ml = 8, 4, 2), then the scaled last stage (ml = 1)
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      const int h = 1 << s;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        if (k & h) continue;
+        const ulonglong2 w = tw[(8u / h) + (uint32_t)(k / (2 * h))];
+        gs_bfly<Lazy>(v[k], v[k + h], w, pc.q, pc.two_q);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) gs_bfly_last<Lazy>(v[k], v[k + 8], pc);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) data[(uint64_t)(u + 16 * k) * kRow + c] = v[k];
+  }
+}
+
+// ---------------------------------------------------------------- pass B (rows of 256)
+// R = rows in the tower (N / 256); row_factor for ml is R + r.
+template <bool Inverse, bool Lazy>
+__global__ __launch_bounds__(256) void ntt_pass_b(TowerMap map, const PrimeConst *__restrict__ primes, uint32_t R) {
+  __shared__ uint64_t lds[16 * kLdsRow];
+  uint64_t *data;
+  const uint64_t *src;
+  uint32_t prime;
+  tower_of(map, blockIdx.y, Inverse, data, src, prime);
+  const PrimeConst pc = primes[prime];
+  const ulonglong2 *tw = Inverse ? pc.inv : pc.fwd;
+  const int t = threadIdx.x, u = t & 15, rl = t >> 4;
+  const uint32_t r = blockIdx.x * 16 + rl;
+  uint64_t *row = data + (uint64_t)r * kRow;
+  const uint64_t *src_row = src + (uint64_t)r * kRow;
+  const uint32_t rf = R + r;
+  uint64_t v[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = src_row[u + 16 * k];
+  if constexpr (!Inverse) {
+    reg_stages<false, Lazy>(v, tw, 1, rf, 0, pc.q, pc.two_q);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) lds[lds_idx(rl, u + 16 * k)] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = lds[lds_idx(rl, 16 * u + k)];
+    reg_stages<false, Lazy>(v, tw, 16, rf, u, pc.q, pc.two_q);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = canon<Lazy>(v[k], pc.q, pc.two_q);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) lds[lds_idx(rl, 16 * u + k)] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = lds[lds_idx(rl, u + 16 * k)];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) lds[lds_idx(rl, u + 16 * k)] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = lds[lds_idx(rl, 16 * u + k)];
+    reg_stages<true, Lazy>(v, tw, 16, rf, u, pc.q, pc.two_q);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) lds[lds_idx(rl, 16 * u + k)] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = lds[lds_idx(rl, u + 16 * k)];
+    reg_stages<true, Lazy>(v, tw, 1, rf, 0, pc.q, pc.two_q);
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) row[u + 16 * k] = v[k];
+}
+
+// ---------------------------------------------------------------- generic LDS kernel
+// Runs `stages` radix-2 stages on `sub` independent sub-transforms of length
+// L = 2^log_len held in LDS: element j of sub-transform b is at
+// data[b_origin(b) + j * stride]. Used for whole small transforms
+// (stride 1, one sub-transform) and for pass A when R < 256 (stride 256,
+// 4096/R columns per workgroup). Forward runs ml = 1..L/2, inverse the mirror
+// with n^-1 folded into the last stage when `last` is set.
+template <bool Inverse, bool Lazy>
+__global__ __launch_bounds__(256) void ntt_lds_generic(TowerMap map, const PrimeConst *__restrict__ primes,
+                                                       uint32_t log_len, uint32_t stride, uint32_t cols_per_block,
+                                                       uint32_t finish, uint32_t first_pass) {
+  __shared__ uint64_t lds[4096];
+  uint64_t *data;
+  const uint64_t *src;
+  uint32_t prime;
+  tower_of(map, blockIdx.y, first_pass != 0, data, src, prime);
+  const PrimeConst pc = primes[prime];
+  const ulonglong2 *tw = Inverse ? pc.inv : pc.fwd;
+  const uint32_t L = 1u << log_len;
+  const uint32_t total = L * cols_per_block;  // <= 4096
+  const uint32_t col0 = blockIdx.x * cols_per_block;
+  // LDS layout: lds[col * L + j]; global: data[(col0 + col) + j * stride] (stride > 1)
+  // or data[j] (stride == 1, cols_per_block == 1).
+  for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
+    const uint32_t col = e % cols_per_block, j = e / cols_per_block;
+    lds[col * L + j] = src[(uint64_t)j * stride + col0 + col];
+  }
+  __syncthreads();
+  const uint32_t half = total / 2;
+  for (uint32_t s = 0; s < log_len; ++s) {
+    const uint32_t lg_d = Inverse ? s : log_len - 1 - s;  // distance 2^lg_d
+    const uint32_t ml = L >> (lg_d + 1);
+    const bool last = Inverse && finish && s == log_len - 1;
+    for (uint32_t p = threadIdx.x; p < half; p += blockDim.x) {
+      const uint32_t col = p / (L / 2), pp = p % (L / 2);
+      const uint32_t g = pp >> lg_d;
+      const uint32_t j = pp + (g << lg_d);
+      uint64_t x = lds[col * L + j], y = lds[col * L + j + (1u << lg_d)];
+      if constexpr (Inverse) {
+        if (last) gs_bfly_last<Lazy>(x, y, pc);
+        else gs_bfly<Lazy>(x, y, tw[ml + g], pc.q, pc.two_q);
+      } else {
+        ct_bfly<Lazy>(x, y, tw[ml + g], pc.q, pc.two_q);
+      }
+      lds[col * L + j] = x;
+      lds[col * L + j + (1u << lg_d)] = y;
+    }
+    __syncthreads();
+  }
+  for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
+    const uint32_t col = e % cols_per_block, j = e / cols_per_block;
+    uint64_t x = lds[col * L + j];
+    if (!Inverse && finish) x = canon<Lazy>(x, pc.q, pc.two_q);
+    data[(uint64_t)j * stride + col0 + col] = x;
+  }
+}
+
+// ---------------------------------------------------------------- launcher
+template <bool Inverse, bool Lazy>
+static hipError_t launch_t(const TowerMap &map, const PrimeConst *primes, uint32_t log_n, uint32_t towers,
+                           hipStream_t stream) {
+  const uint32_t n = 1u << log_n;
+  if (log_n <= 12) {
+    hipLaunchKernelGGL((ntt_lds_generic<Inverse, Lazy>), dim3(1, towers), dim3(256), 0, stream, map, primes, log_n,
+                       1u, 1u, 1u, 1u);
+    return hipGetLastError();
+  }
+  const uint32_t R = n / kRow, log_r = log_n - 8;
+  auto pass_a = [&]() {
+    if (R == 256) {
+      hipLaunchKernelGGL((ntt_pass_a256<Inverse, Lazy>), dim3(kRow / 16, towers), dim3(256), 0, stream, map, primes);
+    } else {
+      const uint32_t cols = 4096 / R;
+      hipLaunchKernelGGL((ntt_lds_generic<Inverse, Lazy>), dim3(kRow / cols, towers), dim3(256), 0, stream, map,
+                         primes, log_r, (uint32_t)kRow, cols, (uint32_t)Inverse, (uint32_t)!Inverse);
+    }
+  };
+  auto pass_b = [&]() {
+    hipLaunchKernelGGL((ntt_pass_b<Inverse, Lazy>), dim3(R / 16, towers), dim3(256), 0, stream, map, primes, R);
+  };
+  if (!Inverse) {
+    pass_a();
+    pass_b();
+  } else {
+    pass_b();
+    pass_a();
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_ntt(const TowerMap &map, const PrimeConst *primes, uint32_t log_n, uint32_t towers, bool inverse,
+                      bool lazy, hipStream_t stream) {
+  if (towers == 0) return hipSuccess;
+  if (inverse) return lazy ? launch_t<true, true>(map, primes, log_n, towers, stream)
+                           : launch_t<true, false>(map, primes, log_n, towers, stream);
+  return lazy ? launch_t<false, true>(map, primes, log_n, towers, stream)
+              : launch_t<false, false>(map, primes, log_n, towers, stream);
+}
+
+}  // namespace nttamd

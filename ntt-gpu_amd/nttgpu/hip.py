@@ -1,0 +1,342 @@
+"""ctypes binding of libntt_hip.so (include/ntt_hip.h).
+
+The product path: every call goes to the hand-written gfx950 kernels. There is
+no CPU fallback — if the library is missing or no GPU is visible, calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libntt_hip.so")
+
+OK, NOT_READY, EINVAL, ERUNTIME = 0, 1, -1, -2
+H2D, D2H, D2D = 1, 2, 3
+
+U64 = ctypes.c_uint64
+U64P = ctypes.POINTER(ctypes.c_uint64)
+SZ = ctypes.c_size_t
+SZP = ctypes.POINTER(ctypes.c_size_t)
+VP = ctypes.c_void_p
+INT = ctypes.c_int
+
+# name -> argtypes (restype int unless listed in _RESTYPES)
+SIGNATURES = {
+    "ntt_hip_last_error": [],
+    "ntt_hip_abi_version": [],
+    "ntt_hip_device_count": [ctypes.POINTER(INT)],
+    "ntt_hip_device_name": [INT, ctypes.c_char_p, SZ],
+    "ntt_hip_malloc": [INT, ctypes.POINTER(VP), SZ],
+    "ntt_hip_free": [INT, VP],
+    "ntt_hip_host_alloc": [ctypes.POINTER(VP), SZ],
+    "ntt_hip_host_free": [VP],
+    "ntt_hip_memcpy": [INT, VP, VP, SZ, INT, VP],
+    "ntt_hip_stream_create": [INT, ctypes.POINTER(VP)],
+    "ntt_hip_stream_destroy": [INT, VP],
+    "ntt_hip_stream_synchronize": [INT, VP],
+    "ntt_hip_event_create": [INT, ctypes.POINTER(VP), INT],
+    "ntt_hip_event_destroy": [INT, VP],
+    "ntt_hip_event_record": [INT, VP, VP],
+    "ntt_hip_event_query": [INT, VP],
+    "ntt_hip_event_synchronize": [INT, VP],
+    "ntt_hip_event_elapsed": [ctypes.POINTER(ctypes.c_float), VP, VP],
+    "ntt_hip_context_create": [ctypes.POINTER(VP), INT, U64, U64P, U64P, SZ],
+    "ntt_hip_context_destroy": [VP],
+    "ntt_hip_context_stream": [VP],
+    "ntt_hip_context_device": [VP],
+    "ntt_hip_context_table_bytes": [VP],
+    "ntt_hip_transform": [VP, VP, VP, SZ, SZ, INT, VP],
+    "ntt_hip_transform_batch": [VP, VP, VP, SZ, SZ, SZ, SZ, SZ, INT, VP],
+    "ntt_hip_pointwise": [VP, VP, VP, SZ, SZ, VP],
+    "ntt_hip_switch_basis": [VP, VP, VP, SZP, SZ, SZP, SZ, VP],
+    "ntt_hip_correct": [VP, VP, VP, VP, SZ, SZ, SZP, SZ, VP],
+    "ntt_hip_hybrid_mac": [VP, VP, VP, VP, VP, VP, SZ, SZ, INT, VP],
+    "ntt_hip_hybrid_rotate": [VP, VP, VP, VP, VP, VP, SZ, SZ, SZ, SZP, SZ, ctypes.c_uint32, VP],
+    "ntt_hip_keyswitch_create": [ctypes.POINTER(VP), VP, SZ, SZ, SZ, SZ],
+    "ntt_hip_keyswitch_destroy": [VP],
+    "ntt_hip_keyswitch_upload_key": [VP, U64P, U64P, SZ, U64P],
+    "ntt_hip_keyswitch_workspace_bytes": [VP, SZ, SZP],
+    "ntt_hip_keyswitch_run": [VP, VP, SZ, VP, VP, VP, VP],
+    "ntt_hip_keyswitch_capture": [VP, VP, SZ, VP, VP, VP, VP, ctypes.POINTER(VP)],
+    "ntt_hip_graph_launch": [VP, VP],
+    "ntt_hip_graph_destroy": [VP],
+}
+_RESTYPES = {
+    "ntt_hip_last_error": ctypes.c_char_p,
+    "ntt_hip_context_stream": VP,
+    "ntt_hip_context_table_bytes": SZ,
+}
+
+_lib = None
+
+
+class NttHipError(RuntimeError):
+    pass
+
+
+class ContractError(NttHipError, ValueError):
+    """Raised for NTT_HIP_EINVAL: a contract violation caught before device work."""
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NttHipError(f"HIP extension not built: {LIB_PATH} (run __graft_entry__.build())")
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPES.get(name, INT)
+        _lib = handle
+    return _lib
+
+
+def check(status, what=""):
+    if status in (OK, NOT_READY):
+        return status
+    msg = lib().ntt_hip_last_error().decode(errors="replace")
+    if status == EINVAL:
+        raise ContractError(f"{what}: {msg}")
+    raise NttHipError(f"{what}: {msg}")
+
+
+def device_count():
+    n = INT(0)
+    check(lib().ntt_hip_device_count(ctypes.byref(n)), "device_count")
+    return n.value
+
+
+def device_name(device=0):
+    buf = ctypes.create_string_buffer(256)
+    check(lib().ntt_hip_device_name(device, buf, 256), "device_name")
+    return buf.value.decode()
+
+
+def _u64(a):
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    return a, a.ctypes.data_as(U64P)
+
+
+def _sizes(v):
+    arr = (ctypes.c_size_t * len(v))(*[int(x) for x in v])
+    return arr
+
+
+class Stream:
+    def __init__(self, device=0, handle=None):
+        self.device = device
+        self.owned = handle is None
+        if handle is None:
+            h = VP()
+            check(lib().ntt_hip_stream_create(device, ctypes.byref(h)), "stream_create")
+            handle = h.value
+        self.handle = handle
+
+    def synchronize(self):
+        check(lib().ntt_hip_stream_synchronize(self.device, self.handle), "stream_synchronize")
+
+    def close(self):
+        if self.owned and self.handle:
+            check(lib().ntt_hip_stream_destroy(self.device, self.handle), "stream_destroy")
+            self.handle = None
+
+
+class Event:
+    def __init__(self, device=0, timing=True):
+        self.device = device
+        h = VP()
+        check(lib().ntt_hip_event_create(device, ctypes.byref(h), int(timing)), "event_create")
+        self.handle = h.value
+
+    def record(self, stream):
+        check(lib().ntt_hip_event_record(self.device, self.handle, stream), "event_record")
+
+    def synchronize(self):
+        check(lib().ntt_hip_event_synchronize(self.device, self.handle), "event_synchronize")
+
+    def query(self):
+        return check(lib().ntt_hip_event_query(self.device, self.handle), "event_query") == OK
+
+    def elapsed_ms(self, later):
+        ms = ctypes.c_float(0)
+        check(lib().ntt_hip_event_elapsed(ctypes.byref(ms), self.handle, later.handle), "event_elapsed")
+        return ms.value
+
+    def close(self):
+        if self.handle:
+            check(lib().ntt_hip_event_destroy(self.device, self.handle), "event_destroy")
+            self.handle = None
+
+
+class DeviceBuffer:
+    """Raw device allocation of uint64 words."""
+
+    def __init__(self, words, device=0):
+        self.device = device
+        self.words = int(words)
+        p = VP()
+        check(lib().ntt_hip_malloc(device, ctypes.byref(p), max(1, self.words) * 8), "malloc")
+        self.ptr = p.value
+
+    def at(self, word_offset=0):
+        return self.ptr + 8 * int(word_offset)
+
+    def upload(self, values, stream=None, offset=0):
+        a, _ = _u64(values)
+        assert offset + a.size <= self.words
+        check(lib().ntt_hip_memcpy(self.device, self.at(offset), a.ctypes.data, a.size * 8, H2D, stream), "upload")
+        if stream is not None:
+            Stream(self.device, stream).synchronize()
+        else:
+            check(lib().ntt_hip_stream_synchronize(self.device, None), "sync")
+        return self
+
+    def download(self, count=None, offset=0, stream=None):
+        count = self.words - offset if count is None else int(count)
+        out = np.empty(count, dtype=np.uint64)
+        check(lib().ntt_hip_memcpy(self.device, out.ctypes.data, self.at(offset), count * 8, D2H, stream), "download")
+        if stream is not None:
+            Stream(self.device, stream).synchronize()
+        else:
+            check(lib().ntt_hip_stream_synchronize(self.device, None), "sync")
+        return out
+
+    def free(self):
+        if self.ptr:
+            check(lib().ntt_hip_free(self.device, self.ptr), "free")
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """A fixed basis of towers (q, psi) at ring dimension n with device tables."""
+
+    def __init__(self, n, q, psi, device=0):
+        self.n, self.q, self.psi, self.device = int(n), [int(x) for x in q], [int(x) for x in psi], device
+        qa, qp = _u64(self.q)
+        pa, pp = _u64(self.psi)
+        h = VP()
+        check(lib().ntt_hip_context_create(ctypes.byref(h), device, self.n, qp, pp, len(self.q)), "context_create")
+        self.handle = h.value
+        self.stream = lib().ntt_hip_context_stream(self.handle)
+
+    @property
+    def towers(self):
+        return len(self.q)
+
+    def table_bytes(self):
+        return lib().ntt_hip_context_table_bytes(self.handle)
+
+    def synchronize(self, stream=None):
+        check(lib().ntt_hip_stream_synchronize(self.device, stream or self.stream), "sync")
+
+    def transform(self, data_ptr, first, towers, inverse, src_ptr=None, stream=None):
+        check(lib().ntt_hip_transform(self.handle, data_ptr, src_ptr, first, towers, int(inverse), stream),
+              "transform")
+
+    def transform_batch(self, data_ptr, polys, poly_stride, first, towers, inverse, src_ptr=None,
+                        src_poly_stride=0, stream=None):
+        check(lib().ntt_hip_transform_batch(self.handle, data_ptr, src_ptr, polys, poly_stride, src_poly_stride,
+                                            first, towers, int(inverse), stream), "transform_batch")
+
+    def pointwise(self, out_ptr, right_ptr, first, towers, stream=None):
+        check(lib().ntt_hip_pointwise(self.handle, out_ptr, right_ptr, first, towers, stream), "pointwise")
+
+    def switch_basis(self, out_ptr, in_ptr, src, dst, stream=None):
+        check(lib().ntt_hip_switch_basis(self.handle, out_ptr, in_ptr, _sizes(src), len(src), _sizes(dst),
+                                         len(dst), stream), "switch_basis")
+
+    def correct(self, out_ptr, left_ptr, right_ptr, first, towers, p, stream=None):
+        check(lib().ntt_hip_correct(self.handle, out_ptr, left_ptr, right_ptr, first, towers, _sizes(p), len(p),
+                                    stream), "correct")
+
+    def hybrid_mac(self, b_ptr, a_ptr, digit_ptr, kb_ptr, ka_ptr, first, towers, first_digit, stream=None):
+        check(lib().ntt_hip_hybrid_mac(self.handle, b_ptr, a_ptr, digit_ptr, kb_ptr, ka_ptr, first, towers,
+                                       int(first_digit), stream), "hybrid_mac")
+
+    def hybrid_rotate(self, ob_ptr, oa_ptr, eb_ptr, ea_ptr, c0_ptr, first, towers, active_q, p, automorphism,
+                      stream=None):
+        check(lib().ntt_hip_hybrid_rotate(self.handle, ob_ptr, oa_ptr, eb_ptr, ea_ptr, c0_ptr, first, towers,
+                                          active_q, _sizes(p), len(p), automorphism, stream), "hybrid_rotate")
+
+    def close(self):
+        if getattr(self, "handle", None):
+            check(lib().ntt_hip_context_destroy(self.handle), "context_destroy")
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """An instantiated HIP graph (captured key-switch batch)."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    def launch(self, stream):
+        check(lib().ntt_hip_graph_launch(self.handle, stream), "graph_launch")
+
+    def close(self):
+        if self.handle:
+            check(lib().ntt_hip_graph_destroy(self.handle), "graph_destroy")
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class KeySwitch:
+    """Fused batched Hybrid key switch on a context whose basis is Q (L) then P (K)."""
+
+    def __init__(self, ctx, L, K, alpha, max_batch=1):
+        self.ctx, self.L, self.K, self.alpha, self.max_batch = ctx, L, K, alpha, max_batch
+        self.beta = (L + alpha - 1) // alpha
+        h = VP()
+        check(lib().ntt_hip_keyswitch_create(ctypes.byref(h), ctx.handle, L, K, alpha, max_batch), "keyswitch_create")
+        self.handle = h.value
+
+    def upload_key(self, key_b, key_a, M, q_full):
+        b, bp = _u64(key_b)
+        a, ap = _u64(key_a)
+        qf, qp = _u64(q_full)
+        check(lib().ntt_hip_keyswitch_upload_key(self.handle, bp, ap, M, qp), "keyswitch_upload_key")
+
+    def workspace_words(self, batch):
+        out = SZ(0)
+        check(lib().ntt_hip_keyswitch_workspace_bytes(self.handle, batch, ctypes.byref(out)), "workspace_bytes")
+        return out.value // 8
+
+    def run(self, in_ptr, batch, out_b_ptr, out_a_ptr, workspace_ptr, stream=None):
+        check(lib().ntt_hip_keyswitch_run(self.handle, in_ptr, batch, out_b_ptr, out_a_ptr, workspace_ptr, stream),
+              "keyswitch_run")
+
+    def capture(self, in_ptr, batch, out_b_ptr, out_a_ptr, workspace_ptr, stream):
+        g = VP()
+        check(lib().ntt_hip_keyswitch_capture(self.handle, in_ptr, batch, out_b_ptr, out_a_ptr, workspace_ptr,
+                                              stream, ctypes.byref(g)), "keyswitch_capture")
+        return Graph(g.value)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            check(lib().ntt_hip_keyswitch_destroy(self.handle), "keyswitch_destroy")
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,194 @@
+"""GPU parity of basis conversion, ModDown correction, key inner product,
+automorphism and the fused batched Hybrid key switch against the CPU oracle.
+
+Bit-exact on full arrays: the Driver-seam operations one by one
+(src/cuda/resident_adapter.inc:204-280), then the whole KeySwitchHybrid core
+(include/ntt/openfhe_hybrid_adapter.hpp:163-181) at the north-star shape
+N=2^16, L=24, K=6, dnum=4 and at reduced levels (key offset i + (M - L)).
+"""
+import numpy as np
+import pytest
+
+import oracle as o
+from common import log_case, uniform
+from nttgpu import hip, params
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(x):
+    return hip.DeviceBuffer(x.size).upload(x)
+
+
+@pytest.fixture(scope="module")
+def small():
+    hp = params.openfhe_like(1024, 6, 2, 2)
+    ctx = hip.Context(hp.n, hp.q + hp.p, hp.q_psi + hp.p_psi)
+    return hp, ctx
+
+
+@pytest.mark.parametrize("src,dst", [([0, 1], [2, 3, 4, 5, 6, 7]), ([6, 7], [0, 1, 2, 3, 4, 5]),
+                                     ([2, 3, 4], [0, 1, 6]), ([5], [0])])
+def test_switch_basis(small, src, dst):
+    hp, ctx = small
+    mods = hp.q + hp.p
+    x = uniform(hp.n, [mods[i] for i in src], 7)
+    log_case(f"switch_basis {src}->{dst}")
+    out = hip.DeviceBuffer(hp.n * len(dst))
+    ctx.switch_basis(out.ptr, _dev(x).ptr, src, dst)
+    ctx.synchronize()
+    want = o.basis_switch(x, hp.n, [mods[i] for i in src], [mods[i] for i in dst])
+    assert np.array_equal(out.download(), want)
+
+
+def test_switch_basis_per_term_path():
+    """Four 63-bit sources: sum b_i >= 2^64 selects per-term reduction."""
+    n = 256
+    qs = params.find_primes(n, 6, 63)
+    ctx = hip.Context(n, qs, [params.find_psi(q, n) for q in qs])
+    x = np.concatenate([np.full(n, q - 1, np.uint64) for q in qs[:4]])
+    x[::7] = 0
+    log_case("switch_basis per-term 63-bit")
+    out = hip.DeviceBuffer(2 * n)
+    ctx.switch_basis(out.ptr, _dev(x).ptr, [0, 1, 2, 3], [4, 5])
+    ctx.synchronize()
+    assert np.array_equal(out.download(), o.basis_switch(x, n, qs[:4], qs[4:]))
+
+
+def test_switch_basis_rejections(small):
+    hp, ctx = small
+    buf = hip.DeviceBuffer(hp.n * 2)
+    for src, dst in (([0, 0], [1]), ([0], [0]), ([], [1]), ([0], []), ([0], [99])):
+        with pytest.raises(hip.ContractError):
+            ctx.switch_basis(buf.ptr, buf.ptr, src, dst)
+
+
+def test_correct_mac_rotate_pointwise(small):
+    hp, ctx = small
+    n, L, K = hp.n, hp.M, hp.K
+    mods = hp.q + hp.p
+    T = L + K
+    left = uniform(n, hp.q, 1)
+    right = uniform(n, hp.q, 2)
+    log_case("correct")
+    out = hip.DeviceBuffer(L * n)
+    ctx.correct(out.ptr, _dev(left).ptr, _dev(right).ptr, 0, L, [L + j for j in range(K)])
+    ctx.synchronize()
+    want = np.empty_like(left)
+    for l, q in enumerate(hp.q):
+        pinv = o.inv_mod(o.special_product(hp.p, q), q)
+        sl = slice(l * n, (l + 1) * n)
+        want[sl] = [o.mul_mod(o.sub_mod(int(a), int(b), q), pinv, q) for a, b in zip(left[sl], right[sl])]
+    assert np.array_equal(out.download(), want)
+
+    log_case("hybrid_mac per digit")
+    beta = 3
+    digits = [uniform(n, mods, 10 + d) for d in range(beta)]
+    kb = [uniform(n, mods, 20 + d) for d in range(beta)]
+    ka = [uniform(n, mods, 30 + d) for d in range(beta)]
+    b, a = hip.DeviceBuffer(T * n), hip.DeviceBuffer(T * n)
+    keep = []
+    for d in range(beta):
+        bufs = [_dev(digits[d]), _dev(kb[d]), _dev(ka[d])]
+        keep += bufs
+        ctx.hybrid_mac(b.ptr, a.ptr, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, 0, T, d == 0)
+    ctx.synchronize()
+    wb, wa = o.hybrid_mac(np.concatenate(digits), beta, np.concatenate(kb), np.concatenate(ka), n, hp.q, hp.p, L)
+    assert np.array_equal(b.download(), wb) and np.array_equal(a.download(), wa)
+
+    log_case("hybrid_rotate")
+    c0 = uniform(n, hp.q, 40)
+    for k in (5, 25, 2 * n - 1):
+        ob, oa = hip.DeviceBuffer(T * n), hip.DeviceBuffer(T * n)
+        ctx.hybrid_rotate(ob.ptr, oa.ptr, b.ptr, a.ptr, _dev(c0).ptr, 0, T, L, [L + j for j in range(K)], k)
+        ctx.synchronize()
+        rb, ra = o.hybrid_rotate(wb, wa, c0, n, hp.q, hp.p, k)
+        assert np.array_equal(ob.download(), rb) and np.array_equal(oa.download(), ra)
+    with pytest.raises(hip.ContractError):
+        ctx.hybrid_rotate(ob.ptr, oa.ptr, b.ptr, a.ptr, b.ptr, 0, T, L, [L], 4)
+
+    log_case("pointwise")
+    x, y = uniform(n, mods, 50), uniform(n, mods, 51)
+    bx = _dev(x)
+    ctx.pointwise(bx.ptr, _dev(y).ptr, 0, T)
+    ctx.synchronize()
+    assert np.array_equal(bx.download(), o.pointwise_towers(x, y, n, mods))
+
+
+def _key(hp, seed):
+    full = hp.q + hp.p
+    beta_full = hp.beta()
+    kb = np.concatenate([uniform(hp.n, full, seed + d) for d in range(beta_full)])
+    ka = np.concatenate([uniform(hp.n, full, seed + 100 + d) for d in range(beta_full)])
+    return kb, ka
+
+
+def _keyswitch(hp, L, batch, seed, max_batch=None):
+    """Run the fused key switch on `batch` components at active level L."""
+    qa, pa = hp.q[:L], hp.p
+    ctx = hip.Context(hp.n, qa + pa, hp.q_psi[:L] + hp.p_psi)
+    ks = hip.KeySwitch(ctx, L, hp.K, hp.alpha, max_batch or batch)
+    kb, ka = _key(hp, seed)
+    ks.upload_key(kb, ka, hp.M, hp.q)
+    x = np.concatenate([uniform(hp.n, qa, seed + 1000 + c) for c in range(batch)])
+    inp = _dev(x)
+    ob, oa = hip.DeviceBuffer(batch * L * hp.n), hip.DeviceBuffer(batch * L * hp.n)
+    ws = hip.DeviceBuffer(ks.workspace_words(batch))
+    ks.run(inp.ptr, batch, ob.ptr, oa.ptr, ws.ptr)
+    ctx.synchronize()
+    return x, kb, ka, ob.download(), oa.download()
+
+
+@pytest.mark.parametrize("L", [6, 5, 3, 1])
+def test_keyswitch_small_levels(L):
+    hp = params.openfhe_like(1024, 6, 2, 2)
+    log_case(f"keyswitch n=1024 M=6 L={L}")
+    x, kb, ka, gb, ga = _keyswitch(hp, L, 2, 500 + L)
+    n = hp.n
+    for c in range(2):
+        xs = x[c * L * n:(c + 1) * L * n]
+        wb, wa = o.key_switch(xs, n, hp.q[:L], hp.q_psi[:L], hp.M, hp.p, hp.p_psi, hp.alpha, kb, ka)
+        assert np.array_equal(gb[c * L * n:(c + 1) * L * n], wb), f"B component {c}"
+        assert np.array_equal(ga[c * L * n:(c + 1) * L * n], wa), f"A component {c}"
+
+
+def test_keyswitch_north_star_shape():
+    """N=2^16, L=24 (~50-bit, first 60-bit), K=6 (60-bit), dnum=4 (alpha=6), batch of 2."""
+    hp = params.openfhe_like(65536, 24, 6, 6)
+    log_case("keyswitch n=65536 L=24 K=6 dnum=4 batch=2")
+    x, kb, ka, gb, ga = _keyswitch(hp, 24, 2, 7000)
+    n, L = hp.n, 24
+    for c in range(2):
+        xs = x[c * L * n:(c + 1) * L * n]
+        wb, wa = o.key_switch(xs, n, hp.q, hp.q_psi, hp.M, hp.p, hp.p_psi, hp.alpha, kb, ka)
+        assert np.array_equal(gb[c * L * n:(c + 1) * L * n], wb), f"B component {c}"
+        assert np.array_equal(ga[c * L * n:(c + 1) * L * n], wa), f"A component {c}"
+
+
+def test_keyswitch_reduced_level_key_offset():
+    """L=20 of M=24: P key rows come from index i + (M - L)."""
+    hp = params.openfhe_like(65536, 24, 6, 6)
+    log_case("keyswitch n=65536 M=24 L=20")
+    x, kb, ka, gb, ga = _keyswitch(hp, 20, 1, 8100)
+    wb, wa = o.key_switch(x, hp.n, hp.q[:20], hp.q_psi[:20], hp.M, hp.p, hp.p_psi, hp.alpha, kb, ka)
+    assert np.array_equal(gb, wb) and np.array_equal(ga, wa)
+
+
+def test_keyswitch_batch_equals_alone():
+    """A component switched inside a batch of 5 equals the same component switched alone."""
+    hp = params.openfhe_like(4096, 8, 3, 3)
+    L, n = 8, 4096
+    log_case("keyswitch batch-vs-alone n=4096")
+    x, kb, ka, gb, ga = _keyswitch(hp, L, 5, 9000)
+    qa = hp.q
+    ctx = hip.Context(n, qa + hp.p, hp.q_psi + hp.p_psi)
+    ks = hip.KeySwitch(ctx, L, hp.K, hp.alpha, 1)
+    ks.upload_key(kb, ka, hp.M, hp.q)
+    ws = hip.DeviceBuffer(ks.workspace_words(1))
+    for c in range(5):
+        xs = x[c * L * n:(c + 1) * L * n]
+        ob, oa = hip.DeviceBuffer(L * n), hip.DeviceBuffer(L * n)
+        ks.run(_dev(xs).ptr, 1, ob.ptr, oa.ptr, ws.ptr)
+        ctx.synchronize()
+        assert np.array_equal(ob.download(), gb[c * L * n:(c + 1) * L * n])
+        assert np.array_equal(oa.download(), ga[c * L * n:(c + 1) * L * n])

@@ -1,0 +1,128 @@
+"""GPU parity of the gfx950 NTT and element-wise kernels against the CPU oracle.
+
+Bit-exact on full arrays (integer work): the reference's seven patterns on the
+pinned OpenFHE QP towers (tests/cuda_openfhe_tests.cpp:67-155), the north-star
+NTT configurations, every ring dimension 2..2^16, and 62-63-bit primes that
+take the canonical (non-lazy) butterfly.
+"""
+import numpy as np
+import pytest
+
+import oracle as o
+from common import PINNED, log_case, patterns, uniform, lcg_values, SEED
+from nttgpu import hip, params
+
+pytestmark = pytest.mark.gpu
+N16 = 65536
+
+
+def _gpu_round(ctx, values, first, towers, inverse):
+    buf = hip.DeviceBuffer(values.size).upload(values)
+    ctx.transform(buf.ptr, first, towers, inverse)
+    ctx.synchronize()
+    return buf.download()
+
+
+@pytest.fixture(scope="module")
+def pinned_ctx():
+    return hip.Context(N16, PINNED["q"], PINNED["psi"])
+
+
+@pytest.mark.parametrize("subset", ["Q5", "Q4P", "Q1", "QP8"])
+def test_seven_patterns_pinned_towers(pinned_ctx, subset):
+    q, psi = PINNED["q"], PINNED["psi"]
+    sel = {"Q5": list(range(5)), "Q4P": [0, 1, 2, 3, 5, 6, 7], "Q1": [0], "QP8": list(range(8))}[subset]
+    # a context per subset keeps towers contiguous, as the reference does per basis
+    ctx = pinned_ctx if subset == "QP8" else hip.Context(N16, [q[i] for i in sel], [psi[i] for i in sel])
+    qs, rs = [q[i] for i in sel], [psi[i] for i in sel]
+    T = len(sel)
+    for pattern, x in patterns(N16, qs):
+        log_case(f"ntt pinned {subset} pattern {pattern}")
+        want_f = o.ntt_towers(x, N16, qs, rs)
+        got_f = _gpu_round(ctx, x, 0, T, False)
+        assert np.array_equal(got_f, want_f), f"forward pattern {pattern}"
+        assert np.array_equal(_gpu_round(ctx, got_f, 0, T, True), x), f"round trip pattern {pattern}"
+        want_i = o.ntt_towers(x, N16, qs, rs, inverse=True)
+        assert np.array_equal(_gpu_round(ctx, x, 0, T, True), want_i), f"independent inverse pattern {pattern}"
+        # pointwise square then inverse == negacyclic ring product x*x
+        a = hip.DeviceBuffer(x.size).upload(want_f)
+        b = hip.DeviceBuffer(x.size).upload(want_f)
+        ctx.pointwise(a.ptr, b.ptr, 0, T)
+        ctx.transform(a.ptr, 0, T, True)
+        ctx.synchronize()
+        want_prod = o.ntt_towers(o.pointwise_towers(want_f, want_f, N16, qs), N16, qs, rs, inverse=True)
+        assert np.array_equal(a.download(), want_prod), f"ring product pattern {pattern}"
+
+
+def test_single_tower_config_c1():
+    """One NTT-friendly prime q ~ 2^59, uniform seeded input, forward then inverse."""
+    q = params.last_prime(59, 2 * N16)
+    psi = params.min_root(q, N16)
+    ctx = hip.Context(N16, [q], [psi])
+    x = lcg_values(N16, [q])
+    log_case("ntt c1 single tower")
+    f = _gpu_round(ctx, x, 0, 1, False)
+    assert np.array_equal(f, o.ntt_forward(x, q, psi))
+    assert np.array_equal(_gpu_round(ctx, f, 0, 1, True), x)
+
+
+def test_batched_rns_config_c2():
+    """24 distinct 50-60-bit primes, 2 polynomials = 48 tower transforms in one launch."""
+    qs = params.mixed_width_primes(N16, 24)
+    rs = [params.min_root(q, N16) for q in qs]
+    ctx = hip.Context(N16, qs, rs)
+    x = uniform(N16, qs + qs, SEED)
+    log_case("ntt c2 batched 48 towers")
+    buf = hip.DeviceBuffer(x.size).upload(x)
+    ctx.transform_batch(buf.ptr, 2, 24 * N16, 0, 24, False)
+    ctx.synchronize()
+    got = buf.download()
+    want = np.concatenate([o.ntt_towers(x[:24 * N16], N16, qs, rs), o.ntt_towers(x[24 * N16:], N16, qs, rs)])
+    assert np.array_equal(got, want)
+    ctx.transform_batch(buf.ptr, 2, 24 * N16, 0, 24, True)
+    ctx.synchronize()
+    assert np.array_equal(buf.download(), x)
+
+
+@pytest.mark.parametrize("log_n", list(range(1, 17)))
+def test_every_ring_dimension(log_n):
+    n = 1 << log_n
+    qs = params.find_primes(max(n, 2), 3, 60)
+    rs = [params.find_psi(q, n) for q in qs]
+    ctx = hip.Context(n, qs, rs)
+    x = uniform(n, qs, log_n)
+    log_case(f"ntt n=2^{log_n}")
+    f = _gpu_round(ctx, x, 0, 3, False)
+    assert np.array_equal(f, o.ntt_towers(x, n, qs, rs))
+    assert np.array_equal(_gpu_round(ctx, f, 0, 3, True), x)
+    assert np.array_equal(_gpu_round(ctx, x, 0, 3, True), o.ntt_towers(x, n, qs, rs, inverse=True))
+
+
+@pytest.mark.parametrize("n", [4096, 65536])
+def test_wide_primes_canonical_path(n):
+    """62- and 63-bit primes disable the lazy [0,4q) butterflies."""
+    qs = params.find_primes(n, 2, 63) + params.find_primes(n, 1, 62)
+    rs = [params.find_psi(q, n) for q in qs]
+    ctx = hip.Context(n, qs, rs)
+    x = uniform(n, qs, 99)
+    log_case(f"ntt wide primes n={n}")
+    f = _gpu_round(ctx, x, 0, 3, False)
+    assert np.array_equal(f, o.ntt_towers(x, n, qs, rs))
+    assert np.array_equal(_gpu_round(ctx, f, 0, 3, True), x)
+
+
+def test_out_of_place_and_subrange():
+    qs = params.mixed_width_primes(N16, 6)
+    rs = [params.min_root(q, N16) for q in qs]
+    ctx = hip.Context(N16, qs, rs)
+    x = uniform(N16, qs[2:5], 5)
+    log_case("ntt out of place, towers 2..5")
+    src = hip.DeviceBuffer(x.size).upload(x)
+    dst = hip.DeviceBuffer(x.size)
+    ctx.transform(dst.ptr, 2, 3, False, src_ptr=src.ptr)
+    ctx.synchronize()
+    assert np.array_equal(dst.download(), o.ntt_towers(x, N16, qs[2:5], rs[2:5]))
+    assert np.array_equal(src.download(), x)  # source untouched
+    ctx.transform(src.ptr, 2, 3, True, src_ptr=dst.ptr)
+    ctx.synchronize()
+    assert np.array_equal(src.download(), x)

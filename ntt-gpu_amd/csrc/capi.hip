@@ -707,7 +707,7 @@ int ntt_hip_keyswitch_create(ntt_hip_keyswitch **out, ntt_hip_context *ctx, size
   return guarded([&] {
     require(out && ctx, "null argument");
     require(L >= 1 && K >= 1 && L + K == ctx->q.size(), "context basis must be L Q towers then K P towers");
-    require(alpha >= 1 && alpha <= L, "invalid digit size");
+    require(alpha >= 1, "invalid digit size");  // alpha may exceed L: one shortened digit
     require(max_batch >= 1, "invalid batch capacity");
     auto ks = std::make_unique<ntt_hip_keyswitch>();
     ks->ctx = ctx;

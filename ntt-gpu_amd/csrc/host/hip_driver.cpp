@@ -1,0 +1,279 @@
+// Device transport of the resident Session: implements the Driver seam
+// (include/ntt/resident_driver.hpp; reference src/cuda/resident_adapter.inc:60-317)
+// on top of the C ABI of libntt_hip.so. One context (device twiddle tables
+// for the session basis) and its ordered stream per driver. Plain C++: no
+// HIP header is needed on this side of the boundary.
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+#include "ntt/resident_driver.hpp"
+#include "ntt_hip.h"
+
+namespace ntt::hip_transport {
+using namespace ::ntt::proposal;
+namespace transport = ::ntt::resident::detail;
+
+[[noreturn]] void raise(int status, const char *what) {
+  const std::string msg = std::string(what) + ": " + ntt_hip_last_error();
+  if (status == NTT_HIP_EINVAL) throw ContractError(msg);
+  throw std::runtime_error("HIP: " + msg);
+}
+inline void ok(int status, const char *what) {
+  if (status != NTT_HIP_OK) raise(status, what);
+}
+
+// Bytes of device constants a session basis needs: forward and inverse
+// {W, Shoup(W)} tables plus one per-prime record (ntt_hip_context_create).
+inline size_t table_bytes(const PolynomialLayout &p) {
+  return checked_product(p.towers.size(), checked_product(static_cast<size_t>(p.parameters->n()), 32) + 128);
+}
+
+struct Context {
+  PolynomialLayout base;
+  int device;
+  ntt_hip_context *ctx = nullptr;
+  bool cleanup_failed = false;
+  Context(PolynomialLayout p, int ordinal) : base(std::move(p)), device(ordinal) {
+    std::vector<uint64_t> q, psi;
+    for (const TowerRef ref : base.towers) {
+      q.push_back(base.parameters->tower(ref).q);
+      psi.push_back(base.parameters->tower(ref).psi);
+    }
+    ok(ntt_hip_context_create(&ctx, device, base.parameters->n(), q.data(), psi.data(), q.size()), "context");
+  }
+  ~Context() {
+    if (ctx && ntt_hip_context_destroy(ctx) != NTT_HIP_OK) cleanup_failed = true;
+  }
+  void *stream() const { return ntt_hip_context_stream(ctx); }
+  // Position of a contiguous layout inside the session basis.
+  size_t first(const PolynomialLayout &p) const {
+    validate(p);
+    require(*p.parameters == *base.parameters && p.active_q <= base.active_q, "foreign resident HIP parameters");
+    const auto it = std::find(base.towers.begin(), base.towers.end(), p.towers.front());
+    const size_t at = static_cast<size_t>(it - base.towers.begin());
+    require(at < base.towers.size() && p.towers.size() <= base.towers.size() - at &&
+                std::equal(p.towers.begin(), p.towers.end(), it),
+            "noncontiguous HIP plan view");
+    return at;
+  }
+  std::vector<size_t> positions(const std::vector<TowerRef> &refs) const {
+    std::vector<size_t> out;
+    for (const TowerRef r : refs) {
+      const auto it = std::find(base.towers.begin(), base.towers.end(), r);
+      require(it != base.towers.end(), "tower outside the session basis");
+      out.push_back(static_cast<size_t>(it - base.towers.begin()));
+    }
+    return out;
+  }
+  std::vector<size_t> p_positions() const {
+    std::vector<TowerRef> refs;
+    for (size_t j = 0; j < base.parameters->p().size(); ++j) refs.push_back({Family::P, j});
+    return positions(refs);
+  }
+};
+
+struct Buffer final : transport::Buffer {
+  std::shared_ptr<Context> context;
+  uint64_t *data = nullptr;
+  size_t elements;
+  Buffer(std::shared_ptr<Context> c, size_t n) : context(std::move(c)), elements(n) {
+    if (context->cleanup_failed) throw std::runtime_error("resident HIP cleanup failed");
+    void *p = nullptr;
+    ok(ntt_hip_malloc(context->device, &p, std::max<size_t>(1, n) * sizeof(uint64_t)), "allocate");
+    data = static_cast<uint64_t *>(p);
+  }
+  ~Buffer() override {
+    if (ntt_hip_free(context->device, data) != NTT_HIP_OK) context->cleanup_failed = true;
+  }
+};
+
+struct Event final : transport::Event {
+  std::shared_ptr<Context> context;
+  void *event = nullptr;
+  bool recorded = false;
+  explicit Event(std::shared_ptr<Context> c) : context(std::move(c)) {
+    if (context->cleanup_failed) throw std::runtime_error("resident HIP cleanup failed");
+    ok(ntt_hip_event_create(context->device, &event, 0), "event");
+  }
+  ~Event() override {
+    if (ntt_hip_event_destroy(context->device, event) != NTT_HIP_OK) context->cleanup_failed = true;
+  }
+};
+
+class Driver final : public transport::Driver {
+  std::shared_ptr<Context> c_;
+  const Buffer &buf(const transport::Buffer &b) const {
+    const auto &out = dynamic_cast<const Buffer &>(b);
+    require(out.context == c_, "foreign HIP buffer");
+    return out;
+  }
+  Event &evt(transport::Event &e) const {
+    auto &out = dynamic_cast<Event &>(e);
+    require(out.context == c_, "foreign HIP event");
+    return out;
+  }
+  void healthy() const {
+    if (c_->cleanup_failed) throw std::runtime_error("resident HIP cleanup failed");
+  }
+
+ public:
+  Driver(PolynomialLayout p, int ordinal) : c_(std::make_shared<Context>(std::move(p), ordinal)) {}
+  int ordinal() const override { return c_->device; }
+  const PolynomialLayout &layout() const override { return c_->base; }
+  size_t constant_bytes() const override { return table_bytes(c_->base); }
+  void check_health() const override { healthy(); }
+  std::unique_ptr<transport::Buffer> allocate(size_t n) override { return std::make_unique<Buffer>(c_, n); }
+  std::unique_ptr<transport::Event> make_event() override { return std::make_unique<Event>(c_); }
+
+  void upload(transport::Buffer &out, std::span<const uint64_t> src) override {
+    const auto &a = buf(out);
+    require(a.elements == src.size(), "HIP upload extent");
+    healthy();
+    ok(ntt_hip_memcpy(c_->device, a.data, src.data(), src.size_bytes(), NTT_HIP_H2D, c_->stream()), "upload");
+  }
+  void copy(transport::Buffer &out, const transport::Buffer &in, size_t offset, size_t n) override {
+    const auto &a = buf(out), &b = buf(in);
+    require(a.elements == n && offset <= b.elements && n <= b.elements - offset, "HIP copy extent");
+    healthy();
+    ok(ntt_hip_memcpy(c_->device, a.data, b.data + offset, n * sizeof(uint64_t), NTT_HIP_D2D, c_->stream()), "copy");
+  }
+  void copy_range(transport::Buffer &out, size_t dst, const transport::Buffer &in, size_t src, size_t n) override {
+    const auto &a = buf(out), &b = buf(in);
+    require(dst <= a.elements && n <= a.elements - dst && src <= b.elements && n <= b.elements - src,
+            "HIP range copy extent");
+    healthy();
+    ok(ntt_hip_memcpy(c_->device, a.data + dst, b.data + src, n * sizeof(uint64_t), NTT_HIP_D2D, c_->stream()),
+       "copy_range");
+  }
+  void transform(transport::Buffer &out, const PolynomialLayout &p, bool inverse) override {
+    const auto &a = buf(out);
+    require(a.elements == bytes(p) / sizeof(uint64_t), "HIP transform extent");
+    healthy();
+    ok(ntt_hip_transform(c_->ctx, a.data, nullptr, c_->first(p), p.towers.size(), inverse, c_->stream()),
+       "transform");
+  }
+  void pointwise(transport::Buffer &out, const transport::Buffer &right, size_t offset,
+                 const PolynomialLayout &p) override {
+    const auto &a = buf(out), &b = buf(right);
+    require(a.elements == bytes(p) / sizeof(uint64_t) && offset <= b.elements && a.elements <= b.elements - offset,
+            "HIP pointwise extent");
+    healthy();
+    ok(ntt_hip_pointwise(c_->ctx, a.data, b.data + offset, c_->first(p), p.towers.size(), c_->stream()),
+       "pointwise");
+  }
+  void hybrid_mac(transport::Buffer &rb, transport::Buffer &ra, const transport::Buffer &digit,
+                  const transport::Buffer &kb, const transport::Buffer &ka, const PolynomialLayout &layout,
+                  bool first_digit) override {
+    const auto &b = buf(rb), &a = buf(ra), &d = buf(digit), &k1 = buf(kb), &k2 = buf(ka);
+    const size_t count = bytes(layout) / sizeof(uint64_t);
+    require(b.elements == count && a.elements == count && d.elements == count && k1.elements == count &&
+                k2.elements == count,
+            "HIP Hybrid MAC extent");
+    healthy();
+    ok(ntt_hip_hybrid_mac(c_->ctx, b.data, a.data, d.data, k1.data, k2.data, c_->first(layout), layout.towers.size(),
+                          first_digit, c_->stream()),
+       "hybrid_mac");
+  }
+  void hybrid_rotate(transport::Buffer &ob, transport::Buffer &oa, const transport::Buffer &eb,
+                     const transport::Buffer &ea, const transport::Buffer &c0, size_t c0_offset,
+                     const transport::Buffer &scales, const PolynomialLayout &layout, uint32_t automorphism) override {
+    const auto &b = buf(ob), &a = buf(oa), &xb = buf(eb), &xa = buf(ea), &first = buf(c0), &table = buf(scales);
+    const size_t n = layout.parameters->n(), elements = n * layout.towers.size();
+    require(b.elements == elements && a.elements == elements && xb.elements == elements && xa.elements == elements &&
+                c0_offset <= first.elements && layout.active_q * n <= first.elements - c0_offset &&
+                table.elements == layout.active_q,
+            "HIP Hybrid rotation extent");
+    healthy();
+    const auto p = c_->p_positions();
+    ok(ntt_hip_hybrid_rotate(c_->ctx, b.data, a.data, xb.data, xa.data, first.data + c0_offset, c_->first(layout),
+                             layout.towers.size(), layout.active_q, p.data(), p.size(), automorphism, c_->stream()),
+       "hybrid_rotate");
+  }
+  void switch_basis(transport::Buffer &out, const transport::Buffer &in, size_t offset,
+                    const transport::Buffer &constants, const PolynomialLayout &source,
+                    const PolynomialLayout &target) override {
+    const auto &a = buf(out), &b = buf(in), &table = buf(constants);
+    const size_t n = target.parameters->n(), s = source.towers.size(), t = target.towers.size();
+    require(a.elements == n * t && offset <= b.elements && n * s <= b.elements - offset &&
+                table.elements == 3 * s + t + s * t,
+            "HIP basis extent");
+    healthy();
+    // The context derives (and caches) its Montgomery-form plan for the same
+    // source/target primes; the session's packed table is the shape check.
+    const auto src = c_->positions(source.towers), dst = c_->positions(target.towers);
+    ok(ntt_hip_switch_basis(c_->ctx, a.data, b.data + offset, src.data(), s, dst.data(), t, c_->stream()),
+       "switch_basis");
+  }
+  void correct(transport::Buffer &out, const transport::Buffer &left, size_t left_offset,
+               const transport::Buffer &right, size_t right_offset, const transport::Buffer &constants,
+               const PolynomialLayout &p) override {
+    const auto &a = buf(out), &b = buf(left), &c = buf(right), &table = buf(constants);
+    require(a.elements == p.parameters->n() * p.towers.size() && left_offset <= b.elements &&
+                a.elements <= b.elements - left_offset && right_offset <= c.elements &&
+                a.elements <= c.elements - right_offset && table.elements == 2 * p.towers.size(),
+            "HIP correction extent");
+    healthy();
+    const auto pp = c_->p_positions();
+    ok(ntt_hip_correct(c_->ctx, a.data, b.data + left_offset, c.data + right_offset, c_->first(p), p.towers.size(),
+                       pp.data(), pp.size(), c_->stream()),
+       "correct");
+  }
+  void download(std::span<uint64_t> target, const transport::Buffer &in, size_t offset) override {
+    const auto &a = buf(in);
+    require(offset <= a.elements && target.size() <= a.elements - offset, "HIP download extent");
+    healthy();
+    ok(ntt_hip_memcpy(c_->device, target.data(), a.data + offset, target.size_bytes(), NTT_HIP_D2H, c_->stream()),
+       "download");
+  }
+  void record(transport::Event &e) override {
+    auto &out = evt(e);
+    healthy();
+    ok(ntt_hip_event_record(c_->device, out.event, c_->stream()), "record");
+    out.recorded = true;
+  }
+  bool ready(transport::Event &e) override {
+    auto &out = evt(e);
+    require(out.recorded, "unrecorded HIP event");
+    healthy();
+    const int s = ntt_hip_event_query(c_->device, out.event);
+    if (s == NTT_HIP_NOT_READY) return false;
+    ok(s, "query");
+    return true;
+  }
+  void wait(transport::Event &e) override {
+    auto &out = evt(e);
+    require(out.recorded, "unrecorded HIP event");
+    healthy();
+    ok(ntt_hip_event_synchronize(c_->device, out.event), "wait");
+  }
+  bool quiesce() noexcept override {
+    try {
+      return ntt_hip_stream_synchronize(c_->device, c_->stream()) == NTT_HIP_OK;
+    } catch (...) {
+      return false;
+    }
+  }
+};
+
+}  // namespace ntt::hip_transport
+
+namespace ntt::resident {
+
+// Preflight before any device call (reference resident_adapter.inc:306-317).
+Session make_hip_session(const PolynomialLayout &p, int device, Limits limits) {
+  proposal::validate(p);
+  proposal::require(p.domain == proposal::Domain::Coefficient && p.order == proposal::Order::Natural &&
+                        p.parameters->n() <= 65536 && p.towers.size() <= 65535 && device >= 0 && limits.operations,
+                    "unsupported resident HIP session");
+  const size_t constants = hip_transport::table_bytes(p);
+  proposal::require(constants <= limits.device_bytes && constants <= limits.host_bytes,
+                    "HIP plan exceeds resident setup budget");
+  return Session(std::make_shared<hip_transport::Driver>(p, device), limits);
+}
+
+Session make_cuda_session(const PolynomialLayout &p, int device, Limits limits) {
+  return make_hip_session(p, device, limits);
+}
+
+}  // namespace ntt::resident

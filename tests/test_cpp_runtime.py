@@ -1,0 +1,29 @@
+"""The C++ include/ntt runtime, as the reference gates it (ctest binaries):
+resident_tests runs the Session on the deferred CPU transport; the GPU
+binary drives the same API over the gfx950 transport."""
+import os
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(HERE, "cpp", "build")
+
+
+def _run(name, timeout):
+    exe = os.path.join(BUILD, name)
+    assert os.path.exists(exe), f"{exe} not built (run __graft_entry__.build())"
+    print(f"[case] {name}", flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=timeout)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+def test_resident_runtime_on_host_transport():
+    assert "RESIDENT HOST RUNTIME CHECKS PASSED" in _run("resident_tests", 600)
+
+
+@pytest.mark.gpu
+def test_resident_runtime_on_hip_transport():
+    assert "HIP RESIDENT CHECKS PASSED" in _run("hip_resident_tests", 600)

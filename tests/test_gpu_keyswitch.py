@@ -35,7 +35,8 @@ def test_switch_basis(small, src, dst):
     x = uniform(hp.n, [mods[i] for i in src], 7)
     log_case(f"switch_basis {src}->{dst}")
     out = hip.DeviceBuffer(hp.n * len(dst))
-    ctx.switch_basis(out.ptr, _dev(x).ptr, src, dst)
+    dx = _dev(x)
+    ctx.switch_basis(out.ptr, dx.ptr, src, dst)
     ctx.synchronize()
     want = o.basis_switch(x, hp.n, [mods[i] for i in src], [mods[i] for i in dst])
     assert np.array_equal(out.download(), want)
@@ -50,7 +51,8 @@ def test_switch_basis_per_term_path():
     x[::7] = 0
     log_case("switch_basis per-term 63-bit")
     out = hip.DeviceBuffer(2 * n)
-    ctx.switch_basis(out.ptr, _dev(x).ptr, [0, 1, 2, 3], [4, 5])
+    dx = _dev(x)
+    ctx.switch_basis(out.ptr, dx.ptr, [0, 1, 2, 3], [4, 5])
     ctx.synchronize()
     assert np.array_equal(out.download(), o.basis_switch(x, n, qs[:4], qs[4:]))
 
@@ -72,7 +74,8 @@ def test_correct_mac_rotate_pointwise(small):
     right = uniform(n, hp.q, 2)
     log_case("correct")
     out = hip.DeviceBuffer(L * n)
-    ctx.correct(out.ptr, _dev(left).ptr, _dev(right).ptr, 0, L, [L + j for j in range(K)])
+    dl, dr = _dev(left), _dev(right)
+    ctx.correct(out.ptr, dl.ptr, dr.ptr, 0, L, [L + j for j in range(K)])
     ctx.synchronize()
     want = np.empty_like(left)
     for l, q in enumerate(hp.q):
@@ -98,9 +101,10 @@ def test_correct_mac_rotate_pointwise(small):
 
     log_case("hybrid_rotate")
     c0 = uniform(n, hp.q, 40)
+    dc0 = _dev(c0)
     for k in (5, 25, 2 * n - 1):
         ob, oa = hip.DeviceBuffer(T * n), hip.DeviceBuffer(T * n)
-        ctx.hybrid_rotate(ob.ptr, oa.ptr, b.ptr, a.ptr, _dev(c0).ptr, 0, T, L, [L + j for j in range(K)], k)
+        ctx.hybrid_rotate(ob.ptr, oa.ptr, b.ptr, a.ptr, dc0.ptr, 0, T, L, [L + j for j in range(K)], k)
         ctx.synchronize()
         rb, ra = o.hybrid_rotate(wb, wa, c0, n, hp.q, hp.p, k)
         assert np.array_equal(ob.download(), rb) and np.array_equal(oa.download(), ra)
@@ -110,7 +114,8 @@ def test_correct_mac_rotate_pointwise(small):
     log_case("pointwise")
     x, y = uniform(n, mods, 50), uniform(n, mods, 51)
     bx = _dev(x)
-    ctx.pointwise(bx.ptr, _dev(y).ptr, 0, T)
+    dy = _dev(y)
+    ctx.pointwise(bx.ptr, dy.ptr, 0, T)
     ctx.synchronize()
     assert np.array_equal(bx.download(), o.pointwise_towers(x, y, n, mods))
 
@@ -188,7 +193,8 @@ def test_keyswitch_batch_equals_alone():
     for c in range(5):
         xs = x[c * L * n:(c + 1) * L * n]
         ob, oa = hip.DeviceBuffer(L * n), hip.DeviceBuffer(L * n)
-        ks.run(_dev(xs).ptr, 1, ob.ptr, oa.ptr, ws.ptr)
+        dxs = _dev(xs)
+        ks.run(dxs.ptr, 1, ob.ptr, oa.ptr, ws.ptr)
         ctx.synchronize()
         assert np.array_equal(ob.download(), gb[c * L * n:(c + 1) * L * n])
         assert np.array_equal(oa.download(), ga[c * L * n:(c + 1) * L * n])

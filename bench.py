@@ -26,6 +26,13 @@ L, K, ALPHA = 24, 6, 6
 HBM_PEAK_GBS = 8000.0
 
 
+T0 = time.perf_counter()
+
+
+def progress(msg):
+    print(f"[bench {time.perf_counter() - T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
+
+
 def pct(values, q):
     v = sorted(values)
     pos = (len(v) - 1) * q
@@ -119,6 +126,7 @@ def main():
                                    "(data-parallel over ciphertexts), so --gpus must be 1"}))
         sys.exit(2)
 
+    progress(f"device {hip.device_name(0)}")
     hp = params.openfhe_like(N, L, K, ALPHA)
     B = args.batch
     ctx = hip.Context(N, hp.q + hp.p, hp.q_psi + hp.p_psi)
@@ -127,7 +135,9 @@ def main():
     beta = hp.beta()
     kb = np.concatenate([seeded(N, full, 1000 + d) for d in range(beta)])
     ka = np.concatenate([seeded(N, full, 2000 + d) for d in range(beta)])
+    progress("key generated")
     ks.upload_key(kb, ka, hp.M, hp.q)
+    progress("key resident")
     x = np.concatenate([seeded(N, hp.q, 5000 + c) for c in range(B)])
     comp = L * N
     d_in = hip.DeviceBuffer(B * comp).upload(x)
@@ -136,6 +146,7 @@ def main():
     stream = hip.Stream(0)
     s = stream.handle
 
+    progress("inputs resident")
     # ---- batched: captured graph, one launch per step
     graph = ks.capture(d_in.ptr, B, d_b.ptr, d_a.ptr, ws.ptr, s)
     for _ in range(args.warmup):
@@ -154,6 +165,7 @@ def main():
     gpu_b = d_b.download(comp, 0)
     gpu_a = d_a.download(comp, 0)
 
+    progress(f"batched done: {batched_kss:.1f} KS/s")
     # ---- plain: one component per submission, per-stage launches, sync per request
     ws1 = hip.DeviceBuffer(ks.workspace_words(1))
     for _ in range(2):
@@ -170,10 +182,13 @@ def main():
     plain_wall = time.perf_counter() - t1
     plain_kss = args.plain_requests / plain_wall
 
+    progress(f"plain done: {plain_kss:.1f} KS/s")
     roof = ntt_roofline(args.ntt_towers, 6, s)
+    progress(f"ntt roofline done: {roof['achieved']} GB/s")
     cpu, exact = (None, None)
     if not args.no_cpu:
         cpu, exact = cpu_baseline(hp, x[:comp], kb, ka, gpu_b, gpu_a, args.cpu_samples)
+        progress(f"cpu baseline done: {cpu['value']} KS/s, exact={exact}")
 
     out = {
         "metric": "KeySwitch/s N=2^16 L=24 dnum=4 batched vs plain; p50/p99 ms; NTT HBM GB/s",

@@ -141,6 +141,17 @@ int ntt_hip_keyswitch_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batc
 int ntt_hip_keyswitch_capture(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batch, uint64_t *out_b,
                               uint64_t *out_a, void *workspace, void *stream, void **graph);
 int ntt_hip_graph_launch(void *graph, void *stream);
+/* Batched ModUp (Session::mod_up, src/resident.cpp:431-447) on the plan's
+ * basis: in = batch * L * n (evaluation if in_eval, else coefficient Q),
+ * out = batch * (L + K) * n evaluation QP. */
+int ntt_hip_modup_workspace_bytes(ntt_hip_keyswitch *ks, size_t batch, size_t *bytes);
+int ntt_hip_modup_run(ntt_hip_keyswitch *ks, const uint64_t *in, int in_eval, size_t batch, uint64_t *out,
+                      void *workspace, void *stream);
+/* Batched ModDown (Session::mod_down, src/resident.cpp:448-457, correction
+ * :404-430): in = batch * (L + K) * n evaluation QP, out = batch * L * n. */
+int ntt_hip_moddown_workspace_bytes(ntt_hip_keyswitch *ks, size_t batch, size_t *bytes);
+int ntt_hip_moddown_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batch, uint64_t *out, void *workspace,
+                        void *stream);
 int ntt_hip_graph_destroy(void *graph);
 
 #pragma GCC visibility pop

@@ -333,6 +333,7 @@ struct ntt_hip_keyswitch {
   size_t max_batch = 0;
   std::vector<std::shared_ptr<BasisPlanDev>> up;  // per digit: digit towers -> complement
   std::shared_ptr<BasisPlanDev> down;             // P -> Q
+  std::shared_ptr<BasisPlanDev> full_up;          // all active Q -> P (ModUp)
   const ulonglong2 *pinv = nullptr;               // P^-1 mod q_i, i < L
   std::unique_ptr<DeviceBlob> tower_prime;        // identity map 0..L+K
   std::unique_ptr<DeviceBlob> jobs;               // complement NTT jobs for max_batch
@@ -733,6 +734,7 @@ int ntt_hip_keyswitch_create(ntt_hip_keyswitch **out, ntt_hip_context *ctx, size
     for (size_t i = 0; i < L; ++i) q.push_back(i);
     require(K <= 40 && L <= 40, "ModDown limited to 40 towers per basis");
     ks->down = basis_plan(*ctx, p, q);
+    ks->full_up = basis_plan(*ctx, q, p);
     ks->pinv = scale_table(*ctx, 0, L, p, 0);
     std::vector<uint32_t> tp(T);
     for (size_t i = 0; i < T; ++i) tp[i] = (uint32_t)i;
@@ -987,6 +989,113 @@ int ntt_hip_graph_destroy(void *graph) {
     if (h->exec) (void)hipGraphExecDestroy(h->exec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
     delete h;
+    return NTT_HIP_OK;
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- batched ModUp / ModDown
+namespace {
+
+void launch_basis(const BasisPlanDev &plan, const uint64_t *in, uint64_t *out, uint64_t in_stride,
+                  uint64_t out_stride, uint32_t n, uint32_t polys, hipStream_t s) {
+  BasisLaunch a{};
+  a.plan = plan.dev;
+  a.in = in;
+  a.out = out;
+  a.in_poly_stride = in_stride;
+  a.out_poly_stride = out_stride;
+  a.n = n;
+  a.targets_per_block = 8;
+  a.plan_host_s = plan.s;
+  a.plan_host_t = plan.t;
+  a.skip_at = 0xffffffffu;
+  a.skip_len = 0;
+  hip_ok(launch_basis_switch(a, plan.per_term, polys, s), "conversion");
+}
+
+}  // namespace
+
+extern "C" {
+
+int ntt_hip_modup_workspace_bytes(ntt_hip_keyswitch *ks, size_t batch, size_t *bytes) {
+  return guarded([&] {
+    require(ks && bytes && batch >= 1, "invalid argument");
+    *bytes = batch * (size_t)ks->L * ks->n() * sizeof(uint64_t);
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_modup_run(ntt_hip_keyswitch *ks, const uint64_t *in, int in_eval, size_t batch, uint64_t *out,
+                      void *workspace, void *stream) {
+  return guarded([&] {
+    require(ks && in && out && workspace && batch >= 1, "null argument");
+    ntt_hip_context &c = *ks->ctx;
+    const size_t L = ks->L, K = ks->K, T = L + K, n = c.n;
+    require(batch * T <= 65535, "batch too large for one launch");
+    const hipStream_t s = c.pick(stream);
+    const PrimeConst *pr = c.dprimes();
+    uint64_t *coeff = (uint64_t *)workspace;  // batch * L * n
+    DeviceGuard g(c.device);
+    if (in_eval) {
+      // Q part of the output is the input itself; coefficients for the lift
+      hip_ok(launch_ntt(affine(coeff, in, n, L, 0, L * n, L * n), pr, c.log_n, (uint32_t)(batch * L), true, c.lazy, s),
+             "modup intt");
+      for (size_t b = 0; b < batch; ++b)
+        HIP_CHECK(hipMemcpyAsync(out + b * T * n, in + b * L * n, L * n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+      launch_basis(*ks->full_up, coeff, out + L * n, L * n, T * n, (uint32_t)n, (uint32_t)batch, s);
+    } else {
+      hip_ok(launch_ntt(affine(out, in, n, L, 0, T * n, L * n), pr, c.log_n, (uint32_t)(batch * L), false, c.lazy, s),
+             "modup ntt q");
+      launch_basis(*ks->full_up, in, out + L * n, L * n, T * n, (uint32_t)n, (uint32_t)batch, s);
+    }
+    hip_ok(launch_ntt(affine(out + L * n, nullptr, n, K, L, T * n, 0), pr, c.log_n, (uint32_t)(batch * K), false,
+                      c.lazy, s),
+           "modup ntt p");
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_moddown_workspace_bytes(ntt_hip_keyswitch *ks, size_t batch, size_t *bytes) {
+  return guarded([&] {
+    require(ks && bytes && batch >= 1, "invalid argument");
+    *bytes = batch * (size_t)(ks->K + ks->L) * ks->n() * sizeof(uint64_t);
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_moddown_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batch, uint64_t *out, void *workspace,
+                        void *stream) {
+  return guarded([&] {
+    require(ks && in && out && workspace && batch >= 1, "null argument");
+    ntt_hip_context &c = *ks->ctx;
+    const size_t L = ks->L, K = ks->K, T = L + K, n = c.n;
+    require(batch * T <= 65535, "batch too large for one launch");
+    const hipStream_t s = c.pick(stream);
+    const PrimeConst *pr = c.dprimes();
+    uint64_t *pc = (uint64_t *)workspace;  // batch * K * n
+    uint64_t *corr = pc + batch * K * n;   // batch * L * n
+    DeviceGuard g(c.device);
+    hip_ok(launch_ntt(affine(pc, in + L * n, n, K, L, K * n, T * n), pr, c.log_n, (uint32_t)(batch * K), true, c.lazy,
+                      s),
+           "moddown intt");
+    launch_basis(*ks->down, pc, corr, K * n, L * n, (uint32_t)n, (uint32_t)batch, s);
+    hip_ok(launch_ntt(affine(corr, nullptr, n, L, 0, L * n, 0), pr, c.log_n, (uint32_t)(batch * L), false, c.lazy, s),
+           "moddown ntt");
+    CorrectLaunch cl{};
+    cl.out = out;
+    cl.left = in;
+    cl.right = corr;
+    cl.out_poly_stride = L * n;
+    cl.left_poly_stride = T * n;
+    cl.right_poly_stride = L * n;
+    cl.primes = pr;
+    cl.scale = ks->pinv;
+    cl.first_prime = 0;
+    cl.n = (uint32_t)n;
+    cl.towers = (uint32_t)L;
+    hip_ok(launch_correct(cl, (uint32_t)batch, s), "moddown correction");
     return NTT_HIP_OK;
   });
 }

@@ -61,6 +61,10 @@ SIGNATURES = {
     "ntt_hip_keyswitch_capture": [VP, VP, SZ, VP, VP, VP, VP, ctypes.POINTER(VP)],
     "ntt_hip_graph_launch": [VP, VP],
     "ntt_hip_graph_destroy": [VP],
+    "ntt_hip_modup_workspace_bytes": [VP, SZ, SZP],
+    "ntt_hip_modup_run": [VP, VP, INT, SZ, VP, VP, VP],
+    "ntt_hip_moddown_workspace_bytes": [VP, SZ, SZP],
+    "ntt_hip_moddown_run": [VP, VP, SZ, VP, VP, VP],
 }
 _RESTYPES = {
     "ntt_hip_last_error": ctypes.c_char_p,
@@ -323,6 +327,23 @@ class KeySwitch:
     def run(self, in_ptr, batch, out_b_ptr, out_a_ptr, workspace_ptr, stream=None):
         check(lib().ntt_hip_keyswitch_run(self.handle, in_ptr, batch, out_b_ptr, out_a_ptr, workspace_ptr, stream),
               "keyswitch_run")
+
+    def modup_workspace_words(self, batch):
+        out = SZ(0)
+        check(lib().ntt_hip_modup_workspace_bytes(self.handle, batch, ctypes.byref(out)), "modup_workspace")
+        return out.value // 8
+
+    def moddown_workspace_words(self, batch):
+        out = SZ(0)
+        check(lib().ntt_hip_moddown_workspace_bytes(self.handle, batch, ctypes.byref(out)), "moddown_workspace")
+        return out.value // 8
+
+    def mod_up(self, in_ptr, batch, out_ptr, workspace_ptr, evaluation=True, stream=None):
+        check(lib().ntt_hip_modup_run(self.handle, in_ptr, int(evaluation), batch, out_ptr, workspace_ptr, stream),
+              "modup_run")
+
+    def mod_down(self, in_ptr, batch, out_ptr, workspace_ptr, stream=None):
+        check(lib().ntt_hip_moddown_run(self.handle, in_ptr, batch, out_ptr, workspace_ptr, stream), "moddown_run")
 
     def capture(self, in_ptr, batch, out_b_ptr, out_a_ptr, workspace_ptr, stream):
         g = VP()

@@ -198,3 +198,29 @@ def test_keyswitch_batch_equals_alone():
         ctx.synchronize()
         assert np.array_equal(ob.download(), gb[c * L * n:(c + 1) * L * n])
         assert np.array_equal(oa.download(), ga[c * L * n:(c + 1) * L * n])
+
+
+@pytest.mark.parametrize("evaluation", [True, False])
+def test_batched_modup_moddown_config_c3(evaluation):
+    """N=2^16, Q: 24 towers (60 + 50-bit), P: 6 towers (60-bit): batched ModUp
+    (evaluation or coefficient input) and ModDown, each bit-exact vs the oracle."""
+    hp = params.openfhe_like(65536, 24, 6, 6)
+    n, L, K, B = hp.n, 24, 6, 2
+    ctx = hip.Context(n, hp.q + hp.p, hp.q_psi + hp.p_psi)
+    ks = hip.KeySwitch(ctx, L, K, hp.alpha, B)
+    x = np.concatenate([uniform(n, hp.q, 300 + c) for c in range(B)])
+    log_case(f"modup/moddown c3 evaluation={evaluation}")
+    din = _dev(x)
+    up = hip.DeviceBuffer(B * (L + K) * n)
+    ws = hip.DeviceBuffer(max(ks.modup_workspace_words(B), ks.moddown_workspace_words(B)))
+    ks.mod_up(din.ptr, B, up.ptr, ws.ptr, evaluation=evaluation)
+    down = hip.DeviceBuffer(B * L * n)
+    ks.mod_down(up.ptr, B, down.ptr, ws.ptr)
+    ctx.synchronize()
+    gu, gd = up.download(), down.download()
+    for c in range(B):
+        xs = x[c * L * n:(c + 1) * L * n]
+        wu = o.mod_up(xs, n, hp.q, hp.q_psi, hp.p, hp.p_psi, evaluation=evaluation)
+        assert np.array_equal(gu[c * (L + K) * n:(c + 1) * (L + K) * n], wu), f"ModUp component {c}"
+        wd = o.mod_down(wu, n, hp.q, hp.q_psi, hp.p, hp.p_psi)
+        assert np.array_equal(gd[c * L * n:(c + 1) * L * n], wd), f"ModDown component {c}"

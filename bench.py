@@ -87,6 +87,56 @@ def ntt_roofline(towers, reps, stream):
     }
 
 
+def served_stream(requests, rates, seed=0xC5):
+    """Config C5: Poisson arrivals of mixed N/L/dnum/op requests at swept rates
+    into the continuous-batching server; p50/p99 latency and throughput per
+    rate, and every response compared with the same request served alone."""
+    from nttgpu import serve, workload
+    results = []
+    reqs, pools = workload.stream(requests, 1.0, seed)
+    srv = serve.Server(streams=4, max_batch=32, hbm_budget_bytes=48 << 30)
+    ms = workload.MixedServer(srv, pools)
+    for r in reqs:
+        ms.params_for(r)
+    alone_srv = serve.Server(streams=1, max_batch=1, hbm_budget_bytes=16 << 30)
+    alone = workload.MixedServer(alone_srv, pools)
+    # warm every (params, level) plan on both servers outside the timed stream
+    seen = set()
+    for r in reqs:
+        key = (r.n, r.level, r.dnum, r.op)
+        if key not in seen:
+            seen.add(key)
+            ms.wait(ms.submit(r), r)
+            alone.wait(alone.submit(r), r)
+    progress(f"serve: {len(seen)} warm plans")
+    reference = [alone.wait(alone.submit(r), r)[0] for r in reqs]
+    progress("serve: alone results ready")
+    for rate in rates:
+        gaps = np.random.default_rng(seed + int(rate)).exponential(1.0 / rate, len(reqs))
+        tickets, t0 = [], time.perf_counter()
+        due = t0
+        for r, g in zip(reqs, gaps):
+            due += g
+            while time.perf_counter() < due:
+                pass
+            tickets.append(ms.submit(r))
+        outs = [ms.wait(t, r) for t, r in zip(tickets, reqs)]
+        wall = time.perf_counter() - t0
+        lat = [o[1] for o in outs]
+        exact = all(np.array_equal(o[0][0], ref[0]) and (ref[1] is None or np.array_equal(o[0][1], ref[1]))
+                    for o, ref in zip(outs, reference))
+        results.append({"offered_rate_per_s": rate, "requests": len(reqs), "throughput_per_s": round(len(reqs) / wall, 1),
+                        "p50_ms": round(pct(lat, 0.5), 3), "p99_ms": round(pct(lat, 0.99), 3),
+                        "mean_batch": round(statistics.mean(o[2] for o in outs), 2),
+                        "all_equal_alone": exact})
+        progress(f"serve rate {rate}/s: p50 {results[-1]['p50_ms']} ms p99 {results[-1]['p99_ms']} ms exact={exact}")
+    stats = srv.stats()
+    srv.close()
+    alone_srv.close()
+    return {"rates": results, "peak_device_bytes": stats["peak_device_bytes"], "max_in_flight": stats["max_in_flight"],
+            "mix": "40% KeySwitch, 30% NTT/INTT, 30% ModUp/ModDown; N in 2^13..2^16; L in [3,30]; dnum in {2,3,4}"}
+
+
 def cpu_baseline(hp, x0, kb, ka, gpu_b, gpu_a, samples):
     """The oracle's key switch (reference algorithm, 1 thread, towers in sequence)
     on component 0 of the same batch; also checks the GPU output bit-exactly."""
@@ -119,6 +169,9 @@ def main():
     ap.add_argument("--ntt-towers", type=int, default=2048)
     ap.add_argument("--cpu-samples", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=0)
+    ap.add_argument("--roofline-only", action="store_true")
+    ap.add_argument("--serve-rates", type=str, default="200,1000")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -127,6 +180,23 @@ def main():
         sys.exit(2)
 
     progress(f"device {hip.device_name(0)}")
+    if args.roofline_only:
+        # For counter runs: the NTT roofline launches plus a plain copy kernel
+        # of known size (1 GiB read + 1 GiB written) to calibrate the counters.
+        s = hip.Stream(0)
+        roof = ntt_roofline(args.ntt_towers, 4, s.handle)
+        words = 1 << 27
+        a, b = hip.DeviceBuffer(words), hip.DeviceBuffer(words)
+        e0, e1 = hip.Event(), hip.Event()
+        hip.check(hip.lib().ntt_hip_copy_words(0, b.ptr, a.ptr, words, s.handle), "copy")
+        e0.record(s.handle)
+        hip.check(hip.lib().ntt_hip_copy_words(0, b.ptr, a.ptr, words, s.handle), "copy")
+        e1.record(s.handle)
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1)
+        print(json.dumps({"roofline": roof, "copy_kernel": {"bytes": 16 * words, "ms": round(ms, 4),
+                                                             "GBps": round(16 * words / ms / 1e6, 1)}}), flush=True)
+        return
     hp = params.openfhe_like(N, L, K, ALPHA)
     B = args.batch
     ctx = hip.Context(N, hp.q + hp.p, hp.q_psi + hp.p_psi)
@@ -185,6 +255,9 @@ def main():
     progress(f"plain done: {plain_kss:.1f} KS/s")
     roof = ntt_roofline(args.ntt_towers, 6, s)
     progress(f"ntt roofline done: {roof['achieved']} GB/s")
+    served = None
+    if args.serve_requests:
+        served = served_stream(args.serve_requests, [float(r) for r in args.serve_rates.split(",")])
     cpu, exact = (None, None)
     if not args.no_cpu:
         cpu, exact = cpu_baseline(hp, x[:comp], kb, ka, gpu_b, gpu_a, args.cpu_samples)
@@ -215,6 +288,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "bit_exact_vs_oracle": exact,
+        "served_stream": served,
         "device": hip.device_name(0),
     }
     print(json.dumps(out), flush=True)

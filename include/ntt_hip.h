@@ -63,6 +63,9 @@ int ntt_hip_event_record(int device, void *event, void *stream);
 int ntt_hip_event_query(int device, void *event); /* NTT_HIP_OK ready, NTT_HIP_NOT_READY pending */
 int ntt_hip_event_synchronize(int device, void *event);
 int ntt_hip_event_elapsed(float *ms, void *start, void *stop);
+/* dst[i] = src[i] for `words` words with a plain streaming kernel: the
+ * calibration kernel for HBM byte counters (no reference counterpart). */
+int ntt_hip_copy_words(int device, uint64_t *dst, const uint64_t *src, size_t words, void *stream);
 
 /* ---- context: one fixed basis of towers (q_t, psi_t) of ring dimension n
  * with its device twiddle tables (reference resident_adapter.inc:60-130

@@ -1101,3 +1101,12 @@ int ntt_hip_moddown_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batch,
 }
 
 }  // extern "C"
+
+extern "C" int ntt_hip_copy_words(int device, uint64_t *dst, const uint64_t *src, size_t words, void *stream) {
+  return guarded([&] {
+    require(dst && src, "null copy pointer");
+    DeviceGuard g(device);
+    hip_ok(launch_copy_words(dst, src, words, (hipStream_t)stream), "copy kernel");
+    return NTT_HIP_OK;
+  });
+}

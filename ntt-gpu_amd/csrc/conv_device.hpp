@@ -75,3 +75,7 @@ hipError_t launch_pointwise(uint64_t *out, const uint64_t *right, const PrimeCon
 hipError_t launch_rotate(const RotateLaunch &r, uint32_t towers, hipStream_t stream);
 
 }  // namespace nttamd
+
+namespace nttamd {
+hipError_t launch_copy_words(uint64_t *dst, const uint64_t *src, uint64_t words, hipStream_t stream);
+}

@@ -9,6 +9,8 @@
 // being stored as (Bhat_i mod m) * 2^64 mod m so the result is the plain
 // residue. A per-term reduction variant covers sums that could exceed
 // m * 2^64 (very wide or very many source primes).
+#include <algorithm>
+
 #include "conv_device.hpp"
 
 namespace nttamd {
@@ -219,4 +221,20 @@ hipError_t launch_rotate(const RotateLaunch &r, uint32_t towers, hipStream_t str
   return hipGetLastError();
 }
 
+}  // namespace nttamd
+
+namespace nttamd {
+// Plain word copy: the calibration kernel for HBM byte counters (its
+// algorithmic traffic is exactly 16 bytes per word).
+__global__ __launch_bounds__(256) void copy_words_kernel(uint64_t *__restrict__ dst, const uint64_t *__restrict__ src,
+                                                         uint64_t words) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < words; i += (uint64_t)gridDim.x * 256)
+    dst[i] = src[i];
+}
+
+hipError_t launch_copy_words(uint64_t *dst, const uint64_t *src, uint64_t words, hipStream_t stream) {
+  const uint64_t blocks = std::min<uint64_t>((words + 255) / 256, 65536);
+  hipLaunchKernelGGL(copy_words_kernel, dim3((uint32_t)blocks), dim3(256), 0, stream, dst, src, words);
+  return hipGetLastError();
+}
 }  // namespace nttamd

@@ -1,0 +1,526 @@
+// Continuous-batching server (include/ntt/serve.hpp, C ABI include/ntt_serve.h).
+//
+// One worker thread owns every device call. Each loop iteration it
+//   1. retires finished batches (event query; results copied out of the
+//      slot's pinned staging, tickets released),
+//   2. fills every idle stream slot with a new batch: the oldest pending
+//      request chooses the (params, level, op) group and all pending requests
+//      of that group join it, up to max_batch and as far as the device budget
+//      allows (admission by HBM footprint),
+//   3. sleeps briefly only when it neither retired nor launched anything.
+// So requests that arrive while others run are admitted as soon as a slot
+// and memory free up, and up to `streams` batches overlap on the device.
+// Contexts, conversion plans and evaluation keys are created on first use of
+// a (params, level) and stay resident.
+#include <algorithm>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <unordered_map>
+
+#include "ntt/serve.hpp"
+#include "ntt_hip.h"
+#include "ntt_serve.h"
+
+namespace ntt::serve {
+namespace {
+using Clock = std::chrono::steady_clock;
+
+void ok(int status, const char *what) {
+  if (status == NTT_HIP_OK) return;
+  const std::string msg = std::string(what) + ": " + ntt_hip_last_error();
+  if (status == NTT_HIP_EINVAL) throw std::invalid_argument(msg);
+  throw std::runtime_error("HIP: " + msg);
+}
+
+struct Request {
+  uint64_t ticket = 0;
+  uint32_t params = 0;
+  Op op = Op::Forward;
+  size_t level = 0;
+  std::vector<uint64_t> in;
+  Clock::time_point t0;
+  Result result;
+  std::string error;
+  bool done = false;
+};
+
+struct LevelPlan {
+  ntt_hip_context *ctx = nullptr;
+  ntt_hip_keyswitch *ks = nullptr;
+  ~LevelPlan() {
+    if (ks) ntt_hip_keyswitch_destroy(ks);
+    if (ctx) ntt_hip_context_destroy(ctx);
+  }
+};
+
+struct Slot {
+  void *stream = nullptr, *event = nullptr;
+  void *device = nullptr;  // grow-only device arena
+  size_t device_bytes = 0;
+  void *pinned = nullptr;  // grow-only pinned host staging
+  size_t pinned_bytes = 0;
+  std::vector<std::shared_ptr<Request>> batch;
+  size_t out_words = 0, out2_words = 0, in_words = 0;
+  bool busy = false;
+};
+
+}  // namespace
+
+struct Server::Impl {
+  Options opt;
+  std::vector<std::shared_ptr<const ParamSet>> params;  // shared: the worker keeps its copy
+  std::map<std::pair<uint32_t, size_t>, std::unique_ptr<LevelPlan>> plans;
+  std::vector<Slot> slots;
+  mutable std::mutex mu;
+  std::condition_variable cv_submit, cv_done;
+  std::deque<std::shared_ptr<Request>> pending;
+  std::unordered_map<uint64_t, std::shared_ptr<Request>> tickets;
+  uint64_t next_ticket = 1;
+  bool stop = false;
+  Stats stats;
+  size_t arena_bytes = 0;  // sum of device arenas (the admission budget)
+  std::thread worker;
+
+  size_t in_words(const ParamSet &p, Op op, size_t L) const {
+    return (op == Op::ModDown ? L + p.p.size() : L) * p.n;
+  }
+  size_t out_words(const ParamSet &p, Op op, size_t L) const {
+    return (op == Op::ModUp ? L + p.p.size() : L) * p.n;
+  }
+
+  std::shared_ptr<const ParamSet> param_set(uint32_t id) const {
+    std::lock_guard<std::mutex> lock(mu);
+    return params.at(id);
+  }
+
+  LevelPlan &plan(uint32_t id, size_t L) {
+    auto key = std::make_pair(id, L);
+    auto it = plans.find(key);
+    if (it != plans.end()) return *it->second;
+    const ParamSet &p = *param_set(id);
+    auto lp = std::make_unique<LevelPlan>();
+    std::vector<uint64_t> q(p.q.begin(), p.q.begin() + L), r(p.q_psi.begin(), p.q_psi.begin() + L);
+    q.insert(q.end(), p.p.begin(), p.p.end());
+    r.insert(r.end(), p.p_psi.begin(), p.p_psi.end());
+    ok(ntt_hip_context_create(&lp->ctx, opt.device, p.n, q.data(), r.data(), q.size()), "context");
+    ok(ntt_hip_keyswitch_create(&lp->ks, lp->ctx, L, p.p.size(), p.alpha, opt.max_batch), "keyswitch plan");
+    ok(ntt_hip_keyswitch_upload_key(lp->ks, p.key_b.data(), p.key_a.data(), p.q.size(), p.q.data()), "key");
+    return *plans.emplace(key, std::move(lp)).first->second;
+  }
+
+  size_t workspace_bytes(LevelPlan &lp, Op op, size_t b) {
+    size_t bytes = 0;
+    if (op == Op::KeySwitch) ok(ntt_hip_keyswitch_workspace_bytes(lp.ks, b, &bytes), "workspace");
+    else if (op == Op::ModUp) ok(ntt_hip_modup_workspace_bytes(lp.ks, b, &bytes), "workspace");
+    else if (op == Op::ModDown) ok(ntt_hip_moddown_workspace_bytes(lp.ks, b, &bytes), "workspace");
+    return bytes;
+  }
+
+  size_t footprint(const ParamSet &p, LevelPlan &lp, Op op, size_t L, size_t b) {
+    const size_t words = in_words(p, op, L) + out_words(p, op, L) * (op == Op::KeySwitch ? 2 : 1);
+    return b * words * 8 + workspace_bytes(lp, op, b);
+  }
+
+  void ensure_arena(Slot &s, size_t device_bytes, size_t pinned_bytes) {
+    if (s.device_bytes < device_bytes) {
+      if (s.device) ok(ntt_hip_free(opt.device, s.device), "free");
+      arena_bytes -= s.device_bytes;
+      s.device = nullptr;
+      s.device_bytes = 0;
+      ok(ntt_hip_malloc(opt.device, &s.device, device_bytes), "arena");
+      s.device_bytes = device_bytes;
+      arena_bytes += device_bytes;
+    }
+    if (s.pinned_bytes < pinned_bytes) {
+      if (s.pinned) ok(ntt_hip_host_free(s.pinned), "host free");
+      s.pinned = nullptr;
+      s.pinned_bytes = 0;
+      ok(ntt_hip_host_alloc(&s.pinned, pinned_bytes), "pinned staging");
+      s.pinned_bytes = pinned_bytes;
+    }
+  }
+
+  // Launch one batch on an idle slot. Returns false if nothing could be admitted.
+  bool launch(Slot &slot) {
+    std::vector<std::shared_ptr<Request>> batch;
+    {
+      std::lock_guard<std::mutex> lock(mu);
+      if (pending.empty()) return false;
+      const auto head = pending.front();
+      for (auto it = pending.begin(); it != pending.end() && batch.size() < opt.max_batch;) {
+        const auto &r = *it;
+        if (r->params == head->params && r->op == head->op && r->level == head->level) {
+          batch.push_back(r);
+          it = pending.erase(it);
+        } else {
+          ++it;
+        }
+      }
+    }
+    const auto &h = *batch.front();
+    const auto pset = param_set(h.params);
+    const ParamSet &p = *pset;
+    LevelPlan *lp = nullptr;
+    try {
+      lp = &plan(h.params, h.level);
+    } catch (const std::exception &e) {
+      fail(batch, e.what());
+      return true;
+    }
+    // admission by device footprint: the whole budget minus the other arenas
+    const size_t others = arena_bytes - slot.device_bytes;
+    size_t b = batch.size();
+    while (b > 1 && others + footprint(p, *lp, h.op, h.level, b) > opt.hbm_budget_bytes) --b;
+    if (others + footprint(p, *lp, h.op, h.level, b) > opt.hbm_budget_bytes) {
+      // idle arenas of other slots hold budget without work: release them
+      for (Slot &o : slots)
+        if (&o != &slot && !o.busy && o.device) {
+          ok(ntt_hip_free(opt.device, o.device), "free");
+          arena_bytes -= o.device_bytes;
+          o.device = nullptr;
+          o.device_bytes = 0;
+        }
+    }
+    if (arena_bytes - slot.device_bytes + footprint(p, *lp, h.op, h.level, b) > opt.hbm_budget_bytes) {
+      const bool others_busy = std::any_of(slots.begin(), slots.end(), [&](const Slot &o) { return o.busy; });
+      if (!others_busy) {
+        fail(batch, "request does not fit the device budget");
+        return true;
+      }
+      requeue(batch);  // wait for in-flight batches to free memory
+      return false;
+    }
+    if (b < batch.size()) {
+      requeue(std::vector<std::shared_ptr<Request>>(batch.begin() + b, batch.end()));
+      batch.resize(b);
+    }
+    try {
+      const size_t iw = in_words(p, h.op, h.level), ow = out_words(p, h.op, h.level);
+      const size_t ow2 = h.op == Op::KeySwitch ? ow : 0;
+      const size_t dev = footprint(p, *lp, h.op, h.level, b);
+      ensure_arena(slot, dev, b * (iw + ow + ow2) * 8);
+      uint64_t *d_in = static_cast<uint64_t *>(slot.device), *d_out = d_in + b * iw, *d_out2 = d_out + b * ow;
+      void *d_ws = d_out2 + b * ow2;
+      uint64_t *h_in = static_cast<uint64_t *>(slot.pinned);
+      for (size_t i = 0; i < b; ++i) std::memcpy(h_in + i * iw, batch[i]->in.data(), iw * 8);
+      ok(ntt_hip_memcpy(opt.device, d_in, h_in, b * iw * 8, NTT_HIP_H2D, slot.stream), "h2d");
+      const size_t L = h.level, n = p.n;
+      switch (h.op) {
+        case Op::KeySwitch:
+          ok(ntt_hip_keyswitch_run(lp->ks, d_in, b, d_out, d_out2, d_ws, slot.stream), "keyswitch");
+          break;
+        case Op::Forward:
+        case Op::Inverse:
+          ok(ntt_hip_transform_batch(lp->ctx, d_out, d_in, b, L * n, L * n, 0, L, h.op == Op::Inverse, slot.stream),
+             "transform");
+          break;
+        case Op::ModUp:
+          ok(ntt_hip_modup_run(lp->ks, d_in, 1, b, d_out, d_ws, slot.stream), "modup");
+          break;
+        case Op::ModDown:
+          ok(ntt_hip_moddown_run(lp->ks, d_in, b, d_out, d_ws, slot.stream), "moddown");
+          break;
+      }
+      uint64_t *h_out = h_in + b * iw;
+      ok(ntt_hip_memcpy(opt.device, h_out, d_out, b * (ow + ow2) * 8, NTT_HIP_D2H, slot.stream), "d2h");
+      ok(ntt_hip_event_record(opt.device, slot.event, slot.stream), "record");
+      slot.batch = std::move(batch);
+      slot.in_words = iw;
+      slot.out_words = ow;
+      slot.out2_words = ow2;
+      slot.busy = true;
+      std::lock_guard<std::mutex> lock(mu);
+      ++stats.batches;
+      stats.device_bytes_in_flight = arena_bytes;
+      stats.peak_device_bytes = std::max(stats.peak_device_bytes, arena_bytes);
+      size_t busy = 0;
+      for (const Slot &s : slots) busy += s.busy;
+      stats.max_in_flight = std::max(stats.max_in_flight, busy);
+    } catch (const std::exception &e) {
+      (void)ntt_hip_stream_synchronize(opt.device, slot.stream);
+      fail(batch, e.what());
+    }
+    return true;
+  }
+
+  bool retire(Slot &slot) {
+    if (!slot.busy) return false;
+    const int q = ntt_hip_event_query(opt.device, slot.event);
+    if (q == NTT_HIP_NOT_READY) return false;
+    const auto now = Clock::now();
+    const size_t b = slot.batch.size();
+    const uint64_t *h_out = static_cast<const uint64_t *>(slot.pinned) + b * slot.in_words;
+    std::lock_guard<std::mutex> lock(mu);
+    for (size_t i = 0; i < b; ++i) {
+      Request &r = *slot.batch[i];
+      if (q != NTT_HIP_OK) {
+        r.error = std::string("HIP: batch failed: ") + ntt_hip_last_error();
+      } else {
+        r.result.out.assign(h_out + i * slot.out_words, h_out + (i + 1) * slot.out_words);
+        if (slot.out2_words) {
+          const uint64_t *o2 = h_out + b * slot.out_words;
+          r.result.out2.assign(o2 + i * slot.out2_words, o2 + (i + 1) * slot.out2_words);
+        }
+      }
+      r.result.latency_ms = std::chrono::duration<double, std::milli>(now - r.t0).count();
+      r.result.batch_size = b;
+      r.done = true;
+      ++stats.completed;
+    }
+    slot.batch.clear();
+    slot.busy = false;
+    cv_done.notify_all();
+    return true;
+  }
+
+  void fail(const std::vector<std::shared_ptr<Request>> &batch, const std::string &why) {
+    std::lock_guard<std::mutex> lock(mu);
+    for (const auto &r : batch) {
+      r->error = why;
+      r->done = true;
+      ++stats.completed;
+    }
+    cv_done.notify_all();
+  }
+
+  void requeue(const std::vector<std::shared_ptr<Request>> &reqs) {
+    std::lock_guard<std::mutex> lock(mu);
+    for (auto it = reqs.rbegin(); it != reqs.rend(); ++it) pending.push_front(*it);
+  }
+
+  void run() {
+    for (;;) {
+      bool progress = false;
+      for (Slot &s : slots) progress |= retire(s);
+      for (Slot &s : slots)
+        if (!s.busy) progress |= launch(s);
+      std::unique_lock<std::mutex> lock(mu);
+      const bool idle = std::none_of(slots.begin(), slots.end(), [](const Slot &s) { return s.busy; });
+      if (stop && pending.empty() && idle) return;
+      if (!progress) {
+        if (idle) cv_submit.wait_for(lock, std::chrono::milliseconds(5));
+        else cv_submit.wait_for(lock, std::chrono::microseconds(50));
+      }
+    }
+  }
+};
+
+Server::Server(Options opt) : impl_(std::make_unique<Impl>()) {
+  if (opt.streams < 1 || opt.max_batch < 1) throw std::invalid_argument("server needs >= 1 stream and batch");
+  impl_->opt = opt;
+  impl_->slots.resize(opt.streams);
+  for (Slot &s : impl_->slots) {
+    ok(ntt_hip_stream_create(opt.device, &s.stream), "stream");
+    ok(ntt_hip_event_create(opt.device, &s.event, 0), "event");
+  }
+  impl_->worker = std::thread([this] { impl_->run(); });
+}
+
+Server::~Server() {
+  {
+    std::lock_guard<std::mutex> lock(impl_->mu);
+    impl_->stop = true;
+  }
+  impl_->cv_submit.notify_all();
+  if (impl_->worker.joinable()) impl_->worker.join();
+  for (Slot &s : impl_->slots) {
+    if (s.stream) ntt_hip_stream_synchronize(impl_->opt.device, s.stream);
+    if (s.device) ntt_hip_free(impl_->opt.device, s.device);
+    if (s.pinned) ntt_hip_host_free(s.pinned);
+    if (s.event) ntt_hip_event_destroy(impl_->opt.device, s.event);
+    if (s.stream) ntt_hip_stream_destroy(impl_->opt.device, s.stream);
+  }
+  impl_->plans.clear();
+}
+
+uint32_t Server::add_params(const ParamSet &p) {
+  const size_t M = p.q.size(), K = p.p.size();
+  if (p.n < 2 || M == 0 || K == 0 || p.q_psi.size() != M || p.p_psi.size() != K || p.alpha == 0)
+    throw std::invalid_argument("parameter set needs n, M >= 1 Q towers, K >= 1 P towers and alpha >= 1");
+  const size_t words = ((M + p.alpha - 1) / p.alpha) * (M + K) * p.n;
+  if (p.key_b.size() != words || p.key_a.size() != words)
+    throw std::invalid_argument("evaluation key must hold ceil(M/alpha) digits of (M+K)*n residues per channel");
+  std::lock_guard<std::mutex> lock(impl_->mu);
+  impl_->params.push_back(std::make_shared<const ParamSet>(p));
+  return static_cast<uint32_t>(impl_->params.size() - 1);
+}
+
+size_t Server::output_words(uint32_t id, Op op, size_t level) const {
+  std::lock_guard<std::mutex> lock(impl_->mu);
+  if (id >= impl_->params.size()) throw std::invalid_argument("unknown parameter set");
+  return impl_->out_words(*impl_->params[id], op, level);
+}
+
+uint64_t Server::submit(uint32_t id, Op op, size_t level, std::span<const uint64_t> input) {
+  auto r = std::make_shared<Request>();
+  {
+    std::lock_guard<std::mutex> lock(impl_->mu);
+    if (impl_->stop) throw std::invalid_argument("server stopping");
+    if (id >= impl_->params.size()) throw std::invalid_argument("unknown parameter set");
+    const ParamSet &p = *impl_->params[id];
+    if (static_cast<int>(op) < 0 || static_cast<int>(op) > 4) throw std::invalid_argument("unknown operation");
+    if (level < 1 || level > p.q.size()) throw std::invalid_argument("level outside the parameter set");
+    if (input.size() != impl_->in_words(p, op, level)) throw std::invalid_argument("input length mismatch");
+    const size_t towers = input.size() / p.n;
+    for (size_t t = 0; t < towers; ++t) {
+      const uint64_t q = t < level ? p.q[t] : p.p[t - level];
+      for (size_t i = 0; i < p.n; ++i)
+        if (input[t * p.n + i] >= q) throw std::invalid_argument("noncanonical input residue");
+    }
+    r->ticket = impl_->next_ticket++;
+    r->params = id;
+    r->op = op;
+    r->level = level;
+    r->in.assign(input.begin(), input.end());
+    r->t0 = Clock::now();
+    impl_->pending.push_back(r);
+    impl_->tickets.emplace(r->ticket, r);
+    ++impl_->stats.submitted;
+  }
+  impl_->cv_submit.notify_one();
+  return r->ticket;
+}
+
+bool Server::ready(uint64_t ticket) const {
+  std::lock_guard<std::mutex> lock(impl_->mu);
+  auto it = impl_->tickets.find(ticket);
+  if (it == impl_->tickets.end()) throw std::invalid_argument("unknown ticket");
+  return it->second->done;
+}
+
+Result Server::wait(uint64_t ticket) {
+  std::unique_lock<std::mutex> lock(impl_->mu);
+  auto it = impl_->tickets.find(ticket);
+  if (it == impl_->tickets.end()) throw std::invalid_argument("unknown ticket");
+  auto r = it->second;
+  impl_->cv_done.wait(lock, [&] { return r->done; });
+  impl_->tickets.erase(it);
+  if (!r->error.empty()) throw std::runtime_error(r->error);
+  return std::move(r->result);
+}
+
+Stats Server::stats() const {
+  std::lock_guard<std::mutex> lock(impl_->mu);
+  return impl_->stats;
+}
+
+}  // namespace ntt::serve
+
+// ---------------------------------------------------------------- C ABI
+namespace {
+thread_local std::string g_serve_error;
+template <class F>
+int serve_guard(F &&f) {
+  try {
+    return f();
+  } catch (const std::invalid_argument &e) {
+    g_serve_error = e.what();
+    return NTT_HIP_EINVAL;
+  } catch (const std::exception &e) {
+    g_serve_error = e.what();
+    return NTT_HIP_ERUNTIME;
+  }
+}
+}  // namespace
+
+struct ntt_serve {
+  ntt::serve::Server server;
+  explicit ntt_serve(ntt::serve::Options o) : server(o) {}
+};
+
+extern "C" {
+
+const char *ntt_serve_last_error(void) { return g_serve_error.c_str(); }
+
+int ntt_serve_create(ntt_serve **out, int device, size_t hbm_budget_bytes, size_t streams, size_t max_batch) {
+  return serve_guard([&] {
+    if (!out) throw std::invalid_argument("null server out");
+    *out = new ntt_serve(ntt::serve::Options{device, hbm_budget_bytes, streams, max_batch});
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_serve_destroy(ntt_serve *s) {
+  return serve_guard([&] {
+    delete s;
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_serve_add_params(ntt_serve *s, uint64_t n, const uint64_t *q, const uint64_t *q_psi, size_t M,
+                         const uint64_t *p, const uint64_t *p_psi, size_t K, size_t alpha, const uint64_t *key_b,
+                         const uint64_t *key_a, uint32_t *id) {
+  return serve_guard([&] {
+    if (!s || !q || !q_psi || !p || !p_psi || !key_b || !key_a || !id) throw std::invalid_argument("null argument");
+    ntt::serve::ParamSet ps;
+    ps.n = n;
+    ps.q.assign(q, q + M);
+    ps.q_psi.assign(q_psi, q_psi + M);
+    ps.p.assign(p, p + K);
+    ps.p_psi.assign(p_psi, p_psi + K);
+    ps.alpha = alpha;
+    const size_t words = alpha ? ((M + alpha - 1) / alpha) * (M + K) * n : 0;
+    ps.key_b.assign(key_b, key_b + words);
+    ps.key_a.assign(key_a, key_a + words);
+    *id = s->server.add_params(ps);
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_serve_output_words(ntt_serve *s, uint32_t params, int op, size_t level, size_t *words) {
+  return serve_guard([&] {
+    if (!s || !words) throw std::invalid_argument("null argument");
+    *words = s->server.output_words(params, static_cast<ntt::serve::Op>(op), level);
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_serve_submit(ntt_serve *s, uint32_t params, int op, size_t level, const uint64_t *in, size_t words,
+                     uint64_t *ticket) {
+  return serve_guard([&] {
+    if (!s || !in || !ticket) throw std::invalid_argument("null argument");
+    *ticket = s->server.submit(params, static_cast<ntt::serve::Op>(op), level, std::span<const uint64_t>(in, words));
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_serve_ready(ntt_serve *s, uint64_t ticket) {
+  return serve_guard([&] {
+    if (!s) throw std::invalid_argument("null server");
+    return s->server.ready(ticket) ? NTT_HIP_OK : NTT_HIP_NOT_READY;
+  });
+}
+
+int ntt_serve_wait(ntt_serve *s, uint64_t ticket, uint64_t *out, size_t out_words, uint64_t *out2, size_t out2_words,
+                   double *latency_ms, size_t *batch_size) {
+  return serve_guard([&] {
+    if (!s || !out) throw std::invalid_argument("null argument");
+    auto r = s->server.wait(ticket);
+    if (r.out.size() != out_words || r.out2.size() != (out2 ? out2_words : r.out2.size()))
+      throw std::invalid_argument("output buffer length mismatch");
+    std::copy(r.out.begin(), r.out.end(), out);
+    if (out2) std::copy(r.out2.begin(), r.out2.end(), out2);
+    if (latency_ms) *latency_ms = r.latency_ms;
+    if (batch_size) *batch_size = r.batch_size;
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_serve_stats(ntt_serve *s, size_t *out6) {
+  return serve_guard([&] {
+    if (!s || !out6) throw std::invalid_argument("null argument");
+    const auto st = s->server.stats();
+    const size_t v[6] = {st.submitted, st.completed, st.batches, st.max_in_flight, st.device_bytes_in_flight,
+                         st.peak_device_bytes};
+    std::copy(v, v + 6, out6);
+    return NTT_HIP_OK;
+  });
+}
+
+}  // extern "C"

@@ -41,6 +41,7 @@ SIGNATURES = {
     "ntt_hip_event_query": [INT, VP],
     "ntt_hip_event_synchronize": [INT, VP],
     "ntt_hip_event_elapsed": [ctypes.POINTER(ctypes.c_float), VP, VP],
+    "ntt_hip_copy_words": [INT, VP, VP, SZ, VP],
     "ntt_hip_context_create": [ctypes.POINTER(VP), INT, U64, U64P, U64P, SZ],
     "ntt_hip_context_destroy": [VP],
     "ntt_hip_context_stream": [VP],

@@ -1,0 +1,78 @@
+"""Continuous batching: a mixed stream of KeySwitch, NTT/INTT and ModUp/ModDown
+requests of different N, L and dnum submitted while earlier batches run.
+Every response equals the same request served alone (batch of one, one
+stream), and responses are checked bit-exactly against the oracle."""
+import numpy as np
+import pytest
+
+import oracle as o
+from common import log_case
+from nttgpu import serve, workload
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle(req, hp, kb, ka):
+    n, L = hp.n, req.level
+    if req.op == serve.KEYSWITCH:
+        return o.key_switch(req.data, n, hp.q, hp.q_psi, hp.M, hp.p, hp.p_psi, hp.alpha, kb, ka)
+    if req.op in (serve.FORWARD, serve.INVERSE):
+        return o.ntt_towers(req.data, n, hp.q, hp.q_psi, inverse=req.op == serve.INVERSE), None
+    if req.op == serve.MODUP:
+        return o.mod_up(req.data, n, hp.q, hp.q_psi, hp.p, hp.p_psi, evaluation=True), None
+    return o.mod_down(req.data, n, hp.q, hp.q_psi, hp.p, hp.p_psi), None
+
+
+def test_mixed_stream_alone_equals_in_mix():
+    reqs, pools = workload.stream(48, rate_per_s=4000.0, seed=5, sizes=(8192, 16384), levels=(3, 12))
+    log_case(f"serve mixed stream: {len(reqs)} requests")
+    mixed_srv = serve.Server(streams=4, max_batch=8, hbm_budget_bytes=8 << 30)
+    mixed = workload.MixedServer(mixed_srv, pools)
+    for r in reqs:  # register and warm every parameter set outside the stream
+        mixed.params_for(r)
+    tickets = [mixed.submit(r) for r in reqs]  # back-to-back: later ones arrive while earlier run
+    got = [mixed.wait(t, r) for t, r in zip(tickets, reqs)]
+    stats = mixed_srv.stats()
+    assert stats["completed"] == len(reqs)
+    assert stats["batches"] < len(reqs), "requests of one group must share launches"
+    assert any(bs > 1 for _, _, bs in got)
+
+    alone_srv = serve.Server(streams=1, max_batch=1, hbm_budget_bytes=8 << 30)
+    alone = workload.MixedServer(alone_srv, pools)
+    for i, r in enumerate(reqs):
+        log_case(f"serve alone {i} op={serve.OP_NAMES[r.op]} n={r.n} L={r.level} dnum={r.dnum}")
+        (a1, a2), _, bs = alone.wait(alone.submit(r), r)
+        (m1, m2), _, _ = got[i]
+        assert bs == 1
+        assert np.array_equal(a1, m1), f"request {i}: mixed result differs from alone"
+        if r.op == serve.KEYSWITCH:
+            assert np.array_equal(a2, m2)
+        hp, kb, ka = mixed.sets[(r.n, r.level, r.dnum)]
+        w1, w2 = _oracle(r, hp, kb, ka)
+        assert np.array_equal(m1, w1), f"request {i} ({serve.OP_NAMES[r.op]}) differs from the oracle"
+        if w2 is not None:
+            assert np.array_equal(m2, w2)
+    mixed_srv.close()
+    alone_srv.close()
+
+
+def test_admission_by_footprint_queues_instead_of_failing():
+    """A budget that fits one key-switch batch at a time still serves every request."""
+    reqs, pools = workload.stream(12, rate_per_s=1e6, seed=9, sizes=(8192,), levels=(6, 6), dnums=(3,))
+    for r in reqs:
+        r.op = serve.KEYSWITCH
+        r.data = r.data[: r.level * r.n]
+    srv = serve.Server(streams=4, max_batch=4, hbm_budget_bytes=64 << 20)
+    ms = workload.MixedServer(srv, pools)
+    for r in reqs:
+        ms.params_for(r)
+    log_case("serve admission under a 64 MiB budget")
+    tickets = [ms.submit(r) for r in reqs]
+    outs = [ms.wait(t, r) for t, r in zip(tickets, reqs)]
+    st = srv.stats()
+    assert st["completed"] == len(reqs) and st["peak_device_bytes"] <= 64 << 20
+    hp, kb, ka = ms.sets[(reqs[0].n, reqs[0].level, reqs[0].dnum)]
+    for r, ((b, a), _, _) in zip(reqs[:3], outs[:3]):
+        wb, wa = o.key_switch(r.data, hp.n, hp.q, hp.q_psi, hp.M, hp.p, hp.p_psi, hp.alpha, kb, ka)
+        assert np.array_equal(b, wb) and np.array_equal(a, wa)
+    srv.close()

@@ -81,6 +81,14 @@ int ntt_hip_context_device(ntt_hip_context *ctx);
  * resident_driver.hpp:28 / resident_adapter.inc:171-174) */
 size_t ntt_hip_context_table_bytes(ntt_hip_context *ctx);
 
+/* ---- autotuner (no reference counterpart): times candidate launch
+ * geometries on `towers` scratch towers and keeps the fastest in the
+ * context — the NTT tower chunk (towers per pass-A/pass-B pair; 0 = all) and
+ * basis-conversion targets per workgroup. `report` receives a JSON line. ---- */
+int ntt_hip_autotune(ntt_hip_context *ctx, size_t towers, char *report, size_t capacity);
+int ntt_hip_set_tuning(ntt_hip_context *ctx, uint32_t ntt_chunk, uint32_t conv_targets);
+int ntt_hip_get_tuning(ntt_hip_context *ctx, uint32_t *ntt_chunk, uint32_t *conv_targets);
+
 /* ---- Driver seam operations (include/ntt/resident_driver.hpp:13-57) on
  * contiguous towers [first, first + towers) of the context basis ---- */
 /* Driver::transform (resident_adapter.inc:198-203): in place on `data`;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -168,6 +169,8 @@ struct ntt_hip_context {
   uint32_t n = 0, log_n = 0;
   std::vector<uint64_t> q, psi;
   bool lazy = true;
+  uint32_t ntt_chunk = 0;      // towers per NTT pass pair (0 = all), set by the autotuner
+  uint32_t conv_targets = 8;   // basis-conversion targets per workgroup, set by the autotuner
   hipStream_t stream = nullptr;
   std::unique_ptr<DeviceBlob> tables;  // twiddles of every tower
   std::unique_ptr<DeviceBlob> primes;  // PrimeConst[towers]
@@ -600,7 +603,7 @@ int ntt_hip_transform_batch(ntt_hip_context *ctx, uint64_t *data, const uint64_t
     DeviceGuard g(ctx->device);
     const TowerMap m = affine(data, src, ctx->n, towers, first, poly_stride, src_poly_stride);
     hip_ok(launch_ntt(m, ctx->dprimes(), ctx->log_n, (uint32_t)(polys * towers), inverse != 0, ctx->lazy,
-                      ctx->pick(stream)),
+                      ctx->pick(stream), ctx->ntt_chunk),
            "ntt launch");
     return NTT_HIP_OK;
   });
@@ -628,7 +631,7 @@ int ntt_hip_switch_basis(ntt_hip_context *ctx, uint64_t *out, const uint64_t *in
     a.in = in;
     a.out = out;
     a.n = ctx->n;
-    a.targets_per_block = (uint32_t)std::min<size_t>(t, 8);
+    a.targets_per_block = (uint32_t)std::min<size_t>(t, ctx->conv_targets);
     a.plan_host_s = plan->s;
     a.plan_host_t = plan->t;
     a.skip_at = 0xffffffffu;
@@ -844,7 +847,7 @@ int ntt_hip_keyswitch_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batc
     const PrimeConst *pr = c.dprimes();
     DeviceGuard g(c.device);
     // 1. ModUp sources: coefficient form of every Q tower (INTT, out of place)
-    hip_ok(launch_ntt(affine(coeff, in, n, L, 0, L * n, L * n), pr, c.log_n, (uint32_t)(B * L), true, c.lazy, s),
+    hip_ok(launch_ntt(affine(coeff, in, n, L, 0, L * n, L * n), pr, c.log_n, (uint32_t)(B * L), true, c.lazy, s, c.ntt_chunk),
            "intt");
     // 2. per digit: unreduced CRT into the complement, around the digit's own towers
     for (uint32_t d = 0; d < ks->beta; ++d) {
@@ -857,7 +860,7 @@ int ntt_hip_keyswitch_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batc
       a.in_poly_stride = ks->coeff_words();
       a.out_poly_stride = ks->digit_words();
       a.n = (uint32_t)n;
-      a.targets_per_block = 8;
+      a.targets_per_block = c.conv_targets;
       a.plan_host_s = plan.s;
       a.plan_host_t = plan.t;
       a.skip_at = first;
@@ -870,7 +873,7 @@ int ntt_hip_keyswitch_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batc
       m.base = digits;
       m.jobs = (const TowerJob *)ks->jobs->ptr;
       m.tower_stride = n;
-      hip_ok(launch_ntt(m, pr, c.log_n, (uint32_t)(B * ks->jobs_per_poly), false, c.lazy, s), "modup ntt");
+      hip_ok(launch_ntt(m, pr, c.log_n, (uint32_t)(B * ks->jobs_per_poly), false, c.lazy, s, c.ntt_chunk), "modup ntt");
     }
     // 4. key inner product over all digits, both channels
     {
@@ -895,7 +898,7 @@ int ntt_hip_keyswitch_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batc
     }
     // 5. ModDown of both channels (2B polynomials): INTT(P) -> CRT to Q -> NTT -> correct
     hip_ok(launch_ntt(affine(pc, z + L * n, n, K, L, K * n, T * n), pr, c.log_n, (uint32_t)(2 * B * K), true, c.lazy,
-                      s),
+                      s, c.ntt_chunk),
            "moddown intt");
     {
       BasisLaunch a{};
@@ -905,14 +908,14 @@ int ntt_hip_keyswitch_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batc
       a.in_poly_stride = K * n;
       a.out_poly_stride = L * n;
       a.n = (uint32_t)n;
-      a.targets_per_block = 8;
+      a.targets_per_block = c.conv_targets;
       a.plan_host_s = ks->down->s;
       a.plan_host_t = ks->down->t;
       a.skip_at = 0xffffffffu;
       a.skip_len = 0;
       hip_ok(launch_basis_switch(a, ks->down->per_term, (uint32_t)(2 * B), s), "moddown conversion");
     }
-    hip_ok(launch_ntt(affine(corr, nullptr, n, L, 0, L * n, 0), pr, c.log_n, (uint32_t)(2 * B * L), false, c.lazy, s),
+    hip_ok(launch_ntt(affine(corr, nullptr, n, L, 0, L * n, 0), pr, c.log_n, (uint32_t)(2 * B * L), false, c.lazy, s, c.ntt_chunk),
            "moddown ntt");
     for (int ch = 0; ch < 2; ++ch) {
       CorrectLaunch cl{};
@@ -998,7 +1001,7 @@ int ntt_hip_graph_destroy(void *graph) {
 // ---------------------------------------------------------------- batched ModUp / ModDown
 namespace {
 
-void launch_basis(const BasisPlanDev &plan, const uint64_t *in, uint64_t *out, uint64_t in_stride,
+void launch_basis(const BasisPlanDev &plan, uint32_t targets, const uint64_t *in, uint64_t *out, uint64_t in_stride,
                   uint64_t out_stride, uint32_t n, uint32_t polys, hipStream_t s) {
   BasisLaunch a{};
   a.plan = plan.dev;
@@ -1007,7 +1010,7 @@ void launch_basis(const BasisPlanDev &plan, const uint64_t *in, uint64_t *out, u
   a.in_poly_stride = in_stride;
   a.out_poly_stride = out_stride;
   a.n = n;
-  a.targets_per_block = 8;
+  a.targets_per_block = targets;
   a.plan_host_s = plan.s;
   a.plan_host_t = plan.t;
   a.skip_at = 0xffffffffu;
@@ -1040,18 +1043,18 @@ int ntt_hip_modup_run(ntt_hip_keyswitch *ks, const uint64_t *in, int in_eval, si
     DeviceGuard g(c.device);
     if (in_eval) {
       // Q part of the output is the input itself; coefficients for the lift
-      hip_ok(launch_ntt(affine(coeff, in, n, L, 0, L * n, L * n), pr, c.log_n, (uint32_t)(batch * L), true, c.lazy, s),
+      hip_ok(launch_ntt(affine(coeff, in, n, L, 0, L * n, L * n), pr, c.log_n, (uint32_t)(batch * L), true, c.lazy, s, c.ntt_chunk),
              "modup intt");
       for (size_t b = 0; b < batch; ++b)
         HIP_CHECK(hipMemcpyAsync(out + b * T * n, in + b * L * n, L * n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
-      launch_basis(*ks->full_up, coeff, out + L * n, L * n, T * n, (uint32_t)n, (uint32_t)batch, s);
+      launch_basis(*ks->full_up, c.conv_targets, coeff, out + L * n, L * n, T * n, (uint32_t)n, (uint32_t)batch, s);
     } else {
-      hip_ok(launch_ntt(affine(out, in, n, L, 0, T * n, L * n), pr, c.log_n, (uint32_t)(batch * L), false, c.lazy, s),
+      hip_ok(launch_ntt(affine(out, in, n, L, 0, T * n, L * n), pr, c.log_n, (uint32_t)(batch * L), false, c.lazy, s, c.ntt_chunk),
              "modup ntt q");
-      launch_basis(*ks->full_up, in, out + L * n, L * n, T * n, (uint32_t)n, (uint32_t)batch, s);
+      launch_basis(*ks->full_up, c.conv_targets, in, out + L * n, L * n, T * n, (uint32_t)n, (uint32_t)batch, s);
     }
     hip_ok(launch_ntt(affine(out + L * n, nullptr, n, K, L, T * n, 0), pr, c.log_n, (uint32_t)(batch * K), false,
-                      c.lazy, s),
+                      c.lazy, s, c.ntt_chunk),
            "modup ntt p");
     return NTT_HIP_OK;
   });
@@ -1078,10 +1081,10 @@ int ntt_hip_moddown_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batch,
     uint64_t *corr = pc + batch * K * n;   // batch * L * n
     DeviceGuard g(c.device);
     hip_ok(launch_ntt(affine(pc, in + L * n, n, K, L, K * n, T * n), pr, c.log_n, (uint32_t)(batch * K), true, c.lazy,
-                      s),
+                      s, c.ntt_chunk),
            "moddown intt");
-    launch_basis(*ks->down, pc, corr, K * n, L * n, (uint32_t)n, (uint32_t)batch, s);
-    hip_ok(launch_ntt(affine(corr, nullptr, n, L, 0, L * n, 0), pr, c.log_n, (uint32_t)(batch * L), false, c.lazy, s),
+    launch_basis(*ks->down, c.conv_targets, pc, corr, K * n, L * n, (uint32_t)n, (uint32_t)batch, s);
+    hip_ok(launch_ntt(affine(corr, nullptr, n, L, 0, L * n, 0), pr, c.log_n, (uint32_t)(batch * L), false, c.lazy, s, c.ntt_chunk),
            "moddown ntt");
     CorrectLaunch cl{};
     cl.out = out;
@@ -1110,3 +1113,119 @@ extern "C" int ntt_hip_copy_words(int device, uint64_t *dst, const uint64_t *src
     return NTT_HIP_OK;
   });
 }
+
+// ---------------------------------------------------------------- autotuner
+// Times candidate launch geometries on scratch data of this context and keeps
+// the fastest: the NTT tower chunk (how many towers one pass-A/pass-B pair
+// covers before the next group starts, trading launch count for L2/Infinity
+// Cache reuse of the inter-pass hand-off) and the basis-conversion targets per
+// workgroup (parallelism vs. recomputing the scaled sources).
+namespace {
+float time_ms(ntt_hip_context &c, const std::function<void()> &body, int reps) {
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  body();  // warm
+  HIP_CHECK(hipEventRecord(e0, c.stream));
+  for (int r = 0; r < reps; ++r) body();
+  HIP_CHECK(hipEventRecord(e1, c.stream));
+  HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms / reps;
+}
+}  // namespace
+
+extern "C" {
+
+int ntt_hip_set_tuning(ntt_hip_context *ctx, uint32_t ntt_chunk, uint32_t conv_targets) {
+  return guarded([&] {
+    require(ctx && conv_targets >= 1, "invalid tuning");
+    ctx->ntt_chunk = ntt_chunk;
+    ctx->conv_targets = conv_targets;
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_get_tuning(ntt_hip_context *ctx, uint32_t *ntt_chunk, uint32_t *conv_targets) {
+  return guarded([&] {
+    require(ctx && ntt_chunk && conv_targets, "null argument");
+    *ntt_chunk = ctx->ntt_chunk;
+    *conv_targets = ctx->conv_targets;
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_autotune(ntt_hip_context *ctx, size_t towers, char *report, size_t capacity) {
+  return guarded([&] {
+    require(ctx && towers >= 1 && towers <= 65535, "autotune needs a context and 1..65535 towers");
+    ntt_hip_context &c = *ctx;
+    const size_t n = c.n;
+    DeviceGuard g(c.device);
+    DeviceBlob scratch(c.device, towers * n * sizeof(uint64_t) * 2);
+    uint64_t *a = (uint64_t *)scratch.ptr, *b = a + towers * n;
+    HIP_CHECK(hipMemsetAsync(a, 0, towers * n * sizeof(uint64_t) * 2, c.stream));
+    // NTT chunk: every tower uses context tower 0 (one prime, as a batch does)
+    std::string rep = "{\"ntt\":{";
+    uint32_t best_chunk = 0;
+    float best = 1e30f;
+    for (uint32_t chunk : {0u, 1024u, 512u, 256u, 128u, 64u, 32u}) {
+      if (chunk && chunk >= towers) continue;
+      TowerMap m = affine(a, nullptr, n, 1, 0, n, 0);
+      const float ms = time_ms(c, [&] {
+        hip_ok(launch_ntt(m, c.dprimes(), c.log_n, (uint32_t)towers, false, c.lazy, c.stream, chunk), "tune ntt");
+        hip_ok(launch_ntt(m, c.dprimes(), c.log_n, (uint32_t)towers, true, c.lazy, c.stream, chunk), "tune ntt");
+      }, 3);
+      rep += "\"" + std::to_string(chunk) + "\":" + std::to_string(ms) + ",";
+      if (ms < best * 0.98f) {  // prefer fewer launches unless clearly faster
+        best = ms;
+        best_chunk = chunk;
+      }
+    }
+    rep.back() = '}';
+    // conversion geometry: first half of the basis -> second half (<= 6 -> <= 24)
+    uint32_t best_targets = c.conv_targets;
+    if (c.q.size() >= 2) {
+      const size_t s = std::min<size_t>(6, c.q.size() / 2), t = std::min<size_t>(24, c.q.size() - s);
+      std::vector<size_t> src, dst;
+      for (size_t i = 0; i < s; ++i) src.push_back(i);
+      for (size_t i = 0; i < t; ++i) dst.push_back(s + i);
+      auto plan = basis_plan(c, src, dst);
+      const uint32_t polys = (uint32_t)std::max<size_t>(1, std::min(towers / t, towers / s));
+      rep += ",\"conversion\":{";
+      best = 1e30f;
+      for (uint32_t tpb : {1u, 2u, 4u, 6u, 8u, 12u, 16u, 24u}) {
+        if (tpb > t) continue;
+        BasisLaunch bl{};
+        bl.plan = plan->dev;
+        bl.in = a;
+        bl.out = b;
+        bl.in_poly_stride = s * n;
+        bl.out_poly_stride = t * n;
+        bl.n = (uint32_t)n;
+        bl.targets_per_block = tpb;
+        bl.plan_host_s = plan->s;
+        bl.plan_host_t = plan->t;
+        bl.skip_at = 0xffffffffu;
+        const float ms = time_ms(c, [&] { hip_ok(launch_basis_switch(bl, plan->per_term, polys, c.stream), "tune conv"); }, 3);
+        rep += "\"" + std::to_string(tpb) + "\":" + std::to_string(ms) + ",";
+        if (ms < best) {
+          best = ms;
+          best_targets = tpb;
+        }
+      }
+      rep.back() = '}';
+    }
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    c.ntt_chunk = best_chunk;
+    c.conv_targets = best_targets;
+    rep += ",\"chosen\":{\"ntt_chunk\":" + std::to_string(best_chunk) + ",\"conv_targets\":" +
+           std::to_string(best_targets) + "},\"towers\":" + std::to_string(towers) + ",\"n\":" + std::to_string(n) + "}";
+    if (report && capacity) std::snprintf(report, capacity, "%s", rep.c_str());
+    return NTT_HIP_OK;
+  });
+}
+
+}  // extern "C"

@@ -35,9 +35,10 @@ struct TowerMap {
   const TowerJob *jobs;
   uint64_t poly_stride, tower_stride, src_poly_stride;
   uint32_t towers_per_poly, prime_offset;
+  uint32_t y_offset;  // tower index of blockIdx.y == 0 (chunked launches)
 };
 
 hipError_t launch_ntt(const TowerMap &map, const PrimeConst *primes, uint32_t log_n, uint32_t towers, bool inverse,
-                      bool lazy, hipStream_t stream);
+                      bool lazy, hipStream_t stream, uint32_t chunk = 0);
 
 }  // namespace nttamd

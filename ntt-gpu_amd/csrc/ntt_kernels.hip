@@ -106,6 +106,7 @@ __device__ __forceinline__ void reg_stages(uint64_t (&v)[16], const ulonglong2 *
 // place or when this is not the first pass) and prime.
 __device__ __forceinline__ void tower_of(const TowerMap &m, uint32_t y, bool first_pass, uint64_t *&data,
                                          const uint64_t *&src, uint32_t &prime) {
+  y += m.y_offset;
   if (m.jobs) {
     const TowerJob j = m.jobs[y];
     data = m.base + j.offset;
@@ -313,13 +314,25 @@ static hipError_t launch_t(const TowerMap &map, const PrimeConst *primes, uint32
   return hipGetLastError();
 }
 
+// `chunk` > 0 splits the towers into groups run pass after pass (A then B for
+// one group before the next group), so a group's intermediate hand-off stays
+// in L2 / Infinity Cache instead of a full HBM round trip.
 hipError_t launch_ntt(const TowerMap &map, const PrimeConst *primes, uint32_t log_n, uint32_t towers, bool inverse,
-                      bool lazy, hipStream_t stream) {
+                      bool lazy, hipStream_t stream, uint32_t chunk) {
   if (towers == 0) return hipSuccess;
-  if (inverse) return lazy ? launch_t<true, true>(map, primes, log_n, towers, stream)
-                           : launch_t<true, false>(map, primes, log_n, towers, stream);
-  return lazy ? launch_t<false, true>(map, primes, log_n, towers, stream)
-              : launch_t<false, false>(map, primes, log_n, towers, stream);
+  const uint32_t step = chunk ? chunk : towers;
+  for (uint32_t first = 0; first < towers; first += step) {
+    TowerMap m = map;
+    m.y_offset = map.y_offset + first;
+    const uint32_t count = towers - first < step ? towers - first : step;
+    hipError_t e;
+    if (inverse) e = lazy ? launch_t<true, true>(m, primes, log_n, count, stream)
+                          : launch_t<true, false>(m, primes, log_n, count, stream);
+    else e = lazy ? launch_t<false, true>(m, primes, log_n, count, stream)
+                  : launch_t<false, false>(m, primes, log_n, count, stream);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace nttamd

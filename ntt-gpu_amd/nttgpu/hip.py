@@ -47,6 +47,9 @@ SIGNATURES = {
     "ntt_hip_context_stream": [VP],
     "ntt_hip_context_device": [VP],
     "ntt_hip_context_table_bytes": [VP],
+    "ntt_hip_autotune": [VP, SZ, ctypes.c_char_p, SZ],
+    "ntt_hip_set_tuning": [VP, ctypes.c_uint32, ctypes.c_uint32],
+    "ntt_hip_get_tuning": [VP, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)],
     "ntt_hip_transform": [VP, VP, VP, SZ, SZ, INT, VP],
     "ntt_hip_transform_batch": [VP, VP, VP, SZ, SZ, SZ, SZ, SZ, INT, VP],
     "ntt_hip_pointwise": [VP, VP, VP, SZ, SZ, VP],
@@ -362,3 +365,21 @@ class KeySwitch:
             self.close()
         except Exception:
             pass
+
+
+def autotune(ctx, towers):
+    """Run the device autotuner for `towers` towers; returns its JSON report."""
+    import json
+    buf = ctypes.create_string_buffer(4096)
+    check(lib().ntt_hip_autotune(ctx.handle, towers, buf, 4096), "autotune")
+    return json.loads(buf.value.decode())
+
+
+def set_tuning(ctx, ntt_chunk, conv_targets):
+    check(lib().ntt_hip_set_tuning(ctx.handle, ntt_chunk, conv_targets), "set_tuning")
+
+
+def get_tuning(ctx):
+    a, b = ctypes.c_uint32(0), ctypes.c_uint32(0)
+    check(lib().ntt_hip_get_tuning(ctx.handle, ctypes.byref(a), ctypes.byref(b)), "get_tuning")
+    return {"ntt_chunk": a.value, "conv_targets": b.value}

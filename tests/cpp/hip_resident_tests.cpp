@@ -22,6 +22,26 @@ void check(bool ok, const std::string &why) {
   ++checks;
   if (!ok) throw std::runtime_error(why);
 }
+// Equality of residue arrays; the failure names the count and first differing
+// (tower, index) so a device mismatch says which stage and tower went wrong.
+// `got` is a download's span: it must be compared while that download lives,
+// so callers pass `s.download(x).wait_values()` inside this call's expression.
+void same(std::span<const uint64_t> got, const std::vector<uint64_t> &want, size_t n, const std::string &why) {
+  size_t bad = 0, first = 0;
+  for (size_t i = 0; i < std::min(got.size(), want.size()); ++i)
+    if (got[i] != want[i] && bad++ == 0) first = i;
+  if (got.size() == want.size() && bad == 0) {
+    ++checks;
+    return;
+  }
+  std::string detail = " (" + std::to_string(bad) + " of " + std::to_string(want.size()) + " differ";
+  if (bad)
+    detail += "; first at tower " + std::to_string(first / n) + " index " + std::to_string(first % n) + ": got " +
+              std::to_string(got[first]) + " want " + std::to_string(want[first]);
+  if (got.size() != want.size()) detail += "; size " + std::to_string(got.size()) + " vs " + std::to_string(want.size());
+  check(false, why + detail + ")");
+}
+
 template <class F>
 void rejects(F f, const char *why) {
   ++checks;
@@ -91,18 +111,25 @@ void session_chain_and_conversion() {
   or_ntt_towers(fb.data(), N, 5, qm.data(), qr.data(), 0);
   for (size_t i = 0; i < fa.size(); ++i) fa[i] = ntt::mul_mod(fa[i], fb[i], kQ[i / N]);
   or_ntt_towers(fa.data(), N, 5, qm.data(), qr.data(), 1);
-  const auto got = prod.wait_values();
-  check(std::equal(got.begin(), got.end(), fa.begin()), "device ring product equals the oracle");
+  same(prod.wait_values(), fa, N, "device ring product equals the oracle");
+  // ModUp stages one by one: Q -> P basis switch of the coefficients, then the
+  // P-tower forward transform (towers 5..7 of the session basis)
+  PolynomialLayout p_coeff = base;
+  p_coeff.towers.erase(p_coeff.towers.begin(), p_coeff.towers.begin() + 5);
+  std::vector<uint64_t> want_p(3 * N);
+  check(or_basis_switch(a.data(), N, qm.data(), 5, pm.data(), 3, want_p.data()) == 0, "oracle basis switch ran");
+  const auto dp = s.switch_basis(da, p_coeff);
+  same(s.download(dp).wait_values(), want_p, N, "device Q->P basis switch equals the oracle");
+  or_ntt_towers(want_p.data(), N, 3, pm.data(), pr.data(), 0);
+  same(s.download(s.forward(dp)).wait_values(), want_p, N, "device P-tower forward equals the oracle");
   // ModUp (coefficient input) then ModDown on the device
   auto up = s.mod_up(da);
   std::vector<uint64_t> want_up(8 * N), want_down(5 * N);
   or_mod_up(a.data(), 0, N, qm.data(), qr.data(), 5, pm.data(), pr.data(), 3, want_up.data());
-  const auto gu = s.download(up).wait_values();
-  check(std::equal(gu.begin(), gu.end(), want_up.begin()), "device ModUp equals the oracle");
+  same(s.download(up).wait_values(), want_up, N, "device ModUp equals the oracle");
   or_mod_down(want_up.data(), N, qm.data(), qr.data(), 5, pm.data(), pr.data(), 3, want_down.data());
-  const auto gd = s.download(s.mod_down(up)).wait_values();
-  check(std::equal(gd.begin(), gd.end(), want_down.begin()), "device ModDown equals the oracle");
-  check(s.stats().uploads == 2 && s.stats().downloads == 3, "no hidden transfers");
+  same(s.download(s.mod_down(up)).wait_values(), want_down, N, "device ModDown equals the oracle");
+  check(s.stats().uploads == 2 && s.stats().downloads == 5, "no hidden transfers");
   s.close();
 }
 
@@ -134,9 +161,8 @@ void session_key_switch(size_t L) {
   check(or_key_switch(c.data(), N, qm.data(), qr.data(), L, M, pm.data(), pr.data(), K, alpha, kb.data(), ka.data(),
                       wb.data(), wa.data()) == 0,
         "oracle ran");
-  const auto gb = ob.wait_values(), ga = oa.wait_values();
-  check(std::equal(gb.begin(), gb.end(), wb.begin()), "device Session key switch B equals the oracle, L=" + std::to_string(L));
-  check(std::equal(ga.begin(), ga.end(), wa.begin()), "device Session key switch A equals the oracle, L=" + std::to_string(L));
+  same(ob.wait_values(), wb, N, "device Session key switch B equals the oracle, L=" + std::to_string(L));
+  same(oa.wait_values(), wa, N, "device Session key switch A equals the oracle, L=" + std::to_string(L));
   s.close();
 }
 

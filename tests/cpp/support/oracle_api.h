@@ -8,6 +8,8 @@ int or_key_switch(const uint64_t *in, uint64_t n, const uint64_t *q, const uint6
                   const uint64_t *p, const uint64_t *ppsi, uint64_t K, uint64_t alpha, const uint64_t *key_b,
                   const uint64_t *key_a, uint64_t *out_b, uint64_t *out_a);
 int or_ntt_towers(uint64_t *a, uint64_t n, uint64_t towers, const uint64_t *q, const uint64_t *psi, int inverse);
+int or_basis_switch(const uint64_t *in, uint64_t n, const uint64_t *src, uint64_t s, const uint64_t *dst, uint64_t t,
+                    uint64_t *out);
 int or_mod_up(const uint64_t *in, int in_eval, uint64_t n, const uint64_t *q, const uint64_t *qpsi, uint64_t L,
               const uint64_t *p, const uint64_t *ppsi, uint64_t K, uint64_t *out);
 int or_mod_down(const uint64_t *in, uint64_t n, const uint64_t *q, const uint64_t *qpsi, uint64_t L,

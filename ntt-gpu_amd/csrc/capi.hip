@@ -170,7 +170,7 @@ struct ntt_hip_context {
   std::vector<uint64_t> q, psi;
   bool lazy = true;
   uint32_t ntt_chunk = 0;      // towers per NTT pass pair (0 = all), set by the autotuner
-  uint32_t conv_targets = 8;   // basis-conversion targets per workgroup, set by the autotuner
+  uint32_t conv_targets = 0;   // basis-conversion targets per workgroup (0 = from the grid size)
   hipStream_t stream = nullptr;
   std::unique_ptr<DeviceBlob> tables;  // twiddles of every tower
   std::unique_ptr<DeviceBlob> primes;  // PrimeConst[towers]
@@ -250,17 +250,28 @@ std::shared_ptr<BasisPlanDev> basis_plan(ntt_hip_context &c, const std::vector<s
   plan->t = (uint32_t)t;
   // exact single reduction needs sum_i y_i * hat < m * 2^64, i.e. sum b_i < 2^64
   plan->per_term = sum_src >= ((u128)1 << 64);
+  static_assert(sizeof(BasisConst) <= 64, "plan header");
   const size_t off_src = 64, off_dst = off_src + s * sizeof(SourceConst), off_hat = off_dst + t * sizeof(TargetConst);
-  const size_t bytes = off_hat + s * t * sizeof(uint64_t);
+  const size_t off_limb = off_hat + s * t * sizeof(uint64_t);
+  const size_t bytes = off_limb + t * s * 3 * sizeof(uint32_t);
   plan->blob = std::make_unique<DeviceBlob>(c.device, bytes);
   char *base = (char *)plan->blob->ptr;
   BasisConst head{(uint32_t)s, (uint32_t)t, (const SourceConst *)(base + off_src),
-                  (const TargetConst *)(base + off_dst), (const uint64_t *)(base + off_hat)};
+                  (const TargetConst *)(base + off_dst), (const uint64_t *)(base + off_hat),
+                  (const uint32_t *)(base + off_limb)};
+  // register kernel's copy: target-major, each constant (< 2^63) as 21-bit limbs
+  std::vector<uint32_t> limbs(t * s * 3);
+  for (size_t m = 0; m < t; ++m)
+    for (size_t k = 0; k < s; ++k) {
+      const uint64_t h = hats[k * t + m];
+      for (int l = 0; l < 3; ++l) limbs[(m * s + k) * 3 + l] = (uint32_t)((h >> (21 * l)) & ((1u << 21) - 1));
+    }
   std::vector<char> host(bytes);
   std::memcpy(host.data(), &head, sizeof(head));
   std::memcpy(host.data() + off_src, sc.data(), s * sizeof(SourceConst));
   std::memcpy(host.data() + off_dst, tc.data(), t * sizeof(TargetConst));
   std::memcpy(host.data() + off_hat, hats.data(), s * t * sizeof(uint64_t));
+  std::memcpy(host.data() + off_limb, limbs.data(), limbs.size() * sizeof(uint32_t));
   {
     DeviceGuard g(c.device);
     HIP_CHECK(hipMemcpy(base, host.data(), bytes, hipMemcpyHostToDevice));
@@ -323,6 +334,20 @@ TowerMap affine(uint64_t *base, const uint64_t *src, uint64_t n, size_t towers, 
   return m;
 }
 
+// Forward-transform map whose last pass writes (sub - NTT(x)) * scale[t]
+// (the ModDown correction) to out0 (polynomials < split) / out1 (the rest).
+TowerMap with_correction(TowerMap m, const uint64_t *sub, uint64_t sub_poly_stride, uint64_t *out0, uint64_t *out1,
+                         uint32_t split, uint64_t out_poly_stride, const ulonglong2 *scale) {
+  m.epi.sub = sub;
+  m.epi.sub_poly_stride = sub_poly_stride;
+  m.epi.out[0] = out0;
+  m.epi.out[1] = out1;
+  m.epi.split = split;
+  m.epi.out_poly_stride = out_poly_stride;
+  m.epi.scale = scale;
+  return m;
+}
+
 void hip_ok(hipError_t e, const char *what) {
   if (e != hipSuccess) throw RuntimeError(std::string("HIP: ") + what + ": " + hipGetErrorString(e));
 }
@@ -347,11 +372,16 @@ struct ntt_hip_keyswitch {
 
   size_t n() const { return ctx->n; }
   // workspace layout (elements per component)
-  size_t coeff_words() const { return (size_t)L * n(); }
-  size_t digit_words() const { return (size_t)beta * (L + K) * n(); }
-  size_t z_words() const { return (size_t)(L + K) * n(); }
-  size_t pc_words() const { return (size_t)K * n(); }
-  size_t corr_words() const { return (size_t)L * n(); }
+  // Per-component strides of the workspace buffers, padded by 64 KiB: launches
+  // run tower-major (the same tower of consecutive components side by side),
+  // and unpadded strides (multiples of 512 KiB) put those concurrent accesses
+  // on the same HBM channels (measured 6% slower NTT passes).
+  static constexpr size_t kPad = 8192;
+  size_t coeff_words() const { return (size_t)L * n() + kPad; }
+  size_t digit_words() const { return (size_t)beta * (L + K) * n() + kPad; }
+  size_t z_words() const { return (size_t)(L + K) * n() + kPad; }
+  size_t pc_words() const { return (size_t)K * n() + kPad; }
+  size_t corr_words() const { return (size_t)L * n() + kPad; }
   size_t per_component() const {
     return coeff_words() + digit_words() + 2 * z_words() + 2 * pc_words() + 2 * corr_words();
   }
@@ -520,7 +550,8 @@ int ntt_hip_context_create(ntt_hip_context **out, int device, uint64_t n, const 
     while ((1u << c->log_n) < n) ++c->log_n;
     c->q.assign(q, q + towers);
     c->psi.assign(psi, psi + towers);
-    c->lazy = std::all_of(c->q.begin(), c->q.end(), [](uint64_t v) { return v < (uint64_t{1} << 62); });
+    // wide-lazy butterflies keep residues below 8q < 2^64
+    c->lazy = std::all_of(c->q.begin(), c->q.end(), [](uint64_t v) { return v < (uint64_t{1} << 61); });
     // Host tables: fwd[k] = {psi^bitrev(k), Shoup}, inv[k] = {psi^-bitrev(k), Shoup}
     // (reference transform_stage.hpp:19-30), one thread per group of towers.
     std::vector<ulonglong2> host(towers * 2 * (size_t)n);
@@ -742,21 +773,20 @@ int ntt_hip_keyswitch_create(ntt_hip_keyswitch **out, ntt_hip_context *ctx, size
     std::vector<uint32_t> tp(T);
     for (size_t i = 0; i < T; ++i) tp[i] = (uint32_t)i;
     ks->tower_prime = std::make_unique<DeviceBlob>(ctx->device, T * sizeof(uint32_t));
-    // complement NTT jobs: for component c, digit d, tower t not owned by d
+    // complement NTT jobs of one component (digit d, tower t not owned by d),
+    // ordered by prime; component c adds c * digit_words
     std::vector<TowerJob> jobs;
-    for (size_t c = 0; c < max_batch; ++c)
+    for (size_t t = 0; t < T; ++t)
       for (uint32_t d = 0; d < ks->beta; ++d) {
         const size_t first = (size_t)d * alpha, count = std::min(alpha, L - first);
-        for (size_t t = 0; t < T; ++t) {
-          if (t >= first && t < first + count) continue;
-          TowerJob j{};
-          j.offset = (c * ks->digit_words() + ((size_t)d * T + t) * n);
-          j.src_offset = j.offset;
-          j.prime = (uint32_t)t;
-          jobs.push_back(j);
-        }
+        if (t >= first && t < first + count) continue;
+        TowerJob j{};
+        j.offset = ((size_t)d * T + t) * n;
+        j.src_offset = j.offset;
+        j.prime = (uint32_t)t;
+        jobs.push_back(j);
       }
-    ks->jobs_per_poly = jobs.size() / max_batch;
+    ks->jobs_per_poly = jobs.size();
     ks->jobs = std::make_unique<DeviceBlob>(ctx->device, jobs.size() * sizeof(TowerJob));
     // Montgomery single-reduction bound for the MAC: beta * max q < 2^64
     u128 bound = 0;
@@ -847,7 +877,8 @@ int ntt_hip_keyswitch_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batc
     const PrimeConst *pr = c.dprimes();
     DeviceGuard g(c.device);
     // 1. ModUp sources: coefficient form of every Q tower (INTT, out of place)
-    hip_ok(launch_ntt(affine(coeff, in, n, L, 0, L * n, L * n), pr, c.log_n, (uint32_t)(B * L), true, c.lazy, s, c.ntt_chunk),
+    hip_ok(launch_ntt(affine(coeff, in, n, L, 0, ks->coeff_words(), L * n), pr, c.log_n, (uint32_t)(B * L), true,
+                      c.lazy, s, c.ntt_chunk),
            "intt");
     // 2. per digit: unreduced CRT into the complement, around the digit's own towers
     for (uint32_t d = 0; d < ks->beta; ++d) {
@@ -873,6 +904,8 @@ int ntt_hip_keyswitch_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batc
       m.base = digits;
       m.jobs = (const TowerJob *)ks->jobs->ptr;
       m.tower_stride = n;
+      m.poly_stride = ks->digit_words();
+      m.towers_per_poly = (uint32_t)ks->jobs_per_poly;
       hip_ok(launch_ntt(m, pr, c.log_n, (uint32_t)(B * ks->jobs_per_poly), false, c.lazy, s, c.ntt_chunk), "modup ntt");
     }
     // 4. key inner product over all digits, both channels
@@ -897,16 +930,16 @@ int ntt_hip_keyswitch_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batc
       hip_ok(launch_mac_fused(m, ks->mac_per_term, (uint32_t)B, s), "mac");
     }
     // 5. ModDown of both channels (2B polynomials): INTT(P) -> CRT to Q -> NTT -> correct
-    hip_ok(launch_ntt(affine(pc, z + L * n, n, K, L, K * n, T * n), pr, c.log_n, (uint32_t)(2 * B * K), true, c.lazy,
-                      s, c.ntt_chunk),
+    hip_ok(launch_ntt(affine(pc, z + L * n, n, K, L, ks->pc_words(), ks->z_words()), pr, c.log_n,
+                      (uint32_t)(2 * B * K), true, c.lazy, s, c.ntt_chunk),
            "moddown intt");
     {
       BasisLaunch a{};
       a.plan = ks->down->dev;
       a.in = pc;
       a.out = corr;
-      a.in_poly_stride = K * n;
-      a.out_poly_stride = L * n;
+      a.in_poly_stride = ks->pc_words();
+      a.out_poly_stride = ks->corr_words();
       a.n = (uint32_t)n;
       a.targets_per_block = c.conv_targets;
       a.plan_host_s = ks->down->s;
@@ -915,23 +948,12 @@ int ntt_hip_keyswitch_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batc
       a.skip_len = 0;
       hip_ok(launch_basis_switch(a, ks->down->per_term, (uint32_t)(2 * B), s), "moddown conversion");
     }
-    hip_ok(launch_ntt(affine(corr, nullptr, n, L, 0, L * n, 0), pr, c.log_n, (uint32_t)(2 * B * L), false, c.lazy, s, c.ntt_chunk),
-           "moddown ntt");
-    for (int ch = 0; ch < 2; ++ch) {
-      CorrectLaunch cl{};
-      cl.out = ch == 0 ? out_b : out_a;
-      cl.left = z + (size_t)ch * B * ks->z_words();
-      cl.right = corr + (size_t)ch * B * L * n;
-      cl.out_poly_stride = L * n;
-      cl.left_poly_stride = ks->z_words();
-      cl.right_poly_stride = L * n;
-      cl.primes = pr;
-      cl.scale = ks->pinv;
-      cl.first_prime = 0;
-      cl.n = (uint32_t)n;
-      cl.towers = (uint32_t)L;
-      hip_ok(launch_correct(cl, (uint32_t)B, s), "correction");
-    }
+    // NTT of the lifted correction with (Z_Q - NTT(corr)) * P^-1 fused into its
+    // last pass: channel b's B polynomials land in out_b, channel a's in out_a
+    hip_ok(launch_ntt(with_correction(affine(corr, nullptr, n, L, 0, ks->corr_words(), 0), z, ks->z_words(), out_b, out_a,
+                                      (uint32_t)B, L * n, ks->pinv),
+                      pr, c.log_n, (uint32_t)(2 * B * L), false, c.lazy, s, c.ntt_chunk),
+           "moddown ntt + correction");
     return NTT_HIP_OK;
   });
 }
@@ -1084,21 +1106,10 @@ int ntt_hip_moddown_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batch,
                       s, c.ntt_chunk),
            "moddown intt");
     launch_basis(*ks->down, c.conv_targets, pc, corr, K * n, L * n, (uint32_t)n, (uint32_t)batch, s);
-    hip_ok(launch_ntt(affine(corr, nullptr, n, L, 0, L * n, 0), pr, c.log_n, (uint32_t)(batch * L), false, c.lazy, s, c.ntt_chunk),
-           "moddown ntt");
-    CorrectLaunch cl{};
-    cl.out = out;
-    cl.left = in;
-    cl.right = corr;
-    cl.out_poly_stride = L * n;
-    cl.left_poly_stride = T * n;
-    cl.right_poly_stride = L * n;
-    cl.primes = pr;
-    cl.scale = ks->pinv;
-    cl.first_prime = 0;
-    cl.n = (uint32_t)n;
-    cl.towers = (uint32_t)L;
-    hip_ok(launch_correct(cl, (uint32_t)batch, s), "moddown correction");
+    hip_ok(launch_ntt(with_correction(affine(corr, nullptr, n, L, 0, L * n, 0), in, T * n, out, out, (uint32_t)batch,
+                                      L * n, ks->pinv),
+                      pr, c.log_n, (uint32_t)(batch * L), false, c.lazy, s, c.ntt_chunk),
+           "moddown ntt + correction");
     return NTT_HIP_OK;
   });
 }
@@ -1142,7 +1153,7 @@ extern "C" {
 
 int ntt_hip_set_tuning(ntt_hip_context *ctx, uint32_t ntt_chunk, uint32_t conv_targets) {
   return guarded([&] {
-    require(ctx && conv_targets >= 1, "invalid tuning");
+    require(ctx != nullptr, "null context");
     ctx->ntt_chunk = ntt_chunk;
     ctx->conv_targets = conv_targets;
     return NTT_HIP_OK;
@@ -1196,7 +1207,7 @@ int ntt_hip_autotune(ntt_hip_context *ctx, size_t towers, char *report, size_t c
       const uint32_t polys = (uint32_t)std::max<size_t>(1, std::min(towers / t, towers / s));
       rep += ",\"conversion\":{";
       best = 1e30f;
-      for (uint32_t tpb : {1u, 2u, 4u, 6u, 8u, 12u, 16u, 24u}) {
+      for (uint32_t tpb : {0u, 1u, 2u, 4u, 6u, 8u, 12u, 16u, 24u}) {
         if (tpb > t) continue;
         BasisLaunch bl{};
         bl.plan = plan->dev;
@@ -1211,7 +1222,9 @@ int ntt_hip_autotune(ntt_hip_context *ctx, size_t towers, char *report, size_t c
         bl.skip_at = 0xffffffffu;
         const float ms = time_ms(c, [&] { hip_ok(launch_basis_switch(bl, plan->per_term, polys, c.stream), "tune conv"); }, 3);
         rep += "\"" + std::to_string(tpb) + "\":" + std::to_string(ms) + ",";
-        if (ms < best) {
+        // 0 (geometry from the grid size, first candidate) adapts to every
+        // batch size; a fixed width must be clearly faster to replace it
+        if (tpb == 0 ? ms < best : ms < best * 0.95f) {
           best = ms;
           best_targets = tpb;
         }

@@ -16,6 +16,7 @@ struct BasisConst {
   const SourceConst *src;    // [s]
   const TargetConst *dst;    // [t]
   const uint64_t *hat_mont;  // [s * t]: (Bhat_i mod m) * 2^64 mod m, source-major
+  const uint32_t *hat_limbs; // [t][s][3]: the same constants as three 21-bit limbs, target-major
 };
 
 struct BasisLaunch {
@@ -23,7 +24,7 @@ struct BasisLaunch {
   const uint64_t *in;      // s towers, tower-major, n each
   uint64_t *out;           // t towers
   uint64_t in_poly_stride, out_poly_stride;
-  uint32_t n, targets_per_block;
+  uint32_t n, targets_per_block;  // 0: chosen from the grid size
   uint32_t plan_host_s, plan_host_t;  // host copies for the launch geometry
   // target m is written to output tower (m < skip_at ? m : m + skip_len), which
   // lets a Hybrid digit's complement land around the digit's own towers.

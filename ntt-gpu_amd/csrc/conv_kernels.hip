@@ -9,6 +9,14 @@
 // being stored as (Bhat_i mod m) * 2^64 mod m so the result is the plain
 // residue. A per-term reduction variant covers sums that could exceed
 // m * 2^64 (very wide or very many source primes).
+//
+// Up to 16 sources (every ModUp digit and ModDown of the key-switch shapes)
+// take a register kernel: the scaled sources y_i stay in VGPRs, the plan is
+// read through scalar loads, and each constant is split into three 21-bit
+// limbs so the six partial products y_{0,1} * h_{0,1,2} of every term are
+// plain v_mad_u64_u32 accumulations that cannot overflow 64 bits (each sum
+// is below 16 * 2^53); one 128-bit combine and one Montgomery reduction
+// finish each output.
 #include <algorithm>
 
 #include "conv_device.hpp"
@@ -53,9 +61,76 @@ __global__ __launch_bounds__(256) void basis_switch_kernel(BasisLaunch a) {
   }
 }
 
-hipError_t launch_basis_switch(const BasisLaunch &a, bool per_term, uint32_t polys, hipStream_t stream) {
+// Scalar-load views of the plan (constant address space: uniform reads go
+// through the scalar cache).
+typedef const __attribute__((address_space(4))) SourceConst *CSrc;
+typedef const __attribute__((address_space(4))) TargetConst *CDst;
+typedef const __attribute__((address_space(4))) uint32_t *CLimbs;
+
+constexpr uint32_t kRegSources = 16;
+
+// Same grid as basis_switch_kernel; s <= kRegSources and sum_i b_i < 2^64.
+__global__ __launch_bounds__(256) void basis_switch_regs(BasisLaunch a) {
+  const BasisConst &p = *a.plan;
+  const CSrc src = (CSrc)(const void *)p.src;
+  const CDst dst = (CDst)(const void *)p.dst;
+  const CLimbs limbs = (CLimbs)(const void *)p.hat_limbs;
+  const uint32_t s = a.plan_host_s, t = a.plan_host_t;
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t chunks = (t + a.targets_per_block - 1) / a.targets_per_block;
+  const uint32_t poly = blockIdx.y / chunks, chunk = blockIdx.y - poly * chunks;
+  const uint64_t *in = a.in + (uint64_t)poly * a.in_poly_stride;
+  uint64_t *out = a.out + (uint64_t)poly * a.out_poly_stride;
+  if (i >= a.n) return;
+  uint64_t y[kRegSources];
+#pragma unroll
+  for (uint32_t k = 0; k < kRegSources; ++k)
+    if (k < s) y[k] = in[(uint64_t)k * a.n + i];
+#pragma unroll
+  for (uint32_t k = 0; k < kRegSources; ++k)
+    if (k < s) y[k] = shoup(y[k], src[k].inv, src[k].inv_shoup, src[k].q);
+  const uint32_t m0 = chunk * a.targets_per_block;
+  const uint32_t m1 = min(t, m0 + a.targets_per_block);
+  for (uint32_t m = m0; m < m1; ++m) {
+    const CLimbs h = limbs + (uint64_t)m * s * 3;
+    uint64_t p00 = 0, p01 = 0, p02 = 0, p10 = 0, p11 = 0, p12 = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kRegSources; ++k) {
+      if (k < s) {
+        const uint32_t y0 = (uint32_t)y[k], y1 = (uint32_t)(y[k] >> 32);
+        const uint32_t h0 = h[3 * k], h1 = h[3 * k + 1], h2 = h[3 * k + 2];
+        p00 += (uint64_t)y0 * h0;
+        p01 += (uint64_t)y0 * h1;
+        p02 += (uint64_t)y0 * h2;
+        p10 += (uint64_t)y1 * h0;
+        p11 += (uint64_t)y1 * h1;
+        p12 += (uint64_t)y1 * h2;
+      }
+    }
+    const uint64_t r = mont_reduce(combine_limbs(p00, p01, p02, p10, p11, p12), dst[m].q, dst[m].qinv_neg);
+    const uint32_t ot = m < a.skip_at ? m : m + a.skip_len;
+    out[(uint64_t)ot * a.n + i] = r;
+  }
+}
+
+// Targets per workgroup when the caller leaves it to the launcher: as many as
+// keep at least ~2048 workgroups in flight (the y_i are recomputed per chunk).
+static uint32_t auto_targets(uint32_t t, uint32_t n, uint32_t polys) {
+  const uint32_t blocks = ((n + 255) / 256) * polys;
+  uint32_t chunks = (2048 + blocks - 1) / blocks;
+  chunks = chunks < 1 ? 1 : (chunks > t ? t : chunks);
+  return (t + chunks - 1) / chunks;
+}
+
+hipError_t launch_basis_switch(const BasisLaunch &in, bool per_term, uint32_t polys, hipStream_t stream) {
+  BasisLaunch a = in;
+  if (a.targets_per_block == 0) a.targets_per_block = auto_targets(a.plan_host_t, a.n, polys);
   const uint32_t chunks = (a.plan_host_t + a.targets_per_block - 1) / a.targets_per_block;
   dim3 grid((a.n + 255) / 256, chunks * polys);
+  if (!per_term && a.plan_host_s <= kRegSources) {
+    hipLaunchKernelGGL(basis_switch_regs, grid, dim3(256), 0, stream, a);
+    return hipGetLastError();
+  }
   const size_t lds = (size_t)a.plan_host_s * 256 * sizeof(uint64_t);
   if (lds > 64 * 1024) {
     // up to 40 sources: 80 KiB of the 160 KiB gfx950 LDS
@@ -99,10 +174,13 @@ hipError_t launch_mac_digit(uint64_t *b, uint64_t *a, const uint64_t *digit, con
 // Fused inner product over all digits with resident Montgomery-form keys:
 // Z[t][i] = sum_d D_d[t][i] * K_d[t][i] for both channels, one reduction per output.
 // digits: poly-major (poly, digit, tower, i); keys: (digit, tower, i) in Montgomery form.
+// blockIdx.y = tower * polys + poly: the polynomials of one tower run back to
+// back, so the tower's key rows (digits x 2 channels) are re-read from L2
+// rather than from HBM once per polynomial.
 template <bool PerTerm>
-__global__ __launch_bounds__(256) void mac_fused_kernel(MacLaunch m) {
+__global__ __launch_bounds__(256) void mac_fused_kernel(MacLaunch m, uint32_t polys) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  const uint32_t poly = blockIdx.y / m.towers, tower = blockIdx.y - poly * m.towers;
+  const uint32_t tower = blockIdx.y / polys, poly = blockIdx.y - tower * polys;
   if (i >= m.n) return;
   const PrimeConst &pc = m.primes[m.tower_prime[tower]];
   const uint64_t *dg = m.digits + (uint64_t)poly * m.digit_poly_stride + (uint64_t)tower * m.n + i;
@@ -136,8 +214,8 @@ __global__ __launch_bounds__(256) void mac_fused_kernel(MacLaunch m) {
 
 hipError_t launch_mac_fused(const MacLaunch &m, bool per_term, uint32_t polys, hipStream_t stream) {
   dim3 grid((m.n + 255) / 256, m.towers * polys);
-  if (per_term) hipLaunchKernelGGL(mac_fused_kernel<true>, grid, dim3(256), 0, stream, m);
-  else hipLaunchKernelGGL(mac_fused_kernel<false>, grid, dim3(256), 0, stream, m);
+  if (per_term) hipLaunchKernelGGL(mac_fused_kernel<true>, grid, dim3(256), 0, stream, m, polys);
+  else hipLaunchKernelGGL(mac_fused_kernel<false>, grid, dim3(256), 0, stream, m, polys);
   return hipGetLastError();
 }
 

@@ -24,7 +24,7 @@ __device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) { return __u
 
 __device__ __forceinline__ U128 mul_wide(uint64_t a, uint64_t b) { return {a * b, __umul64hi(a, b)}; }
 
-__device__ __forceinline__ void add_wide(U128 &acc, U128 v) {
+__host__ __device__ __forceinline__ void add_wide(U128 &acc, U128 v) {
   const uint64_t lo = acc.lo + v.lo;
   acc.hi += v.hi + (lo < acc.lo ? 1u : 0u);
   acc.lo = lo;
@@ -62,6 +62,90 @@ __device__ __forceinline__ uint64_t mont_reduce(U128 s, uint64_t q, uint64_t qin
 // Montgomery product of canonical a and b_mont = b * 2^64 mod q -> a * b mod q.
 __device__ __forceinline__ uint64_t mont_mul(uint64_t a, uint64_t b_mont, uint64_t q, uint64_t qinv_neg) {
   return mont_reduce(mul_wide(a, b_mont), q, qinv_neg);
+}
+
+// sum_k y_k h_k from limb sums p_ab = sum_k y_a h_b, where y = y1 2^32 + y0 and
+// h = h2 2^42 + h1 2^21 + h0 (21-bit limbs): Q_a = p_a0 + p_a1 2^21 + p_a2 2^42,
+// S = Q_0 + Q_1 2^32. Exact whenever S < 2^127 (then Q_1 < 2^95).
+__host__ __device__ __forceinline__ U128 combine_limbs(uint64_t p00, uint64_t p01, uint64_t p02, uint64_t p10,
+                                                       uint64_t p11, uint64_t p12) {
+  U128 q0{p00, 0}, q1{p10, 0};
+  add_wide(q0, {p01 << 21, p01 >> 43});
+  add_wide(q0, {p02 << 42, p02 >> 22});
+  add_wide(q1, {p11 << 21, p11 >> 43});
+  add_wide(q1, {p12 << 42, p12 >> 22});
+  add_wide(q0, {q1.lo << 32, (q1.hi << 32) | (q1.lo >> 32)});
+  return q0;
+}
+
+// ---------------------------------------------------------------- wide-lazy butterflies (q < 2^61)
+// For q < 2^61 the NTT keeps residues in [0, 8q) and multiplies with a Shoup
+// quotient that may be short by up to 2: floor(y * w' / 2^64) is estimated as
+// y1 p1 + hi32(y1 p0) + hi32(y0 p1), dropping y0 p0 and the carry out of the
+// low halves (the dropped part is below 3 * 2^64, so the estimate is low by
+// 0, 1 or 2). Then T = y w - qh q lies in [0, 4q) instead of [0, 2q), and the
+// butterfly folds its add into the multiply-add chain:
+//   x' = x + T,  y' = x - T + 4q,  x first brought below 4q.
+// Per butterfly that is 3 v_mad_u64_u32 + 1 add pair for the quotient,
+// 2 v_mad_u64_u32 + 4 v_mul_lo_u32 + 2 adds for x', and a few 64-bit adds:
+// about 20 VALU instructions against ~31 for the exact form.
+struct Lazy8 {
+  uint64_t q, q4, nq;  // q, 4q, 2^64 - q
+};
+
+__host__ __device__ __forceinline__ Lazy8 lazy8_of(uint64_t q) { return {q, q << 2, 0 - q}; }
+
+// floor(y * p / 2^64) - e with e in {0, 1, 2}, any y, p < 2^64.
+__host__ __device__ __forceinline__ uint64_t mulhi_est(uint64_t y, uint64_t p) {
+  const uint32_t y0 = (uint32_t)y, y1 = (uint32_t)(y >> 32), p0 = (uint32_t)p, p1 = (uint32_t)(p >> 32);
+  const uint64_t a = (uint64_t)y1 * p0, b = (uint64_t)y0 * p1;
+  return (uint64_t)y1 * p1 + ((a >> 32) + (b >> 32));
+}
+
+// (x + y * w + qh * nq) mod 2^64 from 32-bit limbs (nq = 2^64 - q, so the
+// last term is -qh * q): two multiply-adds carry the low words, four low
+// products land in the high word.
+__host__ __device__ __forceinline__ uint64_t mad_lo(uint64_t x, uint64_t y, uint64_t w, uint64_t qh, uint64_t nq) {
+  const uint32_t y0 = (uint32_t)y, y1 = (uint32_t)(y >> 32), w0 = (uint32_t)w, w1 = (uint32_t)(w >> 32);
+  const uint32_t h0 = (uint32_t)qh, h1 = (uint32_t)(qh >> 32), n0 = (uint32_t)nq, n1 = (uint32_t)(nq >> 32);
+  uint64_t r = (uint64_t)y0 * w0 + x;
+  r = (uint64_t)h0 * n0 + r;
+  const uint32_t hi = (uint32_t)(r >> 32) + y0 * w1 + y1 * w0 + h0 * n1 + h1 * n0;
+  return ((uint64_t)hi << 32) | (uint32_t)r;
+}
+
+// y * w mod q in [0, 4q) for any y < 2^64 (w < q, wp = floor(w 2^64 / q), q < 2^61).
+__host__ __device__ __forceinline__ uint64_t shoup4(uint64_t y, uint64_t w, uint64_t wp, const Lazy8 &m) {
+  return mad_lo(0, y, w, mulhi_est(y, wp), m.nq);
+}
+
+// x >= c ? x - c : x, selected on the borrow of the subtraction (sub, subb,
+// two selects) rather than a separate 64-bit compare.
+__host__ __device__ __forceinline__ uint64_t sub_if_ge(uint64_t x, uint64_t c) {
+  uint64_t d;
+  const bool borrow = __builtin_sub_overflow(x, c, &d);
+  return borrow ? x : d;
+}
+
+// Forward (Cooley-Tukey) butterfly: x, y in [0, 8q) -> x + yw, x - yw, both in [0, 8q).
+__host__ __device__ __forceinline__ void ct8(uint64_t &x, uint64_t &y, uint64_t w, uint64_t wp, const Lazy8 &m) {
+  const uint64_t xr = sub_if_ge(x, m.q4);  // [0, 4q)
+  const uint64_t xn = mad_lo(xr, y, w, mulhi_est(y, wp), m.nq);  // xr + T, T in [0, 4q)
+  y = 2 * xr + m.q4 - xn;  // xr - T + 4q in (0, 8q); the wrap of 2xr + 4q cancels mod 2^64
+  x = xn;
+}
+
+// Inverse (Gentleman-Sande) butterfly: x, y in [0, 4q) -> x + y, (x - y) w, both in [0, 4q).
+__host__ __device__ __forceinline__ void gs8(uint64_t &x, uint64_t &y, uint64_t w, uint64_t wp, const Lazy8 &m) {
+  const uint64_t s = x + y;  // [0, 8q)
+  const uint64_t d = x + m.q4 - y;  // (0, 8q)
+  x = sub_if_ge(s, m.q4);
+  y = shoup4(d, w, wp, m);
+}
+
+// [0, 8q) -> [0, q).
+__host__ __device__ __forceinline__ uint64_t canon8(uint64_t x, const Lazy8 &m) {
+  return sub_if_ge(sub_if_ge(sub_if_ge(x, m.q4), m.q << 1), m.q);
 }
 
 }  // namespace nttamd

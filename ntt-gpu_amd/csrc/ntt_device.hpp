@@ -17,25 +17,43 @@ struct PrimeConst {
   const ulonglong2 *inv;
 };
 
-// Explicit tower: data at base + offset (elements), constants primes[prime].
+// Explicit tower of polynomial 0: data at base + offset (elements),
+// constants primes[prime]; polynomial p adds p * poly_stride.
 struct TowerJob {
   uint64_t offset, src_offset;
   uint32_t prime;
   uint32_t pad;
 };
 
-// Which towers a launch touches. blockIdx.y = y selects either jobs[y] or the
-// affine map: poly = y / towers_per_poly, t = y % towers_per_poly,
-// data = base + poly * poly_stride + t * tower_stride, prime = prime_offset + t.
-// `src` (optional) makes the first pass out of place: it reads tower y at
-// src + poly * src_poly_stride + t * tower_stride (or src + jobs[y].src_offset).
+// Which towers a launch touches. Launch index y (blockIdx.y + y_offset) is
+// taken tower-major, so the polynomials sharing a prime (and its twiddle
+// table) run next to each other: k = y / polys, poly = y % polys, then either
+// jobs[k] or the affine map t = k:
+//   data = base + poly * poly_stride + (jobs[k].offset | t * tower_stride),
+//   prime = jobs[k].prime | prime_offset + t.
+// `src` (optional) makes the first pass out of place: it reads from
+// src + poly * src_poly_stride + (jobs[k].src_offset | t * tower_stride).
+// towers_per_poly is the number of jobs or affine towers; launch_ntt sets polys.
 struct TowerMap {
   uint64_t *base;
   const uint64_t *src;
   const TowerJob *jobs;
   uint64_t poly_stride, tower_stride, src_poly_stride;
   uint32_t towers_per_poly, prime_offset;
-  uint32_t y_offset;  // tower index of blockIdx.y == 0 (chunked launches)
+  uint32_t y_offset;  // launch index of blockIdx.y == 0 (chunked launches)
+  uint32_t polys;     // towers / towers_per_poly
+  // Optional epilogue of a forward transform (ModDown correction, affine maps
+  // only): the last pass writes (sub - NTT(x)) * scale[k] mod q to
+  // out[poly < split ? 0 : 1] + (poly mod split) * out_poly_stride + k * tower_stride
+  // instead of writing NTT(x) back; sub is read at
+  // sub + poly * sub_poly_stride + k * tower_stride.
+  struct Epilogue {
+    const uint64_t *sub;
+    uint64_t *out[2];
+    const ulonglong2 *scale;  // [k] {c, Shoup(c)}
+    uint64_t sub_poly_stride, out_poly_stride;
+    uint32_t split;
+  } epi;
 };
 
 hipError_t launch_ntt(const TowerMap &map, const PrimeConst *primes, uint32_t log_n, uint32_t towers, bool inverse,

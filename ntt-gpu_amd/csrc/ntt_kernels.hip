@@ -13,9 +13,10 @@
 // In both passes a thread keeps 16 residues in registers and runs 4 radix-2
 // stages on them, then trades through LDS for the other 4 ("P1" layout:
 // j = u + 16k, "P2" layout: j = 16u + k, u = lane within the sub-transform).
-// Butterflies are Harvey-lazy (values in [0, 4q), q < 2^62) with Shoup
+// Butterflies for q < 2^61 are wide-lazy (forward residues in [0, 8q),
+// inverse in [0, 4q), a Shoup quotient short by at most 2: modarith.hpp) with
 // twiddles {W, floor(W 2^64/q)} read as one 16-byte load; a canonical variant
-// serves 2^62 <= q < 2^63. Every kernel output is canonical except the
+// serves 2^61 <= q < 2^63. Every kernel output is canonical except the
 // internal hand-off between the two passes of one transform.
 //
 // Pass A for R < 256 (N = 2^9..2^15) and whole transforms for N <= 2^12 use
@@ -29,40 +30,44 @@ constexpr int kLdsRow = 256 + 16 + 1;  // padded LDS row (one word per 16, plus 
 
 __device__ __forceinline__ int lds_idx(int row, int j) { return row * kLdsRow + j + (j >> 4); }
 
+// Twiddle pairs {W, floor(W 2^64 / q)} read through a global-address-space
+// pointer (one 16-byte global load, not a flat load).
+typedef uint64_t tw2 __attribute__((ext_vector_type(2)));
+typedef const __attribute__((address_space(1))) tw2 *TwPtr;
+
+__device__ __forceinline__ TwPtr tw_table(const ulonglong2 *p) { return (TwPtr)(const void *)p; }
+
+// Lazy: q < 2^61, forward residues in [0, 8q), inverse in [0, 4q) (ct8/gs8).
+// Exact: 2^61 <= q < 2^63, canonical throughout.
 template <bool Lazy>
-__device__ __forceinline__ void ct_bfly(uint64_t &x, uint64_t &y, ulonglong2 w, uint64_t q, uint64_t two_q) {
+__device__ __forceinline__ void ct_bfly(uint64_t &x, uint64_t &y, tw2 w, const Lazy8 &m) {
   if constexpr (Lazy) {
-    x = x >= two_q ? x - two_q : x;
-    const uint64_t t = shoup_lazy(y, w.x, w.y, q);
-    y = x - t + two_q;
-    x = x + t;
+    ct8(x, y, w.x, w.y, m);
   } else {
-    const uint64_t t = shoup(y, w.x, w.y, q);
-    y = sub_mod(x, t, q);
-    x = add_mod(x, t, q);
+    const uint64_t t = shoup(y, w.x, w.y, m.q);
+    y = sub_mod(x, t, m.q);
+    x = add_mod(x, t, m.q);
   }
 }
 
 template <bool Lazy>
-__device__ __forceinline__ void gs_bfly(uint64_t &x, uint64_t &y, ulonglong2 w, uint64_t q, uint64_t two_q) {
+__device__ __forceinline__ void gs_bfly(uint64_t &x, uint64_t &y, tw2 w, const Lazy8 &m) {
   if constexpr (Lazy) {
-    uint64_t s = x + y;
-    s = s >= two_q ? s - two_q : s;
-    y = shoup_lazy(x - y + two_q, w.x, w.y, q);
-    x = s;
+    gs8(x, y, w.x, w.y, m);
   } else {
-    const uint64_t s = add_mod(x, y, q);
-    y = shoup(sub_mod(x, y, q), w.x, w.y, q);
+    const uint64_t s = add_mod(x, y, m.q);
+    y = shoup(sub_mod(x, y, m.q), w.x, w.y, m.q);
     x = s;
   }
 }
 
-// Final inverse stage with n^-1 folded in: x' = (x+y) n^-1, y' = (x-y) W n^-1.
+// Final inverse stage with n^-1 folded in: x' = (x+y) n^-1, y' = (x-y) W n^-1,
+// both canonical (exact Shoup quotient; lazy inputs in [0, 4q)).
 template <bool Lazy>
-__device__ __forceinline__ void gs_bfly_last(uint64_t &x, uint64_t &y, const PrimeConst &pc) {
+__device__ __forceinline__ void gs_bfly_last(uint64_t &x, uint64_t &y, const PrimeConst &pc, const Lazy8 &m) {
   if constexpr (Lazy) {
     const uint64_t s = x + y;
-    const uint64_t d = x - y + pc.two_q;
+    const uint64_t d = x + m.q4 - y;
     x = reduce_once(shoup_lazy(s, pc.ninv, pc.ninv_shoup, pc.q), pc.q);
     y = reduce_once(shoup_lazy(d, pc.last_w, pc.last_wp, pc.q), pc.q);
   } else {
@@ -74,18 +79,17 @@ __device__ __forceinline__ void gs_bfly_last(uint64_t &x, uint64_t &y, const Pri
 }
 
 template <bool Lazy>
-__device__ __forceinline__ uint64_t canon(uint64_t x, uint64_t q, uint64_t two_q) {
-  if constexpr (Lazy) x = x >= two_q ? x - two_q : x;
-  return x >= q ? x - q : x;
+__device__ __forceinline__ uint64_t canon(uint64_t x, const Lazy8 &m) {
+  if constexpr (Lazy) return canon8(x, m);
+  return x;
 }
 
 // Four register stages on v[0..15]. H runs 8,4,2,1 (forward) or 1,2,4,8
 // (inverse). Twiddle index = ml * row_factor + group, ml = (8/h) * ml_scale,
 // group = g_base * (8/h) + k / (2h).
 template <bool Inverse, bool Lazy>
-__device__ __forceinline__ void reg_stages(uint64_t (&v)[16], const ulonglong2 *__restrict__ tw,
-                                           uint32_t ml_scale, uint32_t row_factor, uint32_t g_base,
-                                           uint64_t q, uint64_t two_q) {
+__device__ __forceinline__ void reg_stages(uint64_t (&v)[16], TwPtr tw, uint32_t ml_scale, uint32_t row_factor,
+                                           uint32_t g_base, const Lazy8 &m) {
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     const int h = Inverse ? (1 << s) : (8 >> s);
@@ -95,29 +99,40 @@ __device__ __forceinline__ void reg_stages(uint64_t (&v)[16], const ulonglong2 *
     for (int k = 0; k < 16; ++k) {
       if (k & h) continue;
       const uint32_t idx = ml * row_factor + gb + (uint32_t)(k / (2 * h));
-      const ulonglong2 w = tw[idx];
-      if constexpr (Inverse) gs_bfly<Lazy>(v[k], v[k + h], w, q, two_q);
-      else ct_bfly<Lazy>(v[k], v[k + h], w, q, two_q);
+      const tw2 w = tw[idx];
+      if constexpr (Inverse) gs_bfly<Lazy>(v[k], v[k + h], w, m);
+      else ct_bfly<Lazy>(v[k], v[k + h], w, m);
     }
   }
 }
 
-// Resolve tower y of a launch: destination, source (== destination when in
-// place or when this is not the first pass) and prime.
+// Resolve launch index y (tower-major): destination, source (== destination
+// when in place or when this is not the first pass) and prime.
 __device__ __forceinline__ void tower_of(const TowerMap &m, uint32_t y, bool first_pass, uint64_t *&data,
-                                         const uint64_t *&src, uint32_t &prime) {
+                                         const uint64_t *&src, uint32_t &prime, uint32_t &k, uint32_t &poly) {
   y += m.y_offset;
+  k = y / m.polys;
+  poly = y - k * m.polys;
+  uint64_t off, src_off;
   if (m.jobs) {
-    const TowerJob j = m.jobs[y];
-    data = m.base + j.offset;
-    src = (first_pass && m.src) ? m.src + j.src_offset : data;
+    const TowerJob j = m.jobs[k];
+    off = j.offset;
+    src_off = j.src_offset;
     prime = j.prime;
   } else {
-    const uint32_t poly = y / m.towers_per_poly, t = y - poly * m.towers_per_poly;
-    data = m.base + (uint64_t)poly * m.poly_stride + (uint64_t)t * m.tower_stride;
-    src = (first_pass && m.src) ? m.src + (uint64_t)poly * m.src_poly_stride + (uint64_t)t * m.tower_stride : data;
-    prime = m.prime_offset + t;
+    off = src_off = (uint64_t)k * m.tower_stride;
+    prime = m.prime_offset + k;
   }
+  data = m.base + (uint64_t)poly * m.poly_stride + off;
+  src = (first_pass && m.src) ? m.src + (uint64_t)poly * m.src_poly_stride + src_off : data;
+}
+
+// Epilogue destinations of tower (k, poly): the subtrahend and the output.
+__device__ __forceinline__ uint64_t *epi_dest(const TowerMap &m, uint32_t k, uint32_t poly, const uint64_t *&sub) {
+  sub = m.epi.sub + (uint64_t)poly * m.epi.sub_poly_stride + (uint64_t)k * m.tower_stride;
+  const bool second = poly >= m.epi.split;
+  return m.epi.out[second ? 1 : 0] + (uint64_t)(second ? poly - m.epi.split : poly) * m.epi.out_poly_stride +
+         (uint64_t)k * m.tower_stride;
 }
 
 // ---------------------------------------------------------------- pass A, R = 256
@@ -126,29 +141,30 @@ __global__ __launch_bounds__(256) void ntt_pass_a256(TowerMap map, const PrimeCo
   __shared__ uint64_t lds[16 * kLdsRow];
   uint64_t *data;
   const uint64_t *src;
-  uint32_t prime;
-  tower_of(map, blockIdx.y, !Inverse, data, src, prime);
+  uint32_t prime, tk, tpoly;
+  tower_of(map, blockIdx.y, !Inverse, data, src, prime, tk, tpoly);
   const PrimeConst pc = primes[prime];
-  const ulonglong2 *tw = Inverse ? pc.inv : pc.fwd;
+  const TwPtr tw = tw_table(Inverse ? pc.inv : pc.fwd);
+  const Lazy8 m = lazy8_of(pc.q);
   const int t = threadIdx.x, cl = t & 15, u = t >> 4;
   const uint32_t c = blockIdx.x * 16 + cl;  // column
   uint64_t v[16];
   if constexpr (!Inverse) {
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = src[(uint64_t)(u + 16 * k) * kRow + c];
-    reg_stages<false, Lazy>(v, tw, 1, 1, 0, pc.q, pc.two_q);
+    reg_stages<false, Lazy>(v, tw, 1, 1, 0, m);
 #pragma unroll
     for (int k = 0; k < 16; ++k) lds[lds_idx(cl, u + 16 * k)] = v[k];
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = lds[lds_idx(cl, 16 * u + k)];
-    reg_stages<false, Lazy>(v, tw, 16, 1, u, pc.q, pc.two_q);
+    reg_stages<false, Lazy>(v, tw, 16, 1, u, m);
 #pragma unroll
     for (int k = 0; k < 16; ++k) data[(uint64_t)(16 * u + k) * kRow + c] = v[k];
   } else {
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = data[(uint64_t)(16 * u + k) * kRow + c];
-    reg_stages<true, Lazy>(v, tw, 16, 1, u, pc.q, pc.two_q);
+    reg_stages<true, Lazy>(v, tw, 16, 1, u, m);
 #pragma unroll
     for (int k = 0; k < 16; ++k) lds[lds_idx(cl, 16 * u + k)] = v[k];
     __syncthreads();
@@ -161,12 +177,12 @@ __global__ __launch_bounds__(256) void ntt_pass_a256(TowerMap map, const PrimeCo
 #pragma unroll
       for (int k = 0; k < 16; ++k) {
         if (k & h) continue;
-        const ulonglong2 w = tw[(8u / h) + (uint32_t)(k / (2 * h))];
-        gs_bfly<Lazy>(v[k], v[k + h], w, pc.q, pc.two_q);
+        const tw2 w = tw[(8u / h) + (uint32_t)(k / (2 * h))];
+        gs_bfly<Lazy>(v[k], v[k + h], w, m);
       }
     }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) gs_bfly_last<Lazy>(v[k], v[k + 8], pc);
+    for (int k = 0; k < 8; ++k) gs_bfly_last<Lazy>(v[k], v[k + 8], pc, m);
 #pragma unroll
     for (int k = 0; k < 16; ++k) data[(uint64_t)(u + 16 * k) * kRow + c] = v[k];
   }
@@ -174,15 +190,20 @@ __global__ __launch_bounds__(256) void ntt_pass_a256(TowerMap map, const PrimeCo
 
 // ---------------------------------------------------------------- pass B (rows of 256)
 // R = rows in the tower (N / 256); row_factor for ml is R + r.
-template <bool Inverse, bool Lazy>
+// Epi (forward only): the ModDown correction epilogue. Its subtrahend is
+// loaded at the start (overlapping the butterflies) at the thread's final
+// positions 16u..16u+15, and the result is written there directly, without
+// the exchange back to the coalesced layout.
+template <bool Inverse, bool Lazy, bool Epi>
 __global__ __launch_bounds__(256) void ntt_pass_b(TowerMap map, const PrimeConst *__restrict__ primes, uint32_t R) {
   __shared__ uint64_t lds[16 * kLdsRow];
   uint64_t *data;
   const uint64_t *src;
-  uint32_t prime;
-  tower_of(map, blockIdx.y, Inverse, data, src, prime);
+  uint32_t prime, tk, tpoly;
+  tower_of(map, blockIdx.y, Inverse, data, src, prime, tk, tpoly);
   const PrimeConst pc = primes[prime];
-  const ulonglong2 *tw = Inverse ? pc.inv : pc.fwd;
+  const TwPtr tw = tw_table(Inverse ? pc.inv : pc.fwd);
+  const Lazy8 m = lazy8_of(pc.q);
   const int t = threadIdx.x, u = t & 15, rl = t >> 4;
   const uint32_t r = blockIdx.x * 16 + rl;
   uint64_t *row = data + (uint64_t)r * kRow;
@@ -191,16 +212,38 @@ __global__ __launch_bounds__(256) void ntt_pass_b(TowerMap map, const PrimeConst
   uint64_t v[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) v[k] = src_row[u + 16 * k];
-  if constexpr (!Inverse) {
-    reg_stages<false, Lazy>(v, tw, 1, rf, 0, pc.q, pc.two_q);
+  if constexpr (Epi) {
+    const uint64_t *sub;
+    uint64_t *out = epi_dest(map, tk, tpoly, sub) + (uint64_t)r * kRow + 16 * u;
+    sub += (uint64_t)r * kRow + 16 * u;
+    uint64_t z[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) z[k] = sub[k];
+    reg_stages<false, Lazy>(v, tw, 1, rf, 0, m);
 #pragma unroll
     for (int k = 0; k < 16; ++k) lds[lds_idx(rl, u + 16 * k)] = v[k];
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = lds[lds_idx(rl, 16 * u + k)];
-    reg_stages<false, Lazy>(v, tw, 16, rf, u, pc.q, pc.two_q);
+    reg_stages<false, Lazy>(v, tw, 16, rf, u, m);
+    const ulonglong2 c = map.epi.scale[tk];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) v[k] = canon<Lazy>(v[k], pc.q, pc.two_q);
+    for (int k = 0; k < 16; ++k) {
+      // lazy: v in [0, 8q) -> [0, 4q); z + 4q - v in (0, 5q) < 2^64; Shoup takes any word
+      const uint64_t d = Lazy ? z[k] + m.q4 - sub_if_ge(v[k], m.q4) : sub_mod(z[k], v[k], pc.q);
+      out[k] = shoup(d, c.x, c.y, pc.q);
+    }
+    return;
+  } else if constexpr (!Inverse) {
+    reg_stages<false, Lazy>(v, tw, 1, rf, 0, m);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) lds[lds_idx(rl, u + 16 * k)] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = lds[lds_idx(rl, 16 * u + k)];
+    reg_stages<false, Lazy>(v, tw, 16, rf, u, m);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = canon<Lazy>(v[k], m);
 #pragma unroll
     for (int k = 0; k < 16; ++k) lds[lds_idx(rl, 16 * u + k)] = v[k];
     __syncthreads();
@@ -212,13 +255,13 @@ __global__ __launch_bounds__(256) void ntt_pass_b(TowerMap map, const PrimeConst
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = lds[lds_idx(rl, 16 * u + k)];
-    reg_stages<true, Lazy>(v, tw, 16, rf, u, pc.q, pc.two_q);
+    reg_stages<true, Lazy>(v, tw, 16, rf, u, m);
 #pragma unroll
     for (int k = 0; k < 16; ++k) lds[lds_idx(rl, 16 * u + k)] = v[k];
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = lds[lds_idx(rl, u + 16 * k)];
-    reg_stages<true, Lazy>(v, tw, 1, rf, 0, pc.q, pc.two_q);
+    reg_stages<true, Lazy>(v, tw, 1, rf, 0, m);
   }
 #pragma unroll
   for (int k = 0; k < 16; ++k) row[u + 16 * k] = v[k];
@@ -238,10 +281,11 @@ __global__ __launch_bounds__(256) void ntt_lds_generic(TowerMap map, const Prime
   __shared__ uint64_t lds[4096];
   uint64_t *data;
   const uint64_t *src;
-  uint32_t prime;
-  tower_of(map, blockIdx.y, first_pass != 0, data, src, prime);
+  uint32_t prime, tk, tpoly;
+  tower_of(map, blockIdx.y, first_pass != 0, data, src, prime, tk, tpoly);
   const PrimeConst pc = primes[prime];
-  const ulonglong2 *tw = Inverse ? pc.inv : pc.fwd;
+  const TwPtr tw = tw_table(Inverse ? pc.inv : pc.fwd);
+  const Lazy8 m = lazy8_of(pc.q);
   const uint32_t L = 1u << log_len;
   const uint32_t total = L * cols_per_block;  // <= 4096
   const uint32_t col0 = blockIdx.x * cols_per_block;
@@ -263,21 +307,31 @@ __global__ __launch_bounds__(256) void ntt_lds_generic(TowerMap map, const Prime
       const uint32_t j = pp + (g << lg_d);
       uint64_t x = lds[col * L + j], y = lds[col * L + j + (1u << lg_d)];
       if constexpr (Inverse) {
-        if (last) gs_bfly_last<Lazy>(x, y, pc);
-        else gs_bfly<Lazy>(x, y, tw[ml + g], pc.q, pc.two_q);
+        if (last) gs_bfly_last<Lazy>(x, y, pc, m);
+        else gs_bfly<Lazy>(x, y, tw[ml + g], m);
       } else {
-        ct_bfly<Lazy>(x, y, tw[ml + g], pc.q, pc.two_q);
+        ct_bfly<Lazy>(x, y, tw[ml + g], m);
       }
       lds[col * L + j] = x;
       lds[col * L + j + (1u << lg_d)] = y;
     }
     __syncthreads();
   }
+  const bool epilogue = !Inverse && finish && map.epi.sub;
+  const uint64_t *sub = nullptr;
+  uint64_t *out = data;
+  ulonglong2 c{0, 0};
+  if (epilogue) {
+    out = epi_dest(map, tk, tpoly, sub);
+    c = map.epi.scale[tk];
+  }
   for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
     const uint32_t col = e % cols_per_block, j = e / cols_per_block;
+    const uint64_t at = (uint64_t)j * stride + col0 + col;
     uint64_t x = lds[col * L + j];
-    if (!Inverse && finish) x = canon<Lazy>(x, pc.q, pc.two_q);
-    data[(uint64_t)j * stride + col0 + col] = x;
+    if (!Inverse && finish) x = canon<Lazy>(x, m);
+    if (epilogue) x = shoup(sub_mod(sub[at], x, pc.q), c.x, c.y, pc.q);
+    out[at] = x;
   }
 }
 
@@ -302,7 +356,11 @@ static hipError_t launch_t(const TowerMap &map, const PrimeConst *primes, uint32
     }
   };
   auto pass_b = [&]() {
-    hipLaunchKernelGGL((ntt_pass_b<Inverse, Lazy>), dim3(R / 16, towers), dim3(256), 0, stream, map, primes, R);
+    if (!Inverse && map.epi.sub)
+      hipLaunchKernelGGL((ntt_pass_b<false, Lazy, true>), dim3(R / 16, towers), dim3(256), 0, stream, map, primes, R);
+    else
+      hipLaunchKernelGGL((ntt_pass_b<Inverse, Lazy, false>), dim3(R / 16, towers), dim3(256), 0, stream, map, primes,
+                         R);
   };
   if (!Inverse) {
     pass_a();
@@ -320,9 +378,11 @@ static hipError_t launch_t(const TowerMap &map, const PrimeConst *primes, uint32
 hipError_t launch_ntt(const TowerMap &map, const PrimeConst *primes, uint32_t log_n, uint32_t towers, bool inverse,
                       bool lazy, hipStream_t stream, uint32_t chunk) {
   if (towers == 0) return hipSuccess;
+  if (map.towers_per_poly == 0 || towers % map.towers_per_poly != 0) return hipErrorInvalidValue;
   const uint32_t step = chunk ? chunk : towers;
   for (uint32_t first = 0; first < towers; first += step) {
     TowerMap m = map;
+    m.polys = towers / map.towers_per_poly;
     m.y_offset = map.y_offset + first;
     const uint32_t count = towers - first < step ? towers - first : step;
     hipError_t e;

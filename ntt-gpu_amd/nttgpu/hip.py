@@ -9,7 +9,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "libntt_hip.so")
+# NTT_HIP_LIBRARY selects another build of the library (for example a
+# different tuning compiled elsewhere); the default is the in-tree build.
+LIB_PATH = os.environ.get("NTT_HIP_LIBRARY") or os.path.join(os.path.dirname(_HERE), "libntt_hip.so")
 
 OK, NOT_READY, EINVAL, ERUNTIME = 0, 1, -1, -2
 H2D, D2H, D2D = 1, 2, 3

@@ -1,0 +1,102 @@
+// Host-side checks of the wide-lazy NTT arithmetic (ntt-gpu_amd/csrc/modarith.hpp)
+// against exact 128-bit arithmetic: the same __host__ __device__ functions the
+// gfx950 kernels inline. Every claimed range is checked on the extremes
+// (y = 2^64 - 1, x = 8q - 1, w = q - 1, q just below 2^61) and on seeded
+// random operands for OpenFHE-sized and adversarial primes.
+#include <cstdio>
+#include <cstdlib>
+#include <initializer_list>
+#include <random>
+#include <string>
+
+#include "../../ntt-gpu_amd/csrc/modarith.hpp"
+
+using u128 = unsigned __int128;
+using namespace nttamd;
+
+namespace {
+size_t checks = 0;
+void check(bool ok, const std::string &why) {
+  ++checks;
+  if (!ok) {
+    std::fprintf(stderr, "FAIL: %s\n", why.c_str());
+    std::exit(1);
+  }
+}
+uint64_t mulmod(uint64_t a, uint64_t b, uint64_t q) { return (uint64_t)((u128)a * b % q); }
+uint64_t shoup_companion(uint64_t w, uint64_t q) { return (uint64_t)(((u128)w << 64) / q); }
+
+void one(uint64_t q, uint64_t w, uint64_t x, uint64_t y) {
+  const Lazy8 m = lazy8_of(q);
+  const uint64_t wp = shoup_companion(w, q);
+  // quotient estimate: short by 0, 1 or 2
+  const uint64_t hi = (uint64_t)(((u128)y * wp) >> 64), est = mulhi_est(y, wp);
+  check(est <= hi && hi - est <= 2, "mulhi_est within 2 of the high product");
+  // multiply: congruent, below 4q
+  const uint64_t t = shoup4(y, w, wp, m);
+  check(t < 4 * q && t % q == mulmod(y % q, w, q), "shoup4 in [0, 4q) and congruent");
+  // forward butterfly on [0, 8q)
+  uint64_t cx = x % (8 * q), cy = y % (8 * q);
+  const uint64_t fx = cx, fy = cy;
+  ct8(cx, cy, w, wp, m);
+  const uint64_t prod = mulmod(fy % q, w, q);
+  check(cx < 8 * q && cy < 8 * q, "ct8 outputs below 8q");
+  check(cx % q == (fx % q + prod) % q && cy % q == (fx % q + q - prod) % q, "ct8 congruent");
+  // inverse butterfly on [0, 4q)
+  uint64_t gx = x % (4 * q), gy = y % (4 * q);
+  const uint64_t ix = gx, iy = gy;
+  gs8(gx, gy, w, wp, m);
+  check(gx < 4 * q && gy < 4 * q, "gs8 outputs below 4q");
+  check(gx % q == (ix % q + iy % q) % q && gy % q == mulmod((ix % q + q - iy % q) % q, w, q),
+        "gs8 congruent");
+  // canonical form of [0, 8q)
+  check(canon8(x % (8 * q), m) == x % (8 * q) % q, "canon8");
+}
+}  // namespace
+
+int main() {
+  std::mt19937_64 rng(0x7251d9);
+  // OpenFHE-shaped 60- and 50-bit primes, a prime just below 2^61 and tiny ones
+  const uint64_t primes[] = {1152921504606584833ULL, 1125899906826241ULL, 2305843009213693951ULL,
+                             1152921504598720513ULL, 562949953216513ULL,  65537ULL, 17ULL};
+  for (uint64_t q : primes) {
+    const uint64_t ws[] = {0, 1, 2, q - 1, q - 2, q / 2, q / 3 + 1};
+    const uint64_t xs[] = {0, 1, q - 1, q, 2 * q - 1, 4 * q - 1, 4 * q, 8 * q - 1, ~0ULL};
+    for (uint64_t w : ws)
+      for (uint64_t x : xs)
+        for (uint64_t y : xs) one(q, w, x, y);
+    for (int i = 0; i < 200000; ++i) {
+      const uint64_t w = rng() % q;
+      const uint64_t x = rng(), y = (i & 1) ? rng() : ~0ULL - (rng() & 0xffff);
+      one(q, w, x, y);
+    }
+  }
+  // mulhi_est on arbitrary 64-bit operands (not only Shoup companions)
+  for (int i = 0; i < 1000000; ++i) {
+    const uint64_t y = rng() | ((i & 3) == 0 ? 0xffffffff00000000ULL : 0), p = rng() | ((i & 7) == 1 ? ~0ULL : 0);
+    const uint64_t hi = (uint64_t)(((u128)y * p) >> 64), est = mulhi_est(y, p);
+    check(est <= hi && hi - est <= 2, "mulhi_est on arbitrary operands");
+  }
+  // limb-split dot product of the register conversion kernel: up to 16 terms,
+  // y below its source prime, constants below 2^63, total below 2^127
+  for (int i = 0; i < 200000; ++i) {
+    const int s = 1 + (int)(rng() % 16);
+    const bool extreme = (i % 5) == 0;
+    uint64_t p[6] = {0, 0, 0, 0, 0, 0};
+    u128 want = 0;
+    for (int k = 0; k < s; ++k) {
+      // sum of source primes < 2^64: each below 2^64 / 16
+      const uint64_t y = extreme ? (~0ULL >> 4) - 1 : rng() >> 4;
+      const uint64_t h = extreme ? (~0ULL >> 1) : rng() >> 1;
+      want += (u128)y * h;
+      const uint32_t ya[2] = {(uint32_t)y, (uint32_t)(y >> 32)};
+      const uint32_t hb[3] = {(uint32_t)(h & 0x1fffff), (uint32_t)((h >> 21) & 0x1fffff), (uint32_t)(h >> 42)};
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 3; ++b) p[a * 3 + b] += (uint64_t)ya[a] * hb[b];
+    }
+    const U128 got = combine_limbs(p[0], p[1], p[2], p[3], p[4], p[5]);
+    check(got.lo == (uint64_t)want && got.hi == (uint64_t)(want >> 64), "limb-split dot product is exact");
+  }
+  std::printf("%zu ARITH CHECKS PASSED\n", checks);
+  return 0;
+}

@@ -1,6 +1,7 @@
 """The C++ include/ntt runtime, as the reference gates it (ctest binaries):
 resident_tests runs the Session on the deferred CPU transport; the GPU
-binary drives the same API over the gfx950 transport."""
+binary drives the same API over the gfx950 transport; arith_tests checks the
+NTT kernels' wide-lazy modular arithmetic on the host against 128-bit math."""
 import os
 import subprocess
 
@@ -18,6 +19,10 @@ def _run(name, timeout):
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0, r.stderr
     return r.stdout
+
+
+def test_wide_lazy_arithmetic_ranges():
+    assert "ARITH CHECKS PASSED" in _run("arith_tests", 120)
 
 
 def test_resident_runtime_on_host_transport():

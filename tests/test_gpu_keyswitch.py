@@ -157,6 +157,48 @@ def test_keyswitch_small_levels(L):
         assert np.array_equal(ga[c * L * n:(c + 1) * L * n], wa), f"A component {c}"
 
 
+def _check_keyswitch(hp, L, batch, seed):
+    x, kb, ka, gb, ga = _keyswitch(hp, L, batch, seed)
+    n = hp.n
+    for c in range(batch):
+        xs = x[c * L * n:(c + 1) * L * n]
+        wb, wa = o.key_switch(xs, n, hp.q[:L], hp.q_psi[:L], hp.M, hp.p, hp.p_psi, hp.alpha, kb, ka)
+        assert np.array_equal(gb[c * L * n:(c + 1) * L * n], wb), f"B component {c}"
+        assert np.array_equal(ga[c * L * n:(c + 1) * L * n], wa), f"A component {c}"
+
+
+@pytest.mark.parametrize("L", [12, 17])
+def test_keyswitch_many_digits(L):
+    """alpha = 1: L single-tower digits (s = 1 conversions, long MAC sums); at
+    L = 17, 17 * 2^60 >= 2^64 selects the MAC's per-term reduction."""
+    hp = params.openfhe_like(1024, L, 1, 1)
+    log_case(f"keyswitch n=1024 L={L} K=1 alpha=1 ({L} digits) batch=3")
+    _check_keyswitch(hp, L, 3, 640 + L)
+
+
+def test_keyswitch_63bit_primes_per_term():
+    """63-bit primes: canonical NTT, per-term basis conversion and per-term
+    MAC (4 digits * 2^62 >= 2^64), batch of 5."""
+    n = 1024
+    qs = params.find_primes(n, 6, 63)
+    hp = params.HybridParams(n=n, q=qs[:4], p=qs[4:], alpha=1)
+    log_case("keyswitch n=1024 63-bit primes L=4 K=2 alpha=1 batch=5")
+    _check_keyswitch(hp, 4, 5, 650)
+
+
+def test_keyswitch_wide_lazy_boundary_primes():
+    """Every prime just below 2^61 (the edge of the wide-lazy NTT range) at
+    N=8192, where the ModDown correction runs in the NTT's last pass on lazy
+    values: z + 4q - v must stay below 2^64."""
+    n = 8192
+    qs = [params.last_prime(61, 2 * n)]
+    while len(qs) < 5:
+        qs.append(params.previous_prime(qs[-1], 2 * n))
+    hp = params.HybridParams(n=n, q=qs[:3], p=qs[3:], alpha=2)
+    log_case("keyswitch n=8192 primes below 2^61 L=3 K=2 alpha=2 batch=2")
+    _check_keyswitch(hp, 3, 2, 660)
+
+
 def test_keyswitch_north_star_shape():
     """N=2^16, L=24 (~50-bit, first 60-bit), K=6 (60-bit), dnum=4 (alpha=6), batch of 2."""
     hp = params.openfhe_like(65536, 24, 6, 6)

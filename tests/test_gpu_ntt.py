@@ -111,6 +111,22 @@ def test_wide_primes_canonical_path(n):
     assert np.array_equal(_gpu_round(ctx, f, 0, 3, True), x)
 
 
+@pytest.mark.parametrize("n", [4096, 65536])
+def test_wide_lazy_boundary_primes(n):
+    """Largest NTT primes below 2^61 (8q just under 2^64: the edge of the
+    wide-lazy range) with all-(q-1) and seeded inputs, both directions."""
+    qs = [params.last_prime(61, 2 * n)]
+    qs.append(params.previous_prime(qs[0], 2 * n))
+    rs = [params.find_psi(q, n) for q in qs]
+    ctx = hip.Context(n, qs, rs)
+    log_case(f"ntt wide-lazy boundary n={n} q={qs}")
+    for x in (np.concatenate([np.full(n, q - 1, np.uint64) for q in qs]), uniform(n, qs, 61)):
+        f = _gpu_round(ctx, x, 0, 2, False)
+        assert np.array_equal(f, o.ntt_towers(x, n, qs, rs))
+        assert np.array_equal(_gpu_round(ctx, f, 0, 2, True), x)
+        assert np.array_equal(_gpu_round(ctx, x, 0, 2, True), o.ntt_towers(x, n, qs, rs, inverse=True))
+
+
 def test_out_of_place_and_subrange():
     qs = params.mixed_width_primes(N16, 6)
     rs = [params.min_root(q, N16) for q in qs]

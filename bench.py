@@ -11,6 +11,7 @@ import argparse
 import json
 import os
 import statistics
+import subprocess
 import sys
 import time
 
@@ -24,6 +25,11 @@ from nttgpu import hip, params  # noqa: E402
 N = 65536
 L, K, ALPHA = 24, 6, 6
 HBM_PEAK_GBS = 8000.0
+BUTTERFLIES_PER_TOWER = (N // 2) * 16  # one direction of a 2^16-point transform
+# HBM bytes of the roofline launch measured by rocprofv3 --pmc FETCH_SIZE /
+# WRITE_SIZE (per dispatch, calibrated on a copy kernel of known size)
+HBM_COUNTERS = os.path.join(ROOT, "profiles", "ntt_n65536_hbm_counters.json")
+VALU_PROBE = os.path.join(ROOT, "ntt-gpu_amd", "bin", "valu_probe")
 
 
 T0 = time.perf_counter()
@@ -78,12 +84,63 @@ def ntt_roofline(towers, reps, stream):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": measured_traffic(towers),
         "kernel": "forward NTT (ntt_pass_a256 + ntt_pass_b), N=65536",
         "towers_per_launch": towers,
         "launch_ms": round(ms, 4),
         "inverse_launch_ms": round(statistics.mean(inv), 4),
         "alg_bytes_per_launch": alg_bytes,
+    }
+
+
+def measured_traffic(towers):
+    """HBM bytes of one forward roofline launch over `towers` towers, from the
+    committed counter summary (FETCH_SIZE and WRITE_SIZE per dispatch of both
+    passes, each scaled by the copy-kernel calibration), per tower x towers."""
+    try:
+        with open(HBM_COUNTERS) as f:
+            c = json.load(f)
+    except (OSError, ValueError):
+        return None
+    per_tower = 0.0
+    for kern in ("pass_a_forward", "pass_b_forward"):
+        k = c["kernels"][kern]
+        per_tower += (k["fetch_kib"] * c["fetch_scale"] + k["write_kib"] * c["write_scale"]) * 1024 / c["towers"]
+    return int(per_tower * towers)
+
+
+def valu_roofline(roof):
+    """The forward NTT against the VALU roof measured on this device: the rate
+    of the same wide-lazy forward butterfly in independent chains
+    (ntt-gpu_amd/bin/valu_probe), versus butterflies per second achieved by
+    the roofline launch (16 stages x N/2 butterflies per tower)."""
+    if not os.path.exists(VALU_PROBE):
+        return None
+    try:
+        r = subprocess.run([VALU_PROBE, "--json"], capture_output=True, text=True, timeout=120)
+    except (OSError, subprocess.TimeoutExpired):
+        return None
+    if r.returncode != 0:
+        return None
+    rows = {}
+    for line in r.stdout.splitlines():
+        try:
+            d = json.loads(line)
+            rows[d["probe"]] = d
+        except ValueError:
+            continue
+    if "ct8_forward_butterfly" not in rows:
+        return None
+    peak = rows["ct8_forward_butterfly"]["per_s"]
+    achieved = BUTTERFLIES_PER_TOWER * roof["towers_per_launch"] / (roof["launch_ms"] * 1e-3)
+    return {
+        "bound": "valu",
+        "achieved": round(achieved / 1e12, 4),
+        "peak": round(peak / 1e12, 4),
+        "unit": "T butterfly/s",
+        "frac": round(achieved / peak, 4),
+        "peak_source": "valu_probe: forward butterfly chains on this device",
+        "probe": {k: round(v["per_s"] / 1e12, 4) for k, v in rows.items()},
     }
 
 
@@ -255,6 +312,9 @@ def main():
     progress(f"plain done: {plain_kss:.1f} KS/s")
     roof = ntt_roofline(args.ntt_towers, 6, s)
     progress(f"ntt roofline done: {roof['achieved']} GB/s")
+    valu = valu_roofline(roof)
+    if valu:
+        progress(f"valu roofline: {valu['achieved']} of {valu['peak']} T butterfly/s")
     served = None
     if args.serve_requests:
         served = served_stream(args.serve_requests, [float(r) for r in args.serve_rates.split(",")])
@@ -286,6 +346,7 @@ def main():
         "plain_p99_ms": round(pct(lat, 0.99), 3),
         "batched_over_plain": round(batched_kss / plain_kss, 3),
         "roofline": roof,
+        "valu_roofline": valu,
         "cpu_baseline": cpu,
         "bit_exact_vs_oracle": exact,
         "served_stream": served,

@@ -25,9 +25,12 @@
 
 namespace nttamd {
 
-constexpr int kRow = 256;          // pass-B sub-transform length
+constexpr int kRow = 256;     // pass-B sub-transform length
 constexpr int kLdsRow = 256 + 16 + 1;  // padded LDS row (one word per 16, plus one)
 
+// Padding (rather than an XOR swizzle, measured slower: its index arithmetic
+// costs more VALU than the extra workgroup per CU it allows) keeps both
+// exchange patterns free of bank conflicts.
 __device__ __forceinline__ int lds_idx(int row, int j) { return row * kLdsRow + j + (j >> 4); }
 
 // Twiddle pairs {W, floor(W 2^64 / q)} read through a global-address-space
@@ -190,10 +193,9 @@ __global__ __launch_bounds__(256) void ntt_pass_a256(TowerMap map, const PrimeCo
 
 // ---------------------------------------------------------------- pass B (rows of 256)
 // R = rows in the tower (N / 256); row_factor for ml is R + r.
-// Epi (forward only): the ModDown correction epilogue. Its subtrahend is
-// loaded at the start (overlapping the butterflies) at the thread's final
-// positions 16u..16u+15, and the result is written there directly, without
-// the exchange back to the coalesced layout.
+// Epi (forward only): the ModDown correction epilogue, applied after the
+// exchange back to the coalesced layout (measured faster than loading the
+// subtrahend up front at the uncoalesced positions).
 template <bool Inverse, bool Lazy, bool Epi>
 __global__ __launch_bounds__(256) void ntt_pass_b(TowerMap map, const PrimeConst *__restrict__ primes, uint32_t R) {
   __shared__ uint64_t lds[16 * kLdsRow];
@@ -213,12 +215,6 @@ __global__ __launch_bounds__(256) void ntt_pass_b(TowerMap map, const PrimeConst
 #pragma unroll
   for (int k = 0; k < 16; ++k) v[k] = src_row[u + 16 * k];
   if constexpr (Epi) {
-    const uint64_t *sub;
-    uint64_t *out = epi_dest(map, tk, tpoly, sub) + (uint64_t)r * kRow + 16 * u;
-    sub += (uint64_t)r * kRow + 16 * u;
-    uint64_t z[16];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) z[k] = sub[k];
     reg_stages<false, Lazy>(v, tw, 1, rf, 0, m);
 #pragma unroll
     for (int k = 0; k < 16; ++k) lds[lds_idx(rl, u + 16 * k)] = v[k];
@@ -226,12 +222,19 @@ __global__ __launch_bounds__(256) void ntt_pass_b(TowerMap map, const PrimeConst
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = lds[lds_idx(rl, 16 * u + k)];
     reg_stages<false, Lazy>(v, tw, 16, rf, u, m);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) lds[lds_idx(rl, 16 * u + k)] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = lds[lds_idx(rl, u + 16 * k)];
+    const uint64_t *sub;
+    uint64_t *out = epi_dest(map, tk, tpoly, sub) + (uint64_t)r * kRow;
+    sub += (uint64_t)r * kRow;
     const ulonglong2 c = map.epi.scale[tk];
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-      // lazy: v in [0, 8q) -> [0, 4q); z + 4q - v in (0, 5q) < 2^64; Shoup takes any word
-      const uint64_t d = Lazy ? z[k] + m.q4 - sub_if_ge(v[k], m.q4) : sub_mod(z[k], v[k], pc.q);
-      out[k] = shoup(d, c.x, c.y, pc.q);
+      const uint64_t d = Lazy ? sub[u + 16 * k] + m.q4 - sub_if_ge(v[k], m.q4) : sub_mod(sub[u + 16 * k], v[k], pc.q);
+      out[u + 16 * k] = shoup(d, c.x, c.y, pc.q);
     }
     return;
   } else if constexpr (!Inverse) {

@@ -144,6 +144,44 @@ def valu_roofline(roof):
     }
 
 
+def hoisted_rotations(ctx, hp, x, stream, R, B, steps=5):
+    """R hoisted rotations (one ModUp shared by all, per rotation MAC +
+    automorphism + ModDown: ntt_hip_keyswitch_rotate) over B ciphertexts per
+    call; rotations/s counts rotated ciphertexts (R * B per call)."""
+    comp = L * N
+    keys = []
+    for r in range(R):
+        ks = hip.KeySwitch(ctx, L, K, ALPHA, max_batch=B)
+        full = hp.q + hp.p
+        kb = np.concatenate([seeded(N, full, 7000 + 31 * r + d) for d in range(hp.beta())])
+        ka = np.concatenate([seeded(N, full, 8000 + 31 * r + d) for d in range(hp.beta())])
+        ks.upload_key(kb, ka, hp.M, hp.q)
+        keys.append(ks)
+    autos = [pow(5, r + 1, 2 * N) for r in range(R)]
+    c0 = hip.DeviceBuffer(B * comp).upload(x[: B * comp])
+    c1 = hip.DeviceBuffer(B * comp).upload(x[: B * comp][::-1].copy())
+    outs = [(hip.DeviceBuffer(B * comp), hip.DeviceBuffer(B * comp)) for _ in range(R)]
+    ws = hip.DeviceBuffer(keys[0].rotate_workspace_words(B))
+
+    def call():
+        hip.KeySwitch.rotate(keys, autos, c0.ptr, c1.ptr, B, [o[0].ptr for o in outs], [o[1].ptr for o in outs],
+                             ws.ptr, stream)
+    call()
+    hip.Stream(0, stream).synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(stream)
+    for _ in range(steps):
+        call()
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / steps
+    for ks in keys:
+        ks.close()
+    return {"rotations": R, "ciphertexts": B, "ms_per_call": round(ms, 3),
+            "rotations_per_s": round(R * B / (ms * 1e-3), 1),
+            "note": "one ModUp of c1 shared by the R rotations; per rotation MAC, automorphism, ModDown"}
+
+
 def served_stream(requests, rates, seed=0xC5):
     """Config C5: Poisson arrivals of mixed N/L/dnum/op requests at swept rates
     into the continuous-batching server; p50/p99 latency and throughput per
@@ -229,6 +267,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=0)
     ap.add_argument("--roofline-only", action="store_true")
     ap.add_argument("--serve-rates", type=str, default="200,1000")
+    ap.add_argument("--rotations", type=int, default=8)
+    ap.add_argument("--rotation-batch", type=int, default=16)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -310,6 +350,8 @@ def main():
     plain_kss = args.plain_requests / plain_wall
 
     progress(f"plain done: {plain_kss:.1f} KS/s")
+    rot = hoisted_rotations(ctx, hp, x, s, args.rotations, min(B, args.rotation_batch))
+    progress(f"hoisted rotations: {rot['rotations_per_s']} rotations/s")
     roof = ntt_roofline(args.ntt_towers, 6, s)
     progress(f"ntt roofline done: {roof['achieved']} GB/s")
     valu = valu_roofline(roof)
@@ -345,6 +387,7 @@ def main():
         "plain_p50_ms": round(pct(lat, 0.5), 3),
         "plain_p99_ms": round(pct(lat, 0.99), 3),
         "batched_over_plain": round(batched_kss / plain_kss, 3),
+        "hoisted_rotations": rot,
         "roofline": roof,
         "valu_roofline": valu,
         "cpu_baseline": cpu,

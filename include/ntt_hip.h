@@ -164,6 +164,20 @@ int ntt_hip_moddown_workspace_bytes(ntt_hip_keyswitch *ks, size_t batch, size_t 
 int ntt_hip_moddown_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batch, uint64_t *out, void *workspace,
                         void *stream);
 int ntt_hip_graph_destroy(void *graph);
+/* Hoisted rotations (reference openfhe_hybrid_adapter.hpp:307-360
+ * hybrid_rotate_many over Session::hybrid_digits/hybrid_accumulate/
+ * hybrid_rotate_extended/mod_down, src/resident.cpp:458-617): the digits of
+ * c1 are computed once for all `rotations`; for rotation r, the key inner
+ * product with keys[r]'s resident key, sigma_k(B + P c0) and sigma_k(A) on
+ * the QP basis (k = automorphisms[r], odd, < 2n; computed before ModDown as
+ * OpenFHE does) and ModDown of both channels into out_b[r] / out_a[r]
+ * (batch * L * n each, evaluation Q). c0, c1: batch * L * n evaluation
+ * residues. keys[0..rotations) are plans of one context with equal L, K,
+ * alpha; keys[0] sizes the workspace (ntt_hip_rotate_workspace_bytes). */
+int ntt_hip_rotate_workspace_bytes(ntt_hip_keyswitch *ks, size_t batch, size_t *bytes);
+int ntt_hip_keyswitch_rotate(ntt_hip_keyswitch *const *keys, size_t rotations, const uint32_t *automorphisms,
+                             const uint64_t *c0, const uint64_t *c1, size_t batch, uint64_t *const *out_b,
+                             uint64_t *const *out_a, void *workspace, void *stream);
 
 #pragma GCC visibility pop
 #ifdef __cplusplus

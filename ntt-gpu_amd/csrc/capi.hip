@@ -363,6 +363,7 @@ struct ntt_hip_keyswitch {
   std::shared_ptr<BasisPlanDev> down;             // P -> Q
   std::shared_ptr<BasisPlanDev> full_up;          // all active Q -> P (ModUp)
   const ulonglong2 *pinv = nullptr;               // P^-1 mod q_i, i < L
+  const ulonglong2 *pmod = nullptr;               // P mod q_i, i < L (rotation: B + P c0)
   std::unique_ptr<DeviceBlob> tower_prime;        // identity map 0..L+K
   std::unique_ptr<DeviceBlob> jobs;               // complement NTT jobs for max_batch
   size_t jobs_per_poly = 0;
@@ -731,7 +732,7 @@ int ntt_hip_hybrid_rotate(ntt_hip_context *ctx, uint64_t *out_b, uint64_t *out_a
     r.active_q = (uint32_t)active_q;
     r.automorphism = automorphism;
     DeviceGuard g(ctx->device);
-    hip_ok(launch_rotate(r, (uint32_t)towers, ctx->pick(stream)), "rotation launch");
+    hip_ok(launch_rotate(r, (uint32_t)towers, 1, ctx->pick(stream)), "rotation launch");
     return NTT_HIP_OK;
   });
 }
@@ -770,6 +771,7 @@ int ntt_hip_keyswitch_create(ntt_hip_keyswitch **out, ntt_hip_context *ctx, size
     ks->down = basis_plan(*ctx, p, q);
     ks->full_up = basis_plan(*ctx, q, p);
     ks->pinv = scale_table(*ctx, 0, L, p, 0);
+    ks->pmod = scale_table(*ctx, 0, L, p, 1);
     std::vector<uint32_t> tp(T);
     for (size_t i = 0; i < T; ++i) tp[i] = (uint32_t)i;
     ks->tower_prime = std::make_unique<DeviceBlob>(ctx->device, T * sizeof(uint32_t));
@@ -850,6 +852,118 @@ int ntt_hip_keyswitch_upload_key(ntt_hip_keyswitch *ks, const uint64_t *key_b, c
   });
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------- key-switch stages
+namespace {
+struct KsBuffers {
+  uint64_t *coeff, *digits, *z, *pc, *corr;
+};
+
+KsBuffers ks_buffers(const ntt_hip_keyswitch &ks, void *workspace, size_t B) {
+  KsBuffers w;
+  w.coeff = (uint64_t *)workspace;                   // B * coeff_words
+  w.digits = w.coeff + B * ks.coeff_words();         // B * digit_words (beta digits of T towers)
+  w.z = w.digits + B * ks.digit_words();             // [2][B] * z_words (b channel, then a)
+  w.pc = w.z + 2 * B * ks.z_words();                 // [2][B] * pc_words
+  w.corr = w.pc + 2 * B * ks.pc_words();             // [2][B] * corr_words
+  return w;
+}
+
+// ModUp of B components (evaluation Q input, stride L*n): coefficient form of
+// every Q tower, per digit the unreduced CRT into its complement, then the
+// forward NTT of every complement tower (one launch, jobs sorted by prime).
+void ks_modup(const ntt_hip_keyswitch &ks, const uint64_t *in, size_t B, const KsBuffers &w, hipStream_t s) {
+  const ntt_hip_context &c = *ks.ctx;
+  const size_t L = ks.L, T = ks.L + ks.K, n = c.n;
+  const PrimeConst *pr = c.dprimes();
+  hip_ok(launch_ntt(affine(w.coeff, in, n, L, 0, ks.coeff_words(), L * n), pr, c.log_n, (uint32_t)(B * L), true,
+                    c.lazy, s, c.ntt_chunk),
+         "intt");
+  for (uint32_t d = 0; d < ks.beta; ++d) {
+    const auto &plan = *ks.up[d];
+    const uint32_t first = d * ks.alpha, count = std::min<uint32_t>(ks.alpha, ks.L - first);
+    BasisLaunch a{};
+    a.plan = plan.dev;
+    a.in = w.coeff + (size_t)first * n;
+    a.out = w.digits + (size_t)d * T * n;
+    a.in_poly_stride = ks.coeff_words();
+    a.out_poly_stride = ks.digit_words();
+    a.n = (uint32_t)n;
+    a.targets_per_block = c.conv_targets;
+    a.plan_host_s = plan.s;
+    a.plan_host_t = plan.t;
+    a.skip_at = first;
+    a.skip_len = count;
+    hip_ok(launch_basis_switch(a, plan.per_term, (uint32_t)B, s), "modup conversion");
+  }
+  TowerMap m{};
+  m.base = w.digits;
+  m.jobs = (const TowerJob *)ks.jobs->ptr;
+  m.tower_stride = n;
+  m.poly_stride = ks.digit_words();
+  m.towers_per_poly = (uint32_t)ks.jobs_per_poly;
+  hip_ok(launch_ntt(m, pr, c.log_n, (uint32_t)(B * ks.jobs_per_poly), false, c.lazy, s, c.ntt_chunk), "modup ntt");
+}
+
+// Key inner product over all digits of `plan`'s workspace with `key`'s
+// resident key, both channels, into z ([2][B] * z_words).
+void ks_mac(const ntt_hip_keyswitch &key, const ntt_hip_keyswitch &plan, const uint64_t *in, size_t B,
+            const KsBuffers &w, uint64_t *z, hipStream_t s) {
+  const size_t T = plan.L + plan.K, n = plan.n();
+  MacLaunch m{};
+  m.own = in;
+  m.own_poly_stride = plan.L * n;
+  m.alpha = plan.alpha;
+  m.active_q = plan.L;
+  m.digits = w.digits;
+  m.key_b = (const uint64_t *)key.key->ptr;
+  m.key_a = m.key_b + (size_t)plan.beta * T * n;
+  m.out_b = z;
+  m.out_a = z + B * plan.z_words();
+  m.primes = plan.ctx->dprimes();
+  m.tower_prime = (const uint32_t *)plan.tower_prime->ptr;
+  m.digit_poly_stride = plan.digit_words();
+  m.out_poly_stride = plan.z_words();
+  m.n = (uint32_t)n;
+  m.towers = (uint32_t)T;
+  m.digits_count = plan.beta;
+  hip_ok(launch_mac_fused(m, key.mac_per_term, (uint32_t)B, s), "mac");
+}
+
+// ModDown of both channels of z ([2][B] * z_words, evaluation QP): INTT(P) ->
+// CRT to Q -> NTT with (Z_Q - NTT(corr)) P^-1 fused into its last pass;
+// channel b lands in out_b, channel a in out_a (B * L * n each).
+void ks_moddown(const ntt_hip_keyswitch &ks, const uint64_t *z, size_t B, uint64_t *out_b, uint64_t *out_a,
+                const KsBuffers &w, hipStream_t s) {
+  const ntt_hip_context &c = *ks.ctx;
+  const size_t L = ks.L, K = ks.K, n = c.n;
+  const PrimeConst *pr = c.dprimes();
+  hip_ok(launch_ntt(affine(w.pc, z + L * n, n, K, L, ks.pc_words(), ks.z_words()), pr, c.log_n,
+                    (uint32_t)(2 * B * K), true, c.lazy, s, c.ntt_chunk),
+         "moddown intt");
+  BasisLaunch a{};
+  a.plan = ks.down->dev;
+  a.in = w.pc;
+  a.out = w.corr;
+  a.in_poly_stride = ks.pc_words();
+  a.out_poly_stride = ks.corr_words();
+  a.n = (uint32_t)n;
+  a.targets_per_block = c.conv_targets;
+  a.plan_host_s = ks.down->s;
+  a.plan_host_t = ks.down->t;
+  a.skip_at = 0xffffffffu;
+  a.skip_len = 0;
+  hip_ok(launch_basis_switch(a, ks.down->per_term, (uint32_t)(2 * B), s), "moddown conversion");
+  hip_ok(launch_ntt(with_correction(affine(w.corr, nullptr, n, L, 0, ks.corr_words(), 0), z, ks.z_words(), out_b,
+                                    out_a, (uint32_t)B, L * n, ks.pinv),
+                    pr, c.log_n, (uint32_t)(2 * B * L), false, c.lazy, s, c.ntt_chunk),
+         "moddown ntt + correction");
+}
+}  // namespace
+
+extern "C" {
+
 int ntt_hip_keyswitch_workspace_bytes(ntt_hip_keyswitch *ks, size_t batch, size_t *bytes) {
   return guarded([&] {
     require(ks && bytes, "null argument");
@@ -860,100 +974,72 @@ int ntt_hip_keyswitch_workspace_bytes(ntt_hip_keyswitch *ks, size_t batch, size_
 }
 
 int ntt_hip_keyswitch_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batch, uint64_t *out_b,
-                                     uint64_t *out_a, void *workspace, void *stream) {
+                          uint64_t *out_a, void *workspace, void *stream) {
   return guarded([&] {
     require(ks && in && out_b && out_a && workspace, "null argument");
     require(ks->key_loaded, "evaluation key not uploaded");
     require(batch >= 1 && batch <= ks->max_batch, "batch exceeds the plan capacity");
-    ntt_hip_context &c = *ks->ctx;
-    const size_t L = ks->L, K = ks->K, T = L + K, n = c.n, B = batch;
+    const hipStream_t s = ks->ctx->pick(stream);
+    const KsBuffers w = ks_buffers(*ks, workspace, batch);
+    DeviceGuard g(ks->ctx->device);
+    ks_modup(*ks, in, batch, w, s);
+    ks_mac(*ks, *ks, in, batch, w, w.z, s);
+    ks_moddown(*ks, w.z, batch, out_b, out_a, w, s);
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_rotate_workspace_bytes(ntt_hip_keyswitch *ks, size_t batch, size_t *bytes) {
+  return guarded([&] {
+    require(ks && bytes, "null argument");
+    require(batch >= 1 && batch <= ks->max_batch, "batch exceeds the plan capacity");
+    *bytes = batch * (ks->per_component() + 2 * ks->z_words()) * sizeof(uint64_t);
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_keyswitch_rotate(ntt_hip_keyswitch *const *keys, size_t rotations, const uint32_t *automorphisms,
+                             const uint64_t *c0, const uint64_t *c1, size_t batch, uint64_t *const *out_b,
+                             uint64_t *const *out_a, void *workspace, void *stream) {
+  return guarded([&] {
+    require(keys && automorphisms && c0 && c1 && out_b && out_a && workspace && rotations >= 1, "null argument");
+    ntt_hip_keyswitch &plan = *keys[0];
+    require(batch >= 1 && batch <= plan.max_batch, "batch exceeds the plan capacity");
+    for (size_t r = 0; r < rotations; ++r) {
+      const ntt_hip_keyswitch *k = keys[r];
+      require(k && k->ctx == plan.ctx && k->L == plan.L && k->K == plan.K && k->alpha == plan.alpha,
+              "rotation keys must share the context and the (L, K, alpha) plan");
+      require(k->key_loaded, "rotation key not uploaded");
+      require(out_b[r] && out_a[r], "null rotation output");
+      require((automorphisms[r] & 1) && automorphisms[r] < 2 * plan.ctx->n, "invalid negacyclic automorphism index");
+    }
+    ntt_hip_context &c = *plan.ctx;
     const hipStream_t s = c.pick(stream);
-    uint64_t *ws = (uint64_t *)workspace;
-    uint64_t *coeff = ws;                              // B * L * n
-    uint64_t *digits = coeff + B * ks->coeff_words();  // B * beta * T * n
-    uint64_t *z = digits + B * ks->digit_words();      // [2][B] * T * n (b channel, then a)
-    uint64_t *pc = z + 2 * B * ks->z_words();          // [2][B] * K * n
-    uint64_t *corr = pc + 2 * B * ks->pc_words();      // [2][B] * L * n
-    const PrimeConst *pr = c.dprimes();
+    const KsBuffers w = ks_buffers(plan, workspace, batch);
+    uint64_t *rotated = (uint64_t *)workspace + batch * plan.per_component();  // [2][B] * z_words
+    const size_t L = plan.L, T = plan.L + plan.K, n = c.n;
     DeviceGuard g(c.device);
-    // 1. ModUp sources: coefficient form of every Q tower (INTT, out of place)
-    hip_ok(launch_ntt(affine(coeff, in, n, L, 0, ks->coeff_words(), L * n), pr, c.log_n, (uint32_t)(B * L), true,
-                      c.lazy, s, c.ntt_chunk),
-           "intt");
-    // 2. per digit: unreduced CRT into the complement, around the digit's own towers
-    for (uint32_t d = 0; d < ks->beta; ++d) {
-      const auto &plan = *ks->up[d];
-      const uint32_t first = d * ks->alpha, count = std::min<uint32_t>(ks->alpha, ks->L - first);
-      BasisLaunch a{};
-      a.plan = plan.dev;
-      a.in = coeff + (size_t)first * n;
-      a.out = digits + (size_t)d * T * n;
-      a.in_poly_stride = ks->coeff_words();
-      a.out_poly_stride = ks->digit_words();
-      a.n = (uint32_t)n;
-      a.targets_per_block = c.conv_targets;
-      a.plan_host_s = plan.s;
-      a.plan_host_t = plan.t;
-      a.skip_at = first;
-      a.skip_len = count;
-      hip_ok(launch_basis_switch(a, plan.per_term, (uint32_t)B, s), "modup conversion");
+    ks_modup(plan, c1, batch, w, s);  // digits once for every rotation
+    for (size_t r = 0; r < rotations; ++r) {
+      ks_mac(*keys[r], plan, c1, batch, w, w.z, s);
+      RotateLaunch rl{};
+      rl.out_b = rotated;
+      rl.out_a = rotated + batch * plan.z_words();
+      rl.ext_b = w.z;
+      rl.ext_a = w.z + batch * plan.z_words();
+      rl.c0 = c0;
+      rl.primes = c.dprimes();
+      rl.scale = plan.pmod;
+      rl.first_prime = 0;
+      rl.n = (uint32_t)n;
+      rl.log_n = c.log_n;
+      rl.active_q = (uint32_t)L;
+      rl.automorphism = automorphisms[r];
+      rl.ext_poly_stride = rl.out_poly_stride = plan.z_words();
+      rl.c0_poly_stride = L * n;
+      hip_ok(launch_rotate(rl, (uint32_t)T, (uint32_t)batch, s), "rotation");
+      ks_moddown(plan, rotated, batch, out_b[r], out_a[r], w, s);
     }
-    // 3. forward NTT of every complement tower of every digit (one launch)
-    {
-      TowerMap m{};
-      m.base = digits;
-      m.jobs = (const TowerJob *)ks->jobs->ptr;
-      m.tower_stride = n;
-      m.poly_stride = ks->digit_words();
-      m.towers_per_poly = (uint32_t)ks->jobs_per_poly;
-      hip_ok(launch_ntt(m, pr, c.log_n, (uint32_t)(B * ks->jobs_per_poly), false, c.lazy, s, c.ntt_chunk), "modup ntt");
-    }
-    // 4. key inner product over all digits, both channels
-    {
-      MacLaunch m{};
-      m.own = in;
-      m.own_poly_stride = L * n;
-      m.alpha = ks->alpha;
-      m.active_q = ks->L;
-      m.digits = digits;
-      m.key_b = (const uint64_t *)ks->key->ptr;
-      m.key_a = m.key_b + (size_t)ks->beta * T * n;
-      m.out_b = z;
-      m.out_a = z + B * ks->z_words();
-      m.primes = pr;
-      m.tower_prime = (const uint32_t *)ks->tower_prime->ptr;
-      m.digit_poly_stride = ks->digit_words();
-      m.out_poly_stride = ks->z_words();
-      m.n = (uint32_t)n;
-      m.towers = (uint32_t)T;
-      m.digits_count = ks->beta;
-      hip_ok(launch_mac_fused(m, ks->mac_per_term, (uint32_t)B, s), "mac");
-    }
-    // 5. ModDown of both channels (2B polynomials): INTT(P) -> CRT to Q -> NTT -> correct
-    hip_ok(launch_ntt(affine(pc, z + L * n, n, K, L, ks->pc_words(), ks->z_words()), pr, c.log_n,
-                      (uint32_t)(2 * B * K), true, c.lazy, s, c.ntt_chunk),
-           "moddown intt");
-    {
-      BasisLaunch a{};
-      a.plan = ks->down->dev;
-      a.in = pc;
-      a.out = corr;
-      a.in_poly_stride = ks->pc_words();
-      a.out_poly_stride = ks->corr_words();
-      a.n = (uint32_t)n;
-      a.targets_per_block = c.conv_targets;
-      a.plan_host_s = ks->down->s;
-      a.plan_host_t = ks->down->t;
-      a.skip_at = 0xffffffffu;
-      a.skip_len = 0;
-      hip_ok(launch_basis_switch(a, ks->down->per_term, (uint32_t)(2 * B), s), "moddown conversion");
-    }
-    // NTT of the lifted correction with (Z_Q - NTT(corr)) * P^-1 fused into its
-    // last pass: channel b's B polynomials land in out_b, channel a's in out_a
-    hip_ok(launch_ntt(with_correction(affine(corr, nullptr, n, L, 0, ks->corr_words(), 0), z, ks->z_words(), out_b, out_a,
-                                      (uint32_t)B, L * n, ks->pinv),
-                      pr, c.log_n, (uint32_t)(2 * B * L), false, c.lazy, s, c.ntt_chunk),
-           "moddown ntt + correction");
     return NTT_HIP_OK;
   });
 }

@@ -55,12 +55,15 @@ struct CorrectLaunch {
   uint32_t first_prime, n, towers;
 };
 
+// sigma_k(B + P c0), sigma_k(A) for `polys` polynomials: polynomial j at
+// out + j * out_poly_stride, ext + j * ext_poly_stride, c0 + j * c0_poly_stride.
 struct RotateLaunch {
   uint64_t *out_b, *out_a;
   const uint64_t *ext_b, *ext_a, *c0;
   const PrimeConst *primes;
   const ulonglong2 *scale;  // [active_q] {P mod q, Shoup}
   uint32_t first_prime, n, log_n, active_q, automorphism;
+  uint64_t ext_poly_stride, out_poly_stride, c0_poly_stride;
 };
 
 hipError_t launch_basis_switch(const BasisLaunch &a, bool per_term, uint32_t polys, hipStream_t stream);
@@ -73,7 +76,7 @@ hipError_t launch_to_mont(uint64_t *data, const PrimeConst *primes, const uint32
 hipError_t launch_correct(const CorrectLaunch &c, uint32_t polys, hipStream_t stream);
 hipError_t launch_pointwise(uint64_t *out, const uint64_t *right, const PrimeConst *primes, uint32_t first_prime,
                             uint32_t n, uint32_t towers, hipStream_t stream);
-hipError_t launch_rotate(const RotateLaunch &r, uint32_t towers, hipStream_t stream);
+hipError_t launch_rotate(const RotateLaunch &r, uint32_t towers, uint32_t polys, hipStream_t stream);
 
 }  // namespace nttamd
 

@@ -277,25 +277,28 @@ hipError_t launch_pointwise(uint64_t *out, const uint64_t *right, const PrimeCon
 // out_b[t][k] = ext_b[t][src] + P*c0[t][src] on Q towers, out_a[t][k] = ext_a[t][src].
 __device__ __forceinline__ uint32_t brev(uint32_t x, uint32_t lg) { return __brev(x) >> (32 - lg); }
 
-__global__ __launch_bounds__(256) void rotate_kernel(RotateLaunch r) {
+// blockIdx.y = tower * polys + poly.
+__global__ __launch_bounds__(256) void rotate_kernel(RotateLaunch r, uint32_t polys) {
   const uint32_t k = blockIdx.x * 256 + threadIdx.x;
-  const uint32_t tower = blockIdx.y;
+  const uint32_t tower = blockIdx.y / polys, poly = blockIdx.y - tower * polys;
   if (k >= r.n) return;
   const uint32_t odd = 2 * brev(k, r.log_n) + 1;
   const uint32_t src_exp = (uint32_t)(((uint64_t)odd * r.automorphism) & (2ull * r.n - 1));
   const uint32_t src = brev(src_exp >> 1, r.log_n);
   const uint64_t at = (uint64_t)tower * r.n;
-  uint64_t b = r.ext_b[at + src];
+  const uint64_t *eb = r.ext_b + poly * r.ext_poly_stride, *ea = r.ext_a + poly * r.ext_poly_stride;
+  uint64_t b = eb[at + src];
   if (tower < r.active_q) {
     const uint64_t q = r.primes[r.first_prime + tower].q;
-    b = add_mod(b, shoup(r.c0[at + src], r.scale[tower].x, r.scale[tower].y, q), q);
+    const uint64_t *c0 = r.c0 + poly * r.c0_poly_stride;
+    b = add_mod(b, shoup(c0[at + src], r.scale[tower].x, r.scale[tower].y, q), q);
   }
-  r.out_b[at + k] = b;
-  r.out_a[at + k] = r.ext_a[at + src];
+  r.out_b[poly * r.out_poly_stride + at + k] = b;
+  r.out_a[poly * r.out_poly_stride + at + k] = ea[at + src];
 }
 
-hipError_t launch_rotate(const RotateLaunch &r, uint32_t towers, hipStream_t stream) {
-  hipLaunchKernelGGL(rotate_kernel, dim3((r.n + 255) / 256, towers), dim3(256), 0, stream, r);
+hipError_t launch_rotate(const RotateLaunch &r, uint32_t towers, uint32_t polys, hipStream_t stream) {
+  hipLaunchKernelGGL(rotate_kernel, dim3((r.n + 255) / 256, towers * polys), dim3(256), 0, stream, r, polys);
   return hipGetLastError();
 }
 

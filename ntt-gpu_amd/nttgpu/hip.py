@@ -67,6 +67,9 @@ SIGNATURES = {
     "ntt_hip_keyswitch_capture": [VP, VP, SZ, VP, VP, VP, VP, ctypes.POINTER(VP)],
     "ntt_hip_graph_launch": [VP, VP],
     "ntt_hip_graph_destroy": [VP],
+    "ntt_hip_rotate_workspace_bytes": [VP, SZ, SZP],
+    "ntt_hip_keyswitch_rotate": [ctypes.POINTER(VP), SZ, ctypes.POINTER(ctypes.c_uint32), VP, VP, SZ,
+                                 ctypes.POINTER(VP), ctypes.POINTER(VP), VP, VP],
     "ntt_hip_modup_workspace_bytes": [VP, SZ, SZP],
     "ntt_hip_modup_run": [VP, VP, INT, SZ, VP, VP, VP],
     "ntt_hip_moddown_workspace_bytes": [VP, SZ, SZP],
@@ -350,6 +353,26 @@ class KeySwitch:
 
     def mod_down(self, in_ptr, batch, out_ptr, workspace_ptr, stream=None):
         check(lib().ntt_hip_moddown_run(self.handle, in_ptr, batch, out_ptr, workspace_ptr, stream), "moddown_run")
+
+    def rotate_workspace_words(self, batch):
+        out = SZ(0)
+        check(lib().ntt_hip_rotate_workspace_bytes(self.handle, batch, ctypes.byref(out)), "rotate_workspace")
+        return out.value // 8
+
+    @staticmethod
+    def rotate(keys, automorphisms, c0_ptr, c1_ptr, batch, out_b_ptrs, out_a_ptrs, workspace_ptr, stream=None):
+        """Hoisted rotations: digits of c1 once, then per rotation r the MAC with
+        keys[r], sigma_k(B + P c0), sigma_k(A) and ModDown into out_b_ptrs[r] /
+        out_a_ptrs[r]. keys[0] sizes the workspace (rotate_workspace_words)."""
+        R = len(keys)
+        if not (len(automorphisms) == len(out_b_ptrs) == len(out_a_ptrs) == R):
+            raise ContractError("one automorphism and one output pair per rotation key")
+        kh = (VP * R)(*[k.handle for k in keys])
+        au = (ctypes.c_uint32 * R)(*automorphisms)
+        ob = (VP * R)(*out_b_ptrs)
+        oa = (VP * R)(*out_a_ptrs)
+        check(lib().ntt_hip_keyswitch_rotate(kh, R, au, c0_ptr, c1_ptr, batch, ob, oa, workspace_ptr, stream),
+              "keyswitch_rotate")
 
     def capture(self, in_ptr, batch, out_b_ptr, out_a_ptr, workspace_ptr, stream):
         g = VP()

@@ -266,3 +266,92 @@ def test_batched_modup_moddown_config_c3(evaluation):
         assert np.array_equal(gu[c * (L + K) * n:(c + 1) * (L + K) * n], wu), f"ModUp component {c}"
         wd = o.mod_down(wu, n, hp.q, hp.q_psi, hp.p, hp.p_psi)
         assert np.array_equal(gd[c * L * n:(c + 1) * L * n], wd), f"ModDown component {c}"
+
+
+def _rotation_oracle(hp, L, c0, c1, kb, ka, k):
+    """Reference hybrid_rotate_many per rotation (openfhe_hybrid_adapter.hpp:343-355):
+    digits -> MAC -> sigma_k(B + P c0), sigma_k(A) -> ModDown of both channels."""
+    n, q, qr = hp.n, hp.q[:L], hp.q_psi[:L]
+    digits = o.hybrid_digits(c1, n, q, qr, hp.p, hp.p_psi, hp.alpha)
+    zb, za = o.hybrid_mac(digits, (L + hp.alpha - 1) // hp.alpha, kb, ka, n, q, hp.p, hp.M)
+    rb, ra = o.hybrid_rotate(zb, za, c0, n, q, hp.p, k)
+    return o.mod_down(rb, n, q, qr, hp.p, hp.p_psi), o.mod_down(ra, n, q, qr, hp.p, hp.p_psi)
+
+
+def _hoisted(hp, L, autos, batch, seed):
+    n = hp.n
+    ctx = hip.Context(n, hp.q[:L] + hp.p, hp.q_psi[:L] + hp.p_psi)
+    keys = []
+    for r in range(len(autos)):
+        ks = hip.KeySwitch(ctx, L, hp.K, hp.alpha, batch)
+        kb, ka = _key(hp, seed + 37 * r)
+        ks.upload_key(kb, ka, hp.M, hp.q)
+        keys.append((ks, kb, ka))
+    c0 = np.concatenate([uniform(n, hp.q[:L], seed + 500 + c) for c in range(batch)])
+    c1 = np.concatenate([uniform(n, hp.q[:L], seed + 700 + c) for c in range(batch)])
+    d0, d1 = _dev(c0), _dev(c1)
+    ws = hip.DeviceBuffer(keys[0][0].rotate_workspace_words(batch))
+    outs = [(hip.DeviceBuffer(batch * L * n), hip.DeviceBuffer(batch * L * n)) for _ in autos]
+    hip.KeySwitch.rotate([k[0] for k in keys], autos, d0.ptr, d1.ptr, batch, [b.ptr for b, _ in outs],
+                         [a.ptr for _, a in outs], ws.ptr)
+    ctx.synchronize()
+    return ctx, keys, c0, c1, [(b.download(), a.download()) for b, a in outs]
+
+
+@pytest.mark.parametrize("n,M,K,alpha,L,batch,autos", [
+    (1024, 6, 2, 2, 6, 2, [5, 25, 2047]),
+    (1024, 6, 2, 2, 4, 3, [3, 5]),
+    (65536, 24, 6, 6, 24, 1, [5, 131071]),
+])
+def test_hoisted_rotations(n, M, K, alpha, L, batch, autos):
+    """One ModUp of c1 shared by every rotation; each rotation bit-exact vs the oracle."""
+    hp = params.openfhe_like(n, M, K, alpha)
+    log_case(f"hoisted rotations n={n} L={L} batch={batch} k={autos}")
+    _, keys, c0, c1, got = _hoisted(hp, L, autos, batch, 4100 + L)
+    for r, k in enumerate(autos):
+        for c in range(batch):
+            sl = slice(c * L * n, (c + 1) * L * n)
+            wb, wa = _rotation_oracle(hp, L, c0[sl], c1[sl], keys[r][1], keys[r][2], k)
+            assert np.array_equal(got[r][0][sl], wb), f"rotation {k}, component {c}, B"
+            assert np.array_equal(got[r][1][sl], wa), f"rotation {k}, component {c}, A"
+
+
+def test_identity_rotation_is_key_switch_plus_c0():
+    """k = 1: ModDown(B + P c0) = ModDown(B) + c0 exactly, so the outputs are the
+    plain key switch of c1 with c0 added to B (independent of the oracle)."""
+    hp = params.openfhe_like(4096, 8, 3, 3)
+    L, n, batch = 8, 4096, 2
+    log_case("identity rotation vs key switch + c0")
+    ctx, keys, c0, c1, got = _hoisted(hp, L, [1], batch, 4300)
+    ks = keys[0][0]
+    d1 = _dev(c1)
+    ob, oa = hip.DeviceBuffer(batch * L * n), hip.DeviceBuffer(batch * L * n)
+    ws = hip.DeviceBuffer(ks.workspace_words(batch))
+    ks.run(d1.ptr, batch, ob.ptr, oa.ptr, ws.ptr)
+    ctx.synchronize()
+    kb_, ka_ = ob.download(), oa.download()
+    qs = np.repeat(np.array(hp.q[:L], dtype=object), n)
+    want_b = np.array([(int(b) + int(c)) % int(q) for b, c, q in
+                       zip(kb_, c0, np.tile(qs, batch))], dtype=np.uint64)
+    assert np.array_equal(got[0][0], want_b)
+    assert np.array_equal(got[0][1], ka_)
+
+
+def test_rotation_rejections():
+    hp = params.openfhe_like(1024, 6, 2, 2)
+    ctx = hip.Context(1024, hp.q + hp.p, hp.q_psi + hp.p_psi)
+    a = hip.KeySwitch(ctx, 6, 2, 2, 1)
+    other = hip.KeySwitch(ctx, 6, 2, 3, 1)  # same context, other digit size
+    buf = hip.DeviceBuffer(8 * 1024)
+    with pytest.raises(hip.ContractError):  # key not uploaded
+        hip.KeySwitch.rotate([a], [5], buf.ptr, buf.ptr, 1, [buf.ptr], [buf.ptr], buf.ptr)
+    kb, ka = _key(hp, 1)
+    a.upload_key(kb, ka, hp.M, hp.q)
+    hp3 = params.openfhe_like(1024, 6, 2, 3)
+    assert hp3.q == hp.q and hp3.p == hp.p
+    kb3, ka3 = _key(hp3, 2)
+    other.upload_key(kb3, ka3, hp3.M, hp3.q)
+    for keys, autos in (([a], [4]), ([a], [2048]), ([a, other], [5, 5])):  # even, out of range, plan mismatch
+        with pytest.raises(hip.ContractError):
+            hip.KeySwitch.rotate(keys, autos, buf.ptr, buf.ptr, 1, [buf.ptr] * len(keys), [buf.ptr] * len(keys),
+                                 buf.ptr)

@@ -30,6 +30,7 @@ BUTTERFLIES_PER_TOWER = (N // 2) * 16  # one direction of a 2^16-point transform
 # WRITE_SIZE (per dispatch, calibrated on a copy kernel of known size)
 HBM_COUNTERS = os.path.join(ROOT, "profiles", "ntt_n65536_hbm_counters.json")
 VALU_PROBE = os.path.join(ROOT, "ntt-gpu_amd", "bin", "valu_probe")
+SESSION_BENCH = os.path.join(ROOT, "ntt-gpu_amd", "bin", "session_keyswitch_bench")
 
 
 T0 = time.perf_counter()
@@ -182,6 +183,28 @@ def hoisted_rotations(ctx, hp, x, stream, R, B, steps=5):
             "note": "one ModUp of c1 shared by the R rotations; per rotation MAC, automorphism, ModDown"}
 
 
+def session_plain(hp, requests):
+    """The reference-API plain path: KeySwitchHybrid one request at a time
+    through the include/ntt Session (hybrid_digits, hybrid_accumulate, two
+    mod_down; resident input and key), run by session_keyswitch_bench in a
+    child process on the same primes."""
+    if not os.path.exists(SESSION_BENCH):
+        return None
+    text = " ".join(str(v) for v in [N, L, K, ALPHA] + hp.q + hp.q_psi + hp.p + hp.p_psi)
+    try:
+        r = subprocess.run([SESSION_BENCH, str(requests), "2"], input=text, capture_output=True, text=True,
+                           timeout=300)
+    except (OSError, subprocess.TimeoutExpired):
+        return None
+    if r.returncode != 0:
+        progress(f"session bench failed: {r.stderr.strip()[-200:]}")
+        return None
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    out["path"] = ("include/ntt Session chain per request (hybrid_digits -> hybrid_accumulate -> "
+                   "mod_down x2), driver ops launched one by one, fresh device buffers per output")
+    return out
+
+
 def served_stream(requests, rates, seed=0xC5):
     """Config C5: Poisson arrivals of mixed N/L/dnum/op requests at swept rates
     into the continuous-batching server; p50/p99 latency and throughput per
@@ -268,6 +291,7 @@ def main():
     ap.add_argument("--roofline-only", action="store_true")
     ap.add_argument("--serve-rates", type=str, default="200,1000")
     ap.add_argument("--rotations", type=int, default=8)
+    ap.add_argument("--session-requests", type=int, default=16)
     ap.add_argument("--rotation-batch", type=int, default=16)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -350,6 +374,9 @@ def main():
     plain_kss = args.plain_requests / plain_wall
 
     progress(f"plain done: {plain_kss:.1f} KS/s")
+    sess = session_plain(hp, args.session_requests)
+    if sess:
+        progress(f"session plain: {sess['keyswitch_per_s']} KS/s")
     rot = hoisted_rotations(ctx, hp, x, s, args.rotations, min(B, args.rotation_batch))
     progress(f"hoisted rotations: {rot['rotations_per_s']} rotations/s")
     roof = ntt_roofline(args.ntt_towers, 6, s)
@@ -387,6 +414,8 @@ def main():
         "plain_p50_ms": round(pct(lat, 0.5), 3),
         "plain_p99_ms": round(pct(lat, 0.99), 3),
         "batched_over_plain": round(batched_kss / plain_kss, 3),
+        "plain_session": sess,
+        "batched_over_plain_session": round(batched_kss / sess["keyswitch_per_s"], 2) if sess else None,
         "hoisted_rotations": rot,
         "roofline": roof,
         "valu_roofline": valu,

@@ -4,8 +4,10 @@
 // for the session basis) and its ordered stream per driver. Plain C++: no
 // HIP header is needed on this side of the boundary.
 #include <algorithm>
+#include <map>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "ntt/resident_driver.hpp"
 #include "ntt_hip.h"
@@ -43,7 +45,61 @@ struct Context {
     ok(ntt_hip_context_create(&ctx, device, base.parameters->n(), q.data(), psi.data(), q.size()), "context");
   }
   ~Context() {
+    flush_pool();
+    for (void *e : free_events)
+      if (ntt_hip_event_destroy(device, e) != NTT_HIP_OK) cleanup_failed = true;
     if (ctx && ntt_hip_context_destroy(ctx) != NTT_HIP_OK) cleanup_failed = true;
+  }
+
+  // Device blocks and events are recycled. The Session releases a buffer or
+  // an event only after the operation that used it is proven complete, so a
+  // released block can serve the next output of the same size without a
+  // hipMalloc/hipFree pair per operation (the reference allocates every
+  // output afresh). At most kPoolBytes stay cached; an allocation that fails
+  // flushes the cache and retries once.
+  static constexpr size_t kPoolBytes = size_t{8} << 30;
+  std::multimap<size_t, void *> free_blocks;  // bytes -> block
+  size_t pooled_bytes = 0;
+  std::vector<void *> free_events;
+
+  void *take_block(size_t bytes) {
+    const auto it = free_blocks.find(bytes);
+    if (it != free_blocks.end()) {
+      void *p = it->second;
+      free_blocks.erase(it);
+      pooled_bytes -= bytes;
+      return p;
+    }
+    void *p = nullptr;
+    if (ntt_hip_malloc(device, &p, bytes) != NTT_HIP_OK) {
+      flush_pool();
+      ok(ntt_hip_malloc(device, &p, bytes), "allocate");
+    }
+    return p;
+  }
+  void give_block(void *p, size_t bytes) {
+    if (pooled_bytes + bytes <= kPoolBytes) {
+      free_blocks.emplace(bytes, p);
+      pooled_bytes += bytes;
+    } else if (ntt_hip_free(device, p) != NTT_HIP_OK) {
+      cleanup_failed = true;
+    }
+  }
+  void flush_pool() {
+    for (auto &[bytes, p] : free_blocks)
+      if (ntt_hip_free(device, p) != NTT_HIP_OK) cleanup_failed = true;
+    free_blocks.clear();
+    pooled_bytes = 0;
+  }
+  void *take_event() {
+    if (!free_events.empty()) {
+      void *e = free_events.back();
+      free_events.pop_back();
+      return e;
+    }
+    void *e = nullptr;
+    ok(ntt_hip_event_create(device, &e, 0), "event");
+    return e;
   }
   void *stream() const { return ntt_hip_context_stream(ctx); }
   // Position of a contiguous layout inside the session basis.
@@ -79,13 +135,10 @@ struct Buffer final : transport::Buffer {
   size_t elements;
   Buffer(std::shared_ptr<Context> c, size_t n) : context(std::move(c)), elements(n) {
     if (context->cleanup_failed) throw std::runtime_error("resident HIP cleanup failed");
-    void *p = nullptr;
-    ok(ntt_hip_malloc(context->device, &p, std::max<size_t>(1, n) * sizeof(uint64_t)), "allocate");
-    data = static_cast<uint64_t *>(p);
+    data = static_cast<uint64_t *>(context->take_block(bytes()));
   }
-  ~Buffer() override {
-    if (ntt_hip_free(context->device, data) != NTT_HIP_OK) context->cleanup_failed = true;
-  }
+  ~Buffer() override { context->give_block(data, bytes()); }
+  size_t bytes() const { return std::max<size_t>(1, elements) * sizeof(uint64_t); }
 };
 
 struct Event final : transport::Event {
@@ -94,11 +147,9 @@ struct Event final : transport::Event {
   bool recorded = false;
   explicit Event(std::shared_ptr<Context> c) : context(std::move(c)) {
     if (context->cleanup_failed) throw std::runtime_error("resident HIP cleanup failed");
-    ok(ntt_hip_event_create(context->device, &event, 0), "event");
+    event = context->take_event();
   }
-  ~Event() override {
-    if (ntt_hip_event_destroy(context->device, event) != NTT_HIP_OK) context->cleanup_failed = true;
-  }
+  ~Event() override { context->free_events.push_back(event); }
 };
 
 class Driver final : public transport::Driver {

@@ -588,6 +588,9 @@ int ntt_hip_context_create(ntt_hip_context **out, int device, uint64_t n, const 
       const uint64_t qt = c->q[t];
       p.q = qt;
       p.two_q = 2 * qt;
+      p.nq = 0 - qt;
+      p.nq2 = 0 - 2 * qt;
+      p.nq4 = 0 - 4 * qt;
       p.qinv_neg = neg_inv64(qt);
       p.ninv = invmod(n % qt, qt);
       p.ninv_shoup = shoup_of(p.ninv, qt);

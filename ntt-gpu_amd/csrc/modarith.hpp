@@ -89,11 +89,17 @@ __host__ __device__ __forceinline__ U128 combine_limbs(uint64_t p00, uint64_t p0
 // Per butterfly that is 3 v_mad_u64_u32 + 1 add pair for the quotient,
 // 2 v_mad_u64_u32 + 4 v_mul_lo_u32 + 2 adds for x', and a few 64-bit adds:
 // about 20 VALU instructions against ~31 for the exact form.
+// The negated constants come precomputed (PrimeConst on the device): were
+// they derived in the kernel, the compiler would fold x + (2^64 - c) back into
+// a subtract with a carry chain, which is what the selects below avoid.
 struct Lazy8 {
-  uint64_t q, q4, nq;  // q, 4q, 2^64 - q
+  uint64_t q, q4, nq, nq2, nq4;  // q, 4q, 2^64 - q, 2^64 - 2q, 2^64 - 4q
 };
 
-__host__ __device__ __forceinline__ Lazy8 lazy8_of(uint64_t q) { return {q, q << 2, 0 - q}; }
+__host__ __device__ __forceinline__ Lazy8 lazy8_of(uint64_t q) { return {q, q << 2, 0 - q, 0 - 2 * q, 0 - 4 * q}; }
+__host__ __device__ __forceinline__ Lazy8 lazy8_of(uint64_t q, uint64_t nq, uint64_t nq2, uint64_t nq4) {
+  return {q, q << 2, nq, nq2, nq4};
+}
 
 // floor(y * p / 2^64) - e with e in {0, 1, 2}, any y, p < 2^64.
 __host__ __device__ __forceinline__ uint64_t mulhi_est(uint64_t y, uint64_t p) {
@@ -127,9 +133,32 @@ __host__ __device__ __forceinline__ uint64_t sub_if_ge(uint64_t x, uint64_t c) {
   return borrow ? x : d;
 }
 
+// mask ? a : b bitwise: one v_bfi_b32 on the device (written as asm so the
+// compiler cannot turn the select back into a compare and condition code).
+__host__ __device__ __forceinline__ uint32_t bit_select(uint32_t mask, uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t r;
+  asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(mask), "v"(a), "v"(b));
+  return r;
+#else
+  return (mask & a) | (~mask & b);
+#endif
+}
+
+// x >= c ? x - c : x for c <= 2^63 and x < 2c, with neg_c = 2^64 - c: t = x + neg_c
+// is x - c (top bit clear, x >= c) or wraps above 2^63 (x < c), so the sign of
+// t picks x or t. One 64-bit add, one shift, two bitfield selects; no carry
+// chain or condition code, hence no hazard wait states between them.
+__host__ __device__ __forceinline__ uint64_t sub_if_ge_neg(uint64_t x, uint64_t neg_c) {
+  const uint64_t t = x + neg_c;
+  const uint32_t mask = (uint32_t)((int32_t)(uint32_t)(t >> 32) >> 31);  // all ones iff x < c
+  return ((uint64_t)bit_select(mask, (uint32_t)(x >> 32), (uint32_t)(t >> 32)) << 32) |
+         bit_select(mask, (uint32_t)x, (uint32_t)t);
+}
+
 // Forward (Cooley-Tukey) butterfly: x, y in [0, 8q) -> x + yw, x - yw, both in [0, 8q).
 __host__ __device__ __forceinline__ void ct8(uint64_t &x, uint64_t &y, uint64_t w, uint64_t wp, const Lazy8 &m) {
-  const uint64_t xr = sub_if_ge(x, m.q4);  // [0, 4q)
+  const uint64_t xr = sub_if_ge_neg(x, m.nq4);  // [0, 4q)
   const uint64_t xn = mad_lo(xr, y, w, mulhi_est(y, wp), m.nq);  // xr + T, T in [0, 4q)
   y = 2 * xr + m.q4 - xn;  // xr - T + 4q in (0, 8q); the wrap of 2xr + 4q cancels mod 2^64
   x = xn;
@@ -139,13 +168,13 @@ __host__ __device__ __forceinline__ void ct8(uint64_t &x, uint64_t &y, uint64_t 
 __host__ __device__ __forceinline__ void gs8(uint64_t &x, uint64_t &y, uint64_t w, uint64_t wp, const Lazy8 &m) {
   const uint64_t s = x + y;  // [0, 8q)
   const uint64_t d = x + m.q4 - y;  // (0, 8q)
-  x = sub_if_ge(s, m.q4);
+  x = sub_if_ge_neg(s, m.nq4);
   y = shoup4(d, w, wp, m);
 }
 
 // [0, 8q) -> [0, q).
 __host__ __device__ __forceinline__ uint64_t canon8(uint64_t x, const Lazy8 &m) {
-  return sub_if_ge(sub_if_ge(sub_if_ge(x, m.q4), m.q << 1), m.q);
+  return sub_if_ge_neg(sub_if_ge_neg(sub_if_ge_neg(x, m.nq4), m.nq2), m.nq);
 }
 
 }  // namespace nttamd

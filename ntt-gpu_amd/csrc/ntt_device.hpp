@@ -13,6 +13,7 @@ struct PrimeConst {
   uint64_t ninv, ninv_shoup;         // n^-1 mod q and its Shoup companion
   uint64_t last_w, last_wp;          // psi^-bitrev(1) * n^-1 and Shoup (inverse last stage)
   uint64_t r2;                       // 2^128 mod q (to enter Montgomery form)
+  uint64_t nq, nq2, nq4;             // 2^64 - q, 2^64 - 2q, 2^64 - 4q (precomputed: see Lazy8)
   const ulonglong2 *fwd;
   const ulonglong2 *inv;
 };

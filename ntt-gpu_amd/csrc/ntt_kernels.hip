@@ -148,7 +148,7 @@ __global__ __launch_bounds__(256) void ntt_pass_a256(TowerMap map, const PrimeCo
   tower_of(map, blockIdx.y, !Inverse, data, src, prime, tk, tpoly);
   const PrimeConst pc = primes[prime];
   const TwPtr tw = tw_table(Inverse ? pc.inv : pc.fwd);
-  const Lazy8 m = lazy8_of(pc.q);
+  const Lazy8 m = lazy8_of(pc.q, pc.nq, pc.nq2, pc.nq4);
   const int t = threadIdx.x, cl = t & 15, u = t >> 4;
   const uint32_t c = blockIdx.x * 16 + cl;  // column
   uint64_t v[16];
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(256) void ntt_pass_b(TowerMap map, const PrimeConst
   tower_of(map, blockIdx.y, Inverse, data, src, prime, tk, tpoly);
   const PrimeConst pc = primes[prime];
   const TwPtr tw = tw_table(Inverse ? pc.inv : pc.fwd);
-  const Lazy8 m = lazy8_of(pc.q);
+  const Lazy8 m = lazy8_of(pc.q, pc.nq, pc.nq2, pc.nq4);
   const int t = threadIdx.x, u = t & 15, rl = t >> 4;
   const uint32_t r = blockIdx.x * 16 + rl;
   uint64_t *row = data + (uint64_t)r * kRow;
@@ -233,7 +233,7 @@ __global__ __launch_bounds__(256) void ntt_pass_b(TowerMap map, const PrimeConst
     const ulonglong2 c = map.epi.scale[tk];
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-      const uint64_t d = Lazy ? sub[u + 16 * k] + m.q4 - sub_if_ge(v[k], m.q4) : sub_mod(sub[u + 16 * k], v[k], pc.q);
+      const uint64_t d = Lazy ? sub[u + 16 * k] + m.q4 - sub_if_ge_neg(v[k], m.nq4) : sub_mod(sub[u + 16 * k], v[k], pc.q);
       out[u + 16 * k] = shoup(d, c.x, c.y, pc.q);
     }
     return;
@@ -288,7 +288,7 @@ __global__ __launch_bounds__(256) void ntt_lds_generic(TowerMap map, const Prime
   tower_of(map, blockIdx.y, first_pass != 0, data, src, prime, tk, tpoly);
   const PrimeConst pc = primes[prime];
   const TwPtr tw = tw_table(Inverse ? pc.inv : pc.fwd);
-  const Lazy8 m = lazy8_of(pc.q);
+  const Lazy8 m = lazy8_of(pc.q, pc.nq, pc.nq2, pc.nq4);
   const uint32_t L = 1u << log_len;
   const uint32_t total = L * cols_per_block;  // <= 4096
   const uint32_t col0 = blockIdx.x * cols_per_block;

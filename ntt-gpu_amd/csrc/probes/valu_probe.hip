@@ -63,8 +63,8 @@ __global__ void p_shoup_exact(uint64_t *out, uint64_t q, uint64_t w, uint64_t wp
 }
 
 // forward butterflies in chains: (x, y) -> ct8 -> (x, y), residues stay in [0, 8q)
-__global__ void p_ct8(uint64_t *out, uint64_t q, uint64_t w, uint64_t wp) {
-  const Lazy8 m = lazy8_of(q);
+__global__ void p_ct8(uint64_t *out, uint64_t q, uint64_t w, uint64_t wp, uint64_t nq, uint64_t nq2, uint64_t nq4) {
+  const Lazy8 m = lazy8_of(q, nq, nq2, nq4);
   uint64_t x[kChains], y[kChains];
   for (int i = 0; i < kChains; ++i) x[i] = (threadIdx.x * 131 + i) % q, y[i] = (threadIdx.x * 17 + 5 * i) % q;
   for (int it = 0; it < kIter / 4; ++it)
@@ -75,8 +75,8 @@ __global__ void p_ct8(uint64_t *out, uint64_t q, uint64_t w, uint64_t wp) {
   out[blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
-__global__ void p_gs8(uint64_t *out, uint64_t q, uint64_t w, uint64_t wp) {
-  const Lazy8 m = lazy8_of(q);
+__global__ void p_gs8(uint64_t *out, uint64_t q, uint64_t w, uint64_t wp, uint64_t nq, uint64_t nq2, uint64_t nq4) {
+  const Lazy8 m = lazy8_of(q, nq, nq2, nq4);
   uint64_t x[kChains], y[kChains];
   for (int i = 0; i < kChains; ++i) x[i] = (threadIdx.x * 131 + i) % q, y[i] = (threadIdx.x * 17 + 5 * i) % q;
   for (int it = 0; it < kIter / 4; ++it)
@@ -126,9 +126,9 @@ int main(int argc, char **argv) {
        lanes * kIter * kChains},
       {"shoup_mulmod_exact", "mulmod", time_ms([&] { p_shoup_exact<<<blocks, threads>>>(buf, q, w, wp); }),
        lanes * kIter / 4 * kChains},
-      {"ct8_forward_butterfly", "butterfly", time_ms([&] { p_ct8<<<blocks, threads>>>(buf, q, w, wp); }),
+      {"ct8_forward_butterfly", "butterfly", time_ms([&] { p_ct8<<<blocks, threads>>>(buf, q, w, wp, 0 - q, 0 - 2 * q, 0 - 4 * q); }),
        lanes * kIter / 4 * kChains},
-      {"gs8_inverse_butterfly", "butterfly", time_ms([&] { p_gs8<<<blocks, threads>>>(buf, q, w, wp); }),
+      {"gs8_inverse_butterfly", "butterfly", time_ms([&] { p_gs8<<<blocks, threads>>>(buf, q, w, wp, 0 - q, 0 - 2 * q, 0 - 4 * q); }),
        lanes * kIter / 4 * kChains},
   };
   PROBE_CHECK(hipGetLastError());

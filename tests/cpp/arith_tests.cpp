@@ -77,6 +77,16 @@ int main() {
     const uint64_t hi = (uint64_t)(((u128)y * p) >> 64), est = mulhi_est(y, p);
     check(est <= hi && hi - est <= 2, "mulhi_est on arbitrary operands");
   }
+  // sign-selected conditional subtract: c <= 2^63, x < 2c
+  for (int i = 0; i < 2000000; ++i) {
+    const uint64_t c = (i % 7 == 0) ? (1ULL << 63) - (rng() & 0xff) : (rng() >> (1 + i % 40)) | 1;
+    const uint64_t x = (i % 5 == 0) ? (c - 1 + (i % 3)) : (c >> 63) ? rng() : rng() % (2 * c);  // x < 2c
+    check(sub_if_ge_neg(x, 0 - c) == (x >= c ? x - c : x), "sub_if_ge_neg");
+  }
+  const uint64_t edges[] = {1, 2, uint64_t{1} << 63, (uint64_t{1} << 62) + 5, 2305843009213693951ULL};
+  for (const uint64_t c : edges)
+    for (const uint64_t x : {uint64_t{0}, c - 1, c, 2 * c - 1})
+      check(sub_if_ge_neg(x, 0 - c) == (x >= c ? x - c : x), "sub_if_ge_neg at the edges");
   // limb-split dot product of the register conversion kernel: up to 16 terms,
   // y below its source prime, constants below 2^63, total below 2^127
   for (int i = 0; i < 200000; ++i) {

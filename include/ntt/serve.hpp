@@ -34,7 +34,7 @@ struct Options {
 
 struct Result {
   std::vector<uint64_t> out, out2;  // KeySwitch: B then A contributions; otherwise out only
-  double latency_ms = 0;            // submit -> result visible on the host
+  double latency_ms = 0;            // submit -> result visible on the host (device requests: in HBM)
   size_t batch_size = 0;            // requests in the launch that served it
 };
 
@@ -54,6 +54,13 @@ class Server {
   // Input words: KeySwitch/Forward/Inverse/ModUp: level*n (ModUp/KeySwitch:
   // evaluation Q); ModDown: (level + K)*n evaluation QP. Copied at submit.
   uint64_t submit(uint32_t params, Op op, size_t level, std::span<const uint64_t> input);
+  // Device-resident request: `in`, `out` (and `out2`, the A contribution of a
+  // key switch) are device buffers owned by the caller, so ciphertexts stay in
+  // HBM across requests and nothing crosses the host; a batch of one runs on
+  // them directly, larger batches gather into the slot's arena and scatter
+  // back device to device. Inputs must hold canonical residues (the host does
+  // not read them). wait() returns an empty Result::out.
+  uint64_t submit_device(uint32_t params, Op op, size_t level, const uint64_t *in, uint64_t *out, uint64_t *out2);
   bool ready(uint64_t ticket) const;
   Result wait(uint64_t ticket);  // blocks; each ticket can be collected once
   Stats stats() const;

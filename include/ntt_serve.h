@@ -38,8 +38,15 @@ int ntt_serve_output_words(ntt_serve *server, uint32_t params_id, int op, size_t
 /* Copies `words` input residues (level*n; ModDown: (level+K)*n). */
 int ntt_serve_submit(ntt_serve *server, uint32_t params_id, int op, size_t level, const uint64_t *in, size_t words,
                      uint64_t *ticket);
+/* Device-resident request: in/out/out2 are the caller's device buffers
+ * (out2: the A contribution of a key switch, NULL otherwise); ciphertexts stay
+ * in HBM, nothing crosses the host; inputs must be canonical. Collect with
+ * ntt_serve_wait(..., out = NULL, ...). */
+int ntt_serve_submit_device(ntt_serve *server, uint32_t params_id, int op, size_t level, const uint64_t *in,
+                            uint64_t *out, uint64_t *out2, uint64_t *ticket);
 int ntt_serve_ready(ntt_serve *server, uint64_t ticket); /* NTT_HIP_OK ready, NTT_HIP_NOT_READY pending */
-/* Blocks until done; out2 receives the A contribution of a key switch. */
+/* Blocks until done; out2 receives the A contribution of a key switch; out =
+ * NULL for a device-resident request. */
 int ntt_serve_wait(ntt_serve *server, uint64_t ticket, uint64_t *out, size_t out_words, uint64_t *out2,
                    size_t out2_words, double *latency_ms, size_t *batch_size);
 /* submitted, completed, batches, max concurrent batches, device bytes held, peak */

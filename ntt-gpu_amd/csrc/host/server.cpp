@@ -44,6 +44,9 @@ struct Request {
   Op op = Op::Forward;
   size_t level = 0;
   std::vector<uint64_t> in;
+  bool device = false;  // device-resident: the caller's buffers below
+  const uint64_t *d_in = nullptr;
+  uint64_t *d_out = nullptr, *d_out2 = nullptr;
   Clock::time_point t0;
   Result result;
   std::string error;
@@ -155,7 +158,7 @@ struct Server::Impl {
       const auto head = pending.front();
       for (auto it = pending.begin(); it != pending.end() && batch.size() < opt.max_batch;) {
         const auto &r = *it;
-        if (r->params == head->params && r->op == head->op && r->level == head->level) {
+        if (r->params == head->params && r->op == head->op && r->level == head->level && r->device == head->device) {
           batch.push_back(r);
           it = pending.erase(it);
         } else {
@@ -204,12 +207,21 @@ struct Server::Impl {
       const size_t iw = in_words(p, h.op, h.level), ow = out_words(p, h.op, h.level);
       const size_t ow2 = h.op == Op::KeySwitch ? ow : 0;
       const size_t dev = footprint(p, *lp, h.op, h.level, b);
-      ensure_arena(slot, dev, b * (iw + ow + ow2) * 8);
+      ensure_arena(slot, dev, h.device ? 0 : b * (iw + ow + ow2) * 8);
       uint64_t *d_in = static_cast<uint64_t *>(slot.device), *d_out = d_in + b * iw, *d_out2 = d_out + b * ow;
       void *d_ws = d_out2 + b * ow2;
       uint64_t *h_in = static_cast<uint64_t *>(slot.pinned);
-      for (size_t i = 0; i < b; ++i) std::memcpy(h_in + i * iw, batch[i]->in.data(), iw * 8);
-      ok(ntt_hip_memcpy(opt.device, d_in, h_in, b * iw * 8, NTT_HIP_H2D, slot.stream), "h2d");
+      if (!h.device) {
+        for (size_t i = 0; i < b; ++i) std::memcpy(h_in + i * iw, batch[i]->in.data(), iw * 8);
+        ok(ntt_hip_memcpy(opt.device, d_in, h_in, b * iw * 8, NTT_HIP_H2D, slot.stream), "h2d");
+      } else if (b == 1) {  // run on the caller's buffers
+        d_in = const_cast<uint64_t *>(h.d_in);
+        d_out = h.d_out;
+        d_out2 = h.d_out2;
+      } else {
+        for (size_t i = 0; i < b; ++i)
+          ok(ntt_hip_memcpy(opt.device, d_in + i * iw, batch[i]->d_in, iw * 8, NTT_HIP_D2D, slot.stream), "gather");
+      }
       const size_t L = h.level, n = p.n;
       switch (h.op) {
         case Op::KeySwitch:
@@ -227,8 +239,17 @@ struct Server::Impl {
           ok(ntt_hip_moddown_run(lp->ks, d_in, b, d_out, d_ws, slot.stream), "moddown");
           break;
       }
-      uint64_t *h_out = h_in + b * iw;
-      ok(ntt_hip_memcpy(opt.device, h_out, d_out, b * (ow + ow2) * 8, NTT_HIP_D2H, slot.stream), "d2h");
+      if (!h.device) {
+        uint64_t *h_out = h_in + b * iw;
+        ok(ntt_hip_memcpy(opt.device, h_out, d_out, b * (ow + ow2) * 8, NTT_HIP_D2H, slot.stream), "d2h");
+      } else if (b > 1) {
+        for (size_t i = 0; i < b; ++i) {
+          ok(ntt_hip_memcpy(opt.device, batch[i]->d_out, d_out + i * ow, ow * 8, NTT_HIP_D2D, slot.stream), "scatter");
+          if (ow2)
+            ok(ntt_hip_memcpy(opt.device, batch[i]->d_out2, d_out2 + i * ow2, ow2 * 8, NTT_HIP_D2D, slot.stream),
+               "scatter");
+        }
+      }
       ok(ntt_hip_event_record(opt.device, slot.event, slot.stream), "record");
       slot.batch = std::move(batch);
       slot.in_words = iw;
@@ -261,7 +282,7 @@ struct Server::Impl {
       Request &r = *slot.batch[i];
       if (q != NTT_HIP_OK) {
         r.error = std::string("HIP: batch failed: ") + ntt_hip_last_error();
-      } else {
+      } else if (!r.device) {
         r.result.out.assign(h_out + i * slot.out_words, h_out + (i + 1) * slot.out_words);
         if (slot.out2_words) {
           const uint64_t *o2 = h_out + b * slot.out_words;
@@ -387,6 +408,33 @@ uint64_t Server::submit(uint32_t id, Op op, size_t level, std::span<const uint64
   return r->ticket;
 }
 
+uint64_t Server::submit_device(uint32_t id, Op op, size_t level, const uint64_t *in, uint64_t *out, uint64_t *out2) {
+  auto r = std::make_shared<Request>();
+  {
+    std::lock_guard<std::mutex> lock(impl_->mu);
+    if (impl_->stop) throw std::invalid_argument("server stopping");
+    if (id >= impl_->params.size()) throw std::invalid_argument("unknown parameter set");
+    const ParamSet &p = *impl_->params[id];
+    if (static_cast<int>(op) < 0 || static_cast<int>(op) > 4) throw std::invalid_argument("unknown operation");
+    if (level < 1 || level > p.q.size()) throw std::invalid_argument("level outside the parameter set");
+    if (!in || !out || (op == Op::KeySwitch && !out2)) throw std::invalid_argument("null device buffer");
+    r->ticket = impl_->next_ticket++;
+    r->params = id;
+    r->op = op;
+    r->level = level;
+    r->device = true;
+    r->d_in = in;
+    r->d_out = out;
+    r->d_out2 = op == Op::KeySwitch ? out2 : nullptr;
+    r->t0 = Clock::now();
+    impl_->pending.push_back(r);
+    impl_->tickets.emplace(r->ticket, r);
+    ++impl_->stats.submitted;
+  }
+  impl_->cv_submit.notify_one();
+  return r->ticket;
+}
+
 bool Server::ready(uint64_t ticket) const {
   std::lock_guard<std::mutex> lock(impl_->mu);
   auto it = impl_->tickets.find(ticket);
@@ -490,6 +538,15 @@ int ntt_serve_submit(ntt_serve *s, uint32_t params, int op, size_t level, const 
   });
 }
 
+int ntt_serve_submit_device(ntt_serve *s, uint32_t params, int op, size_t level, const uint64_t *in, uint64_t *out,
+                            uint64_t *out2, uint64_t *ticket) {
+  return serve_guard([&] {
+    if (!s || !ticket) throw std::invalid_argument("null argument");
+    *ticket = s->server.submit_device(params, static_cast<ntt::serve::Op>(op), level, in, out, out2);
+    return NTT_HIP_OK;
+  });
+}
+
 int ntt_serve_ready(ntt_serve *s, uint64_t ticket) {
   return serve_guard([&] {
     if (!s) throw std::invalid_argument("null server");
@@ -500,11 +557,13 @@ int ntt_serve_ready(ntt_serve *s, uint64_t ticket) {
 int ntt_serve_wait(ntt_serve *s, uint64_t ticket, uint64_t *out, size_t out_words, uint64_t *out2, size_t out2_words,
                    double *latency_ms, size_t *batch_size) {
   return serve_guard([&] {
-    if (!s || !out) throw std::invalid_argument("null argument");
+    if (!s) throw std::invalid_argument("null argument");
     auto r = s->server.wait(ticket);
-    if (r.out.size() != out_words || r.out2.size() != (out2 ? out2_words : r.out2.size()))
-      throw std::invalid_argument("output buffer length mismatch");
-    std::copy(r.out.begin(), r.out.end(), out);
+    if (out) {  // host requests; a device request's results are already in its buffers
+      if (r.out.size() != out_words || r.out2.size() != (out2 ? out2_words : r.out2.size()))
+        throw std::invalid_argument("output buffer length mismatch");
+      std::copy(r.out.begin(), r.out.end(), out);
+    }
     if (out2) std::copy(r.out2.begin(), r.out2.end(), out2);
     if (latency_ms) *latency_ms = r.latency_ms;
     if (batch_size) *batch_size = r.batch_size;

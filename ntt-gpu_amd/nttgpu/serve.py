@@ -20,6 +20,7 @@ SIGNATURES = {
                              ctypes.POINTER(ctypes.c_uint32)],
     "ntt_serve_output_words": [VP, ctypes.c_uint32, INT, SZ, ctypes.POINTER(SZ)],
     "ntt_serve_submit": [VP, ctypes.c_uint32, INT, SZ, U64P, SZ, ctypes.POINTER(ctypes.c_uint64)],
+    "ntt_serve_submit_device": [VP, ctypes.c_uint32, INT, SZ, VP, VP, VP, ctypes.POINTER(ctypes.c_uint64)],
     "ntt_serve_ready": [VP, ctypes.c_uint64],
     "ntt_serve_wait": [VP, ctypes.c_uint64, U64P, SZ, U64P, SZ, ctypes.POINTER(ctypes.c_double),
                        ctypes.POINTER(SZ)],
@@ -87,6 +88,21 @@ class Server:
         t = ctypes.c_uint64(0)
         _check(lib().ntt_serve_submit(self.handle, pid, op, level, ap, a.size, ctypes.byref(t)), "serve_submit")
         return t.value
+
+    def submit_device(self, pid, op, level, in_ptr, out_ptr, out2_ptr=None):
+        """Device-resident request: the caller's device buffers (ciphertexts
+        stay in HBM); collect with wait_device."""
+        t = ctypes.c_uint64(0)
+        _check(lib().ntt_serve_submit_device(self.handle, pid, op, level, in_ptr, out_ptr, out2_ptr,
+                                             ctypes.byref(t)), "serve_submit_device")
+        return t.value
+
+    def wait_device(self, ticket):
+        lat = ctypes.c_double(0)
+        bs = SZ(0)
+        _check(lib().ntt_serve_wait(self.handle, ticket, None, 0, None, 0, ctypes.byref(lat), ctypes.byref(bs)),
+               "serve_wait")
+        return lat.value, bs.value
 
     def ready(self, ticket):
         return _check(lib().ntt_serve_ready(self.handle, ticket), "serve_ready") == hip.OK

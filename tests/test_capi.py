@@ -40,6 +40,21 @@ def test_binding_covers_the_header():
     assert lib.ntt_hip_abi_version() == 1
 
 
+SERVE_HEADER = os.path.join(ROOT, "include", "ntt_serve.h")
+
+
+def test_serving_library_exports_and_binding_cover_its_header():
+    from nttgpu import serve
+    text = re.sub(r"/\*.*?\*/", "", open(SERVE_HEADER).read(), flags=re.S)
+    want = sorted(set(re.findall(r"\b(ntt_serve_\w+)\s*\(", text)))
+    out = subprocess.run(["nm", "-D", "--defined-only", serve.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r"\bT (ntt_serve_\w+)", out))
+    assert [s for s in want if s not in exported] == []
+    assert sorted(exported - set(want)) == []
+    assert sorted(serve.SIGNATURES) == want
+
+
 def _create(n, q, psi, device=0):
     h = ctypes.c_void_p()
     qa = (ctypes.c_uint64 * len(q))(*q)

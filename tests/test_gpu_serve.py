@@ -7,7 +7,7 @@ import pytest
 
 import oracle as o
 from common import log_case
-from nttgpu import serve, workload
+from nttgpu import hip, serve, workload
 
 pytestmark = pytest.mark.gpu
 
@@ -76,3 +76,47 @@ def test_admission_by_footprint_queues_instead_of_failing():
         wb, wa = o.key_switch(r.data, hp.n, hp.q, hp.q_psi, hp.M, hp.p, hp.p_psi, hp.alpha, kb, ka)
         assert np.array_equal(b, wb) and np.array_equal(a, wa)
     srv.close()
+
+
+def test_device_resident_requests_equal_host_requests():
+    """Ciphertexts resident in HBM (submit_device): a mixed stream plus a burst of
+    one group (batches > 1 gather and scatter device to device) give the same
+    residues as the host path and the oracle."""
+    reqs, pools = workload.stream(24, rate_per_s=4000.0, seed=15, sizes=(8192,), levels=(3, 8))
+    burst, _ = workload.stream(6, rate_per_s=4000.0, seed=16, sizes=(8192,), levels=(5, 5), dnums=(2,))
+    for r in burst:
+        r.op = serve.KEYSWITCH
+        r.data = r.data[: r.level * r.n]
+    for r in burst:
+        pools.setdefault(r.n, workload.PrimePool(r.n))
+    srv = serve.Server(streams=4, max_batch=8, hbm_budget_bytes=8 << 30)
+    ms = workload.MixedServer(srv, pools)
+    everything = reqs + burst
+    for r in everything:
+        ms.params_for(r)
+    log_case(f"device-resident serve: {len(everything)} requests")
+    dev = []
+    for r in everything:
+        pid = ms.params_for(r)
+        w = srv.output_words(pid, r.op, r.level)
+        d_in = hip.DeviceBuffer(r.data.size).upload(r.data)
+        d_out = hip.DeviceBuffer(w)
+        d_out2 = hip.DeviceBuffer(w) if r.op == serve.KEYSWITCH else None
+        dev.append((d_in, d_out, d_out2))
+    tickets = [srv.submit_device(ms.params_for(r), r.op, r.level, d[0].ptr, d[1].ptr, d[2].ptr if d[2] else None)
+               for r, d in zip(everything, dev)]
+    sizes = [srv.wait_device(t)[1] for t in tickets]
+    assert max(sizes) > 1, "the burst must share launches"
+    host = serve.Server(streams=1, max_batch=1, hbm_budget_bytes=8 << 30)
+    hs = workload.MixedServer(host, pools)
+    for i, (r, d) in enumerate(zip(everything, dev)):
+        (h1, h2), _, _ = hs.wait(hs.submit(r), r)
+        assert np.array_equal(d[1].download(), h1), f"request {i} ({serve.OP_NAMES[r.op]}) differs from host path"
+        if r.op == serve.KEYSWITCH:
+            assert np.array_equal(d[2].download(), h2)
+    hp, kb, ka = ms.sets[(burst[0].n, burst[0].level, burst[0].dnum)]
+    wb, wa = o.key_switch(burst[0].data, hp.n, hp.q[:5], hp.q_psi[:5], hp.M, hp.p, hp.p_psi, hp.alpha, kb, ka)
+    i0 = len(reqs)
+    assert np.array_equal(dev[i0][1].download(), wb) and np.array_equal(dev[i0][2].download(), wa)
+    srv.close()
+    host.close()

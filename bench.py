@@ -205,10 +205,13 @@ def session_plain(hp, requests):
     return out
 
 
-def served_stream(requests, rates, seed=0xC5):
+def served_stream(requests, rates, device_rates, seed=0xC5):
     """Config C5: Poisson arrivals of mixed N/L/dnum/op requests at swept rates
     into the continuous-batching server; p50/p99 latency and throughput per
-    rate, and every response compared with the same request served alone."""
+    rate, and every response compared with the same request served alone.
+    Two modes: "host" (requests carry host arrays: upload, compute, download
+    per request) at `rates`, and "device" (inputs and outputs resident in HBM,
+    submit_device) at `device_rates`."""
     from nttgpu import serve, workload
     results = []
     reqs, pools = workload.stream(requests, 1.0, seed)
@@ -229,25 +232,61 @@ def served_stream(requests, rates, seed=0xC5):
     progress(f"serve: {len(seen)} warm plans")
     reference = [alone.wait(alone.submit(r), r)[0] for r in reqs]
     progress("serve: alone results ready")
-    for rate in rates:
+
+    def run(rate, mode, submit, collect):
+        """Poisson arrivals at `rate`; collect(i, ticket) -> (outputs or None, latency ms, batch)."""
         gaps = np.random.default_rng(seed + int(rate)).exponential(1.0 / rate, len(reqs))
         tickets, t0 = [], time.perf_counter()
         due = t0
-        for r, g in zip(reqs, gaps):
+        for i, g in enumerate(gaps):
             due += g
             while time.perf_counter() < due:
                 pass
-            tickets.append(ms.submit(r))
-        outs = [ms.wait(t, r) for t, r in zip(tickets, reqs)]
+            tickets.append(submit(i))
+        waited = [collect(i, t) for i, t in enumerate(tickets)]
         wall = time.perf_counter() - t0
-        lat = [o[1] for o in outs]
-        exact = all(np.array_equal(o[0][0], ref[0]) and (ref[1] is None or np.array_equal(o[0][1], ref[1]))
-                    for o, ref in zip(outs, reference))
-        results.append({"offered_rate_per_s": rate, "requests": len(reqs), "throughput_per_s": round(len(reqs) / wall, 1),
+        lat = [w[1] for w in waited]
+        exact = None
+        if mode == "host":
+            exact = all(np.array_equal(w[0][0], ref[0]) and (ref[1] is None or np.array_equal(w[0][1], ref[1]))
+                        for w, ref in zip(waited, reference))
+        results.append({"mode": mode, "offered_rate_per_s": rate, "requests": len(reqs),
+                        "throughput_per_s": round(len(reqs) / wall, 1),
                         "p50_ms": round(pct(lat, 0.5), 3), "p99_ms": round(pct(lat, 0.99), 3),
-                        "mean_batch": round(statistics.mean(o[2] for o in outs), 2),
-                        "all_equal_alone": exact})
-        progress(f"serve rate {rate}/s: p50 {results[-1]['p50_ms']} ms p99 {results[-1]['p99_ms']} ms exact={exact}")
+                        "mean_batch": round(statistics.mean(w[2] for w in waited), 2), "all_equal_alone": exact})
+
+    def report():
+        r = results[-1]
+        progress(f"serve {r['mode']} rate {r['offered_rate_per_s']}/s: p50 {r['p50_ms']} ms p99 {r['p99_ms']} ms "
+                 f"exact={r['all_equal_alone']}")
+
+    for rate in rates:
+        run(rate, "host", lambda i: ms.submit(reqs[i]), lambda i, t: ms.wait(t, reqs[i]))
+        report()
+    # device-resident: every request's input and outputs live in HBM before the
+    # stream starts (uploaded outside the timed region); outputs are read back
+    # after the timed region for the comparison with the alone results
+    dev = []
+    for r in reqs:
+        w = srv.output_words(ms.params_for(r), r.op, r.level)
+        dev.append((hip.DeviceBuffer(r.data.size).upload(r.data), hip.DeviceBuffer(w),
+                    hip.DeviceBuffer(w) if r.op == serve.KEYSWITCH else None))
+
+    def submit_dev(i):
+        d = dev[i]
+        return srv.submit_device(ms.params_for(reqs[i]), reqs[i].op, reqs[i].level, d[0].ptr, d[1].ptr,
+                                 d[2].ptr if d[2] else None)
+
+    def collect_dev(i, t):
+        lat_ms, b = srv.wait_device(t)
+        return None, lat_ms, b
+
+    for rate in device_rates:
+        run(rate, "device", submit_dev, collect_dev)
+        results[-1]["all_equal_alone"] = all(
+            np.array_equal(d[1].download(), ref[0]) and (d[2] is None or np.array_equal(d[2].download(), ref[1]))
+            for d, ref in zip(dev, reference))
+        report()
     stats = srv.stats()
     srv.close()
     alone_srv.close()
@@ -287,9 +326,10 @@ def main():
     ap.add_argument("--ntt-towers", type=int, default=2048)
     ap.add_argument("--cpu-samples", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=0)
+    ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--roofline-only", action="store_true")
     ap.add_argument("--serve-rates", type=str, default="200,1000")
+    ap.add_argument("--serve-device-rates", type=str, default="1000,5000,20000")
     ap.add_argument("--rotations", type=int, default=8)
     ap.add_argument("--session-requests", type=int, default=16)
     ap.add_argument("--rotation-batch", type=int, default=16)
@@ -386,7 +426,8 @@ def main():
         progress(f"valu roofline: {valu['achieved']} of {valu['peak']} T butterfly/s")
     served = None
     if args.serve_requests:
-        served = served_stream(args.serve_requests, [float(r) for r in args.serve_rates.split(",")])
+        served = served_stream(args.serve_requests, [float(r) for r in args.serve_rates.split(",")],
+                               [float(r) for r in args.serve_device_rates.split(",")])
     cpu, exact = (None, None)
     if not args.no_cpu:
         cpu, exact = cpu_baseline(hp, x[:comp], kb, ka, gpu_b, gpu_a, args.cpu_samples)

@@ -164,6 +164,27 @@ int ntt_hip_moddown_workspace_bytes(ntt_hip_keyswitch *ks, size_t batch, size_t 
 int ntt_hip_moddown_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batch, uint64_t *out, void *workspace,
                         void *stream);
 int ntt_hip_graph_destroy(void *graph);
+/* In-place key switch (reference openfhe_hybrid_adapter.hpp:221-248
+ * hybrid_key_switch_in_place; docs/openfhe-hybrid-operation-graph.md:350-362):
+ * c2 == NULL: two-component ciphertexts (c0, c1): c1 is key-switched,
+ *   c0 += B and c1 = A;
+ * c2 != NULL: three-component (c0, c1, c2): c2 is key-switched, c0 += B and
+ *   c1 += A (c2 is left as it was; the caller drops it).
+ * All components are batch * L * n evaluation residues on the device (the
+ * caller converts coefficient-form components with the forward NTT first, as
+ * the adapter's SetFormat(EVALUATION) does). The additions are fused into the
+ * ModDown's last pass. Workspace: ntt_hip_keyswitch_workspace_bytes. */
+int ntt_hip_keyswitch_in_place(ntt_hip_keyswitch *ks, uint64_t *c0, uint64_t *c1, const uint64_t *c2, size_t batch,
+                               void *workspace, void *stream);
+/* Relinearization (reference openfhe_hybrid_adapter.hpp:250-286
+ * hybrid_relinearize_in_place): component 2 + j (higher[j], j < extra) is
+ * key-switched with keys[j] (the j-th relinearization key, OpenFHE's
+ * EvalMultKeyVector order) and each contribution, reduced by its own ModDown,
+ * is added into c0 (B) and c1 (A). extra == 0 leaves (c0, c1) unchanged. Keys
+ * share one context and (L, K, alpha); keys[0] sizes the workspace
+ * (ntt_hip_keyswitch_workspace_bytes). Evaluation form, as above. */
+int ntt_hip_relinearize(ntt_hip_keyswitch *const *keys, size_t extra, uint64_t *c0, uint64_t *c1,
+                        const uint64_t *const *higher, size_t batch, void *workspace, void *stream);
 /* Hoisted rotations (reference openfhe_hybrid_adapter.hpp:307-360
  * hybrid_rotate_many over Session::hybrid_digits/hybrid_accumulate/
  * hybrid_rotate_extended/mod_down, src/resident.cpp:458-617): the digits of

@@ -937,8 +937,10 @@ void ks_mac(const ntt_hip_keyswitch &key, const ntt_hip_keyswitch &plan, const u
 // ModDown of both channels of z ([2][B] * z_words, evaluation QP): INTT(P) ->
 // CRT to Q -> NTT with (Z_Q - NTT(corr)) P^-1 fused into its last pass;
 // channel b lands in out_b, channel a in out_a (B * L * n each).
+// accumulate: bit 0 adds the B channel into out_b, bit 1 the A channel into
+// out_a (mod q) instead of writing them.
 void ks_moddown(const ntt_hip_keyswitch &ks, const uint64_t *z, size_t B, uint64_t *out_b, uint64_t *out_a,
-                const KsBuffers &w, hipStream_t s) {
+                const KsBuffers &w, hipStream_t s, uint32_t accumulate = 0) {
   const ntt_hip_context &c = *ks.ctx;
   const size_t L = ks.L, K = ks.K, n = c.n;
   const PrimeConst *pr = c.dprimes();
@@ -958,9 +960,10 @@ void ks_moddown(const ntt_hip_keyswitch &ks, const uint64_t *z, size_t B, uint64
   a.skip_at = 0xffffffffu;
   a.skip_len = 0;
   hip_ok(launch_basis_switch(a, ks.down->per_term, (uint32_t)(2 * B), s), "moddown conversion");
-  hip_ok(launch_ntt(with_correction(affine(w.corr, nullptr, n, L, 0, ks.corr_words(), 0), z, ks.z_words(), out_b,
-                                    out_a, (uint32_t)B, L * n, ks.pinv),
-                    pr, c.log_n, (uint32_t)(2 * B * L), false, c.lazy, s, c.ntt_chunk),
+  TowerMap fin = with_correction(affine(w.corr, nullptr, n, L, 0, ks.corr_words(), 0), z, ks.z_words(), out_b, out_a,
+                                 (uint32_t)B, L * n, ks.pinv);
+  fin.epi.accumulate = accumulate;
+  hip_ok(launch_ntt(fin, pr, c.log_n, (uint32_t)(2 * B * L), false, c.lazy, s, c.ntt_chunk),
          "moddown ntt + correction");
 }
 }  // namespace
@@ -988,6 +991,51 @@ int ntt_hip_keyswitch_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batc
     ks_modup(*ks, in, batch, w, s);
     ks_mac(*ks, *ks, in, batch, w, w.z, s);
     ks_moddown(*ks, w.z, batch, out_b, out_a, w, s);
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_keyswitch_in_place(ntt_hip_keyswitch *ks, uint64_t *c0, uint64_t *c1, const uint64_t *c2, size_t batch,
+                               void *workspace, void *stream) {
+  return guarded([&] {
+    require(ks && c0 && c1 && workspace, "null argument");
+    require(ks->key_loaded, "evaluation key not uploaded");
+    require(batch >= 1 && batch <= ks->max_batch, "batch exceeds the plan capacity");
+    const hipStream_t s = ks->ctx->pick(stream);
+    const KsBuffers w = ks_buffers(*ks, workspace, batch);
+    const uint64_t *in = c2 ? c2 : c1;
+    DeviceGuard g(ks->ctx->device);
+    ks_modup(*ks, in, batch, w, s);
+    ks_mac(*ks, *ks, in, batch, w, w.z, s);
+    // c1 is read only by the ModUp and the MAC above, so the two-component
+    // case may overwrite it in the ModDown's last pass (stream order)
+    ks_moddown(*ks, w.z, batch, c0, c1, w, s, c2 ? 3u : 1u);
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_relinearize(ntt_hip_keyswitch *const *keys, size_t extra, uint64_t *c0, uint64_t *c1,
+                        const uint64_t *const *higher, size_t batch, void *workspace, void *stream) {
+  return guarded([&] {
+    require(c0 && c1 && workspace && (extra == 0 || (keys && higher)), "null argument");
+    if (extra == 0) return NTT_HIP_OK;
+    const ntt_hip_keyswitch &plan = *keys[0];
+    require(batch >= 1 && batch <= plan.max_batch, "batch exceeds the plan capacity");
+    for (size_t j = 0; j < extra; ++j) {
+      const ntt_hip_keyswitch *k = keys[j];
+      require(k && k->ctx == plan.ctx && k->L == plan.L && k->K == plan.K && k->alpha == plan.alpha,
+              "relinearization keys must share the context and the (L, K, alpha) plan");
+      require(k->key_loaded, "relinearization key not uploaded");
+      require(higher[j], "null higher component");
+    }
+    const hipStream_t s = plan.ctx->pick(stream);
+    const KsBuffers w = ks_buffers(plan, workspace, batch);
+    DeviceGuard g(plan.ctx->device);
+    for (size_t j = 0; j < extra; ++j) {
+      ks_modup(plan, higher[j], batch, w, s);
+      ks_mac(*keys[j], plan, higher[j], batch, w, w.z, s);
+      ks_moddown(plan, w.z, batch, c0, c1, w, s, 3u);
+    }
     return NTT_HIP_OK;
   });
 }

@@ -54,6 +54,7 @@ struct TowerMap {
     const ulonglong2 *scale;  // [k] {c, Shoup(c)}
     uint64_t sub_poly_stride, out_poly_stride;
     uint32_t split;
+    uint32_t accumulate;  // bit j: out[j] += result (mod q) instead of out[j] = result
   } epi;
 };
 

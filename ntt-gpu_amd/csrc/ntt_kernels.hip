@@ -130,10 +130,13 @@ __device__ __forceinline__ void tower_of(const TowerMap &m, uint32_t y, bool fir
   src = (first_pass && m.src) ? m.src + (uint64_t)poly * m.src_poly_stride + src_off : data;
 }
 
-// Epilogue destinations of tower (k, poly): the subtrahend and the output.
-__device__ __forceinline__ uint64_t *epi_dest(const TowerMap &m, uint32_t k, uint32_t poly, const uint64_t *&sub) {
+// Epilogue destinations of tower (k, poly): the subtrahend and the output;
+// acc says whether the output accumulates (in-place key switch) or is written.
+__device__ __forceinline__ uint64_t *epi_dest(const TowerMap &m, uint32_t k, uint32_t poly, const uint64_t *&sub,
+                                              bool &acc) {
   sub = m.epi.sub + (uint64_t)poly * m.epi.sub_poly_stride + (uint64_t)k * m.tower_stride;
   const bool second = poly >= m.epi.split;
+  acc = (m.epi.accumulate >> (second ? 1 : 0)) & 1;
   return m.epi.out[second ? 1 : 0] + (uint64_t)(second ? poly - m.epi.split : poly) * m.epi.out_poly_stride +
          (uint64_t)k * m.tower_stride;
 }
@@ -228,9 +231,18 @@ __global__ __launch_bounds__(256) void ntt_pass_b(TowerMap map, const PrimeConst
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = lds[lds_idx(rl, u + 16 * k)];
     const uint64_t *sub;
-    uint64_t *out = epi_dest(map, tk, tpoly, sub) + (uint64_t)r * kRow;
+    bool acc;
+    uint64_t *out = epi_dest(map, tk, tpoly, sub, acc) + (uint64_t)r * kRow;
     sub += (uint64_t)r * kRow;
     const ulonglong2 c = map.epi.scale[tk];
+    if (acc) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const uint64_t d = Lazy ? sub[u + 16 * k] + m.q4 - sub_if_ge_neg(v[k], m.nq4) : sub_mod(sub[u + 16 * k], v[k], pc.q);
+        out[u + 16 * k] = add_mod(out[u + 16 * k], shoup(d, c.x, c.y, pc.q), pc.q);
+      }
+      return;
+    }
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       const uint64_t d = Lazy ? sub[u + 16 * k] + m.q4 - sub_if_ge_neg(v[k], m.nq4) : sub_mod(sub[u + 16 * k], v[k], pc.q);
@@ -324,8 +336,9 @@ __global__ __launch_bounds__(256) void ntt_lds_generic(TowerMap map, const Prime
   const uint64_t *sub = nullptr;
   uint64_t *out = data;
   ulonglong2 c{0, 0};
+  bool acc = false;
   if (epilogue) {
-    out = epi_dest(map, tk, tpoly, sub);
+    out = epi_dest(map, tk, tpoly, sub, acc);
     c = map.epi.scale[tk];
   }
   for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
@@ -333,7 +346,10 @@ __global__ __launch_bounds__(256) void ntt_lds_generic(TowerMap map, const Prime
     const uint64_t at = (uint64_t)j * stride + col0 + col;
     uint64_t x = lds[col * L + j];
     if (!Inverse && finish) x = canon<Lazy>(x, m);
-    if (epilogue) x = shoup(sub_mod(sub[at], x, pc.q), c.x, c.y, pc.q);
+    if (epilogue) {
+      x = shoup(sub_mod(sub[at], x, pc.q), c.x, c.y, pc.q);
+      if (acc) x = add_mod(out[at], x, pc.q);
+    }
     out[at] = x;
   }
 }

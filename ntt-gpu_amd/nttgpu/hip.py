@@ -67,6 +67,8 @@ SIGNATURES = {
     "ntt_hip_keyswitch_capture": [VP, VP, SZ, VP, VP, VP, VP, ctypes.POINTER(VP)],
     "ntt_hip_graph_launch": [VP, VP],
     "ntt_hip_graph_destroy": [VP],
+    "ntt_hip_keyswitch_in_place": [VP, VP, VP, VP, SZ, VP, VP],
+    "ntt_hip_relinearize": [ctypes.POINTER(VP), SZ, VP, VP, ctypes.POINTER(VP), SZ, VP, VP],
     "ntt_hip_rotate_workspace_bytes": [VP, SZ, SZP],
     "ntt_hip_keyswitch_rotate": [ctypes.POINTER(VP), SZ, ctypes.POINTER(ctypes.c_uint32), VP, VP, SZ,
                                  ctypes.POINTER(VP), ctypes.POINTER(VP), VP, VP],
@@ -353,6 +355,23 @@ class KeySwitch:
 
     def mod_down(self, in_ptr, batch, out_ptr, workspace_ptr, stream=None):
         check(lib().ntt_hip_moddown_run(self.handle, in_ptr, batch, out_ptr, workspace_ptr, stream), "moddown_run")
+
+    def in_place(self, c0_ptr, c1_ptr, batch, workspace_ptr, c2_ptr=None, stream=None):
+        """KeySwitchInPlace: two components (c2 None): c0 += B, c1 = A of c1;
+        three: c0 += B, c1 += A of c2 (reference hybrid_key_switch_in_place)."""
+        check(lib().ntt_hip_keyswitch_in_place(self.handle, c0_ptr, c1_ptr, c2_ptr, batch, workspace_ptr, stream),
+              "keyswitch_in_place")
+
+    @staticmethod
+    def relinearize(keys, c0_ptr, c1_ptr, higher_ptrs, batch, workspace_ptr, stream=None):
+        """RelinearizeInPlace: component 2 + j switched with keys[j], each
+        contribution ModDown-reduced and added into (c0, c1)."""
+        if len(keys) != len(higher_ptrs):
+            raise ContractError("one relinearization key per higher component")
+        R = len(keys)
+        kh = (VP * max(R, 1))(*[k.handle for k in keys])
+        hi = (VP * max(R, 1))(*higher_ptrs)
+        check(lib().ntt_hip_relinearize(kh, R, c0_ptr, c1_ptr, hi, batch, workspace_ptr, stream), "relinearize")
 
     def rotate_workspace_words(self, batch):
         out = SZ(0)

@@ -355,3 +355,94 @@ def test_rotation_rejections():
         with pytest.raises(hip.ContractError):
             hip.KeySwitch.rotate(keys, autos, buf.ptr, buf.ptr, 1, [buf.ptr] * len(keys), [buf.ptr] * len(keys),
                                  buf.ptr)
+
+
+def _add_towers(a, b, q, n):
+    """(a + b) mod q per tower on tower-major arrays of whole components."""
+    qv = np.tile(np.repeat(np.asarray(q, dtype=np.uint64), n), a.size // (len(q) * n))
+    return (a + b) % qv
+
+
+@pytest.mark.parametrize("n,M,K,alpha,L,batch,components", [
+    (1024, 6, 2, 2, 5, 2, 2),
+    (1024, 6, 2, 2, 5, 2, 3),
+    (65536, 24, 6, 6, 24, 1, 2),
+])
+def test_key_switch_in_place(n, M, K, alpha, L, batch, components):
+    """KeySwitchInPlace (openfhe_hybrid_adapter.hpp:221-248; operation graph
+    doc :350-362): two components -> c0 + B, A; three -> c0 + B, c1 + A, with
+    (B, A) the oracle's key switch of the last component."""
+    hp = params.openfhe_like(n, M, K, alpha)
+    log_case(f"key switch in place n={n} L={L} batch={batch} components={components}")
+    q = hp.q[:L]
+    ctx = hip.Context(n, q + hp.p, hp.q_psi[:L] + hp.p_psi)
+    ks = hip.KeySwitch(ctx, L, hp.K, hp.alpha, batch)
+    kb, ka = _key(hp, 6100 + L)
+    ks.upload_key(kb, ka, hp.M, hp.q)
+    cs = [np.concatenate([uniform(n, q, 6200 + 10 * j + c) for c in range(batch)]) for j in range(components)]
+    dev = [_dev(c) for c in cs]
+    ws = hip.DeviceBuffer(ks.workspace_words(batch))
+    ks.in_place(dev[0].ptr, dev[1].ptr, batch, ws.ptr, c2_ptr=dev[2].ptr if components == 3 else None)
+    ctx.synchronize()
+    g0, g1 = dev[0].download(), dev[1].download()
+    for c in range(batch):
+        sl = slice(c * L * n, (c + 1) * L * n)
+        wb, wa = o.key_switch(cs[-1][sl], n, q, hp.q_psi[:L], hp.M, hp.p, hp.p_psi, hp.alpha, kb, ka)
+        assert np.array_equal(g0[sl], _add_towers(cs[0][sl], wb, q, n)), f"c0 of component {c}"
+        want1 = wa if components == 2 else _add_towers(cs[1][sl], wa, q, n)
+        assert np.array_equal(g1[sl], want1), f"c1 of component {c}"
+    if components == 3:
+        assert np.array_equal(dev[2].download(), cs[2]), "c2 must be left as it was"
+
+
+@pytest.mark.parametrize("extra", [0, 1, 2])
+def test_relinearize(extra):
+    """RelinearizeInPlace (openfhe_hybrid_adapter.hpp:250-286): each higher
+    component switched with its own key and ModDown-reduced on its own, the
+    contributions added into c0 / c1; extra = 0 leaves the ciphertext as is."""
+    hp = params.openfhe_like(1024, 6, 2, 2)
+    n, L, batch = hp.n, 6, 2
+    log_case(f"relinearize n={n} L={L} batch={batch} extra={extra}")
+    q = hp.q[:L]
+    ctx = hip.Context(n, q + hp.p, hp.q_psi[:L] + hp.p_psi)
+    keys = []
+    for j in range(max(extra, 1)):
+        ks = hip.KeySwitch(ctx, L, hp.K, hp.alpha, batch)
+        kb, ka = _key(hp, 6500 + 41 * j)
+        ks.upload_key(kb, ka, hp.M, hp.q)
+        keys.append((ks, kb, ka))
+    cs = [np.concatenate([uniform(n, q, 6600 + 10 * j + c) for c in range(batch)]) for j in range(2 + extra)]
+    dev = [_dev(c) for c in cs]
+    ws = hip.DeviceBuffer(keys[0][0].workspace_words(batch))
+    hip.KeySwitch.relinearize([k[0] for k in keys[:extra]], dev[0].ptr, dev[1].ptr, [d.ptr for d in dev[2:]], batch,
+                              ws.ptr)
+    ctx.synchronize()
+    g0, g1 = dev[0].download(), dev[1].download()
+    for c in range(batch):
+        sl = slice(c * L * n, (c + 1) * L * n)
+        w0, w1 = cs[0][sl].copy(), cs[1][sl].copy()
+        for j in range(extra):
+            wb, wa = o.key_switch(cs[2 + j][sl], n, q, hp.q_psi[:L], hp.M, hp.p, hp.p_psi, hp.alpha, keys[j][1],
+                                  keys[j][2])
+            w0, w1 = _add_towers(w0, wb, q, n), _add_towers(w1, wa, q, n)
+        assert np.array_equal(g0[sl], w0), f"c0 of component {c}"
+        assert np.array_equal(g1[sl], w1), f"c1 of component {c}"
+
+
+def test_relinearize_rejections():
+    hp = params.openfhe_like(1024, 6, 2, 2)
+    ctx = hip.Context(hp.n, hp.q + hp.p, hp.q_psi + hp.p_psi)
+    ks = hip.KeySwitch(ctx, 6, 2, 2, 1)
+    other = hip.KeySwitch(ctx, 6, 2, 3, 1)
+    kb, ka = _key(hp, 6900)
+    ks.upload_key(kb, ka, hp.M, hp.q)
+    d = hip.DeviceBuffer(6 * hp.n)
+    ws = hip.DeviceBuffer(ks.workspace_words(1))
+    with pytest.raises(hip.ContractError, match="not uploaded"):
+        other.in_place(d.ptr, d.ptr, 1, ws.ptr)
+    with pytest.raises(hip.ContractError, match="share the context"):
+        hip.KeySwitch.relinearize([ks, other], d.ptr, d.ptr, [d.ptr, d.ptr], 1, ws.ptr)
+    with pytest.raises(hip.ContractError, match="capacity"):
+        ks.in_place(d.ptr, d.ptr, 2, ws.ptr)
+    with pytest.raises(hip.ContractError, match="one relinearization key"):
+        hip.KeySwitch.relinearize([ks], d.ptr, d.ptr, [], 1, ws.ptr)

@@ -53,10 +53,11 @@ def seeded(n, moduli, seed):
     return np.concatenate([rng.integers(0, q, n, dtype=np.uint64) for q in moduli])
 
 
-def ntt_roofline(towers, reps, stream):
-    """Forward NTT over `towers` distinct 2^16-point towers (one ~2^59-bit prime):
-    the key switch's dominant kernel pair, measured with HIP events on its stream."""
-    q = params.last_prime(59, 2 * N)
+def ntt_roofline(towers, reps, stream, bits=59):
+    """Forward NTT over `towers` distinct 2^16-point towers (one prime just below
+    2^bits): the key switch's dominant kernel pair, measured with HIP events on
+    its stream. bits <= 30 runs the 32-bit butterflies of narrow towers."""
+    q = params.last_prime(bits, 2 * N)
     ctx = hip.Context(N, [q], [params.min_root(q, N)])
     data = hip.DeviceBuffer(towers * N)
     chunk = 256
@@ -85,8 +86,8 @@ def ntt_roofline(towers, reps, stream):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": measured_traffic(towers),
-        "kernel": "forward NTT (ntt_pass_a256 + ntt_pass_b), N=65536",
+        "traffic": measured_traffic(towers) if bits > 30 else None,
+        "kernel": f"forward NTT (ntt_pass_a256 + ntt_pass_b), N=65536, {bits}-bit prime",
         "towers_per_launch": towers,
         "launch_ms": round(ms, 4),
         "inverse_launch_ms": round(statistics.mean(inv), 4),
@@ -110,11 +111,12 @@ def measured_traffic(towers):
     return int(per_tower * towers)
 
 
-def valu_roofline(roof):
+def valu_roofline(roof, narrow=None):
     """The forward NTT against the VALU roof measured on this device: the rate
     of the same wide-lazy forward butterfly in independent chains
     (ntt-gpu_amd/bin/valu_probe), versus butterflies per second achieved by
-    the roofline launch (16 stages x N/2 butterflies per tower)."""
+    the roofline launch (16 stages x N/2 butterflies per tower); `narrow`
+    (the 30-bit roofline launch) against the 32-bit butterfly's rate."""
     if not os.path.exists(VALU_PROBE):
         return None
     try:
@@ -134,6 +136,12 @@ def valu_roofline(roof):
         return None
     peak = rows["ct8_forward_butterfly"]["per_s"]
     achieved = BUTTERFLIES_PER_TOWER * roof["towers_per_launch"] / (roof["launch_ms"] * 1e-3)
+    narrow_frac = None
+    if narrow and "ct32_forward_butterfly" in rows:
+        rate = BUTTERFLIES_PER_TOWER * narrow["towers_per_launch"] / (narrow["launch_ms"] * 1e-3)
+        narrow_frac = {"achieved": round(rate / 1e12, 4),
+                       "peak": round(rows["ct32_forward_butterfly"]["per_s"] / 1e12, 4),
+                       "frac": round(rate / rows["ct32_forward_butterfly"]["per_s"], 4)}
     return {
         "bound": "valu",
         "achieved": round(achieved / 1e12, 4),
@@ -142,6 +150,7 @@ def valu_roofline(roof):
         "frac": round(achieved / peak, 4),
         "peak_source": "valu_probe: forward butterfly chains on this device",
         "probe": {k: round(v["per_s"] / 1e12, 4) for k, v in rows.items()},
+        "narrow": narrow_frac,
     }
 
 
@@ -421,7 +430,9 @@ def main():
     progress(f"hoisted rotations: {rot['rotations_per_s']} rotations/s")
     roof = ntt_roofline(args.ntt_towers, 6, s)
     progress(f"ntt roofline done: {roof['achieved']} GB/s")
-    valu = valu_roofline(roof)
+    narrow = ntt_roofline(args.ntt_towers, 6, s, bits=30)
+    progress(f"narrow (30-bit) ntt roofline: {narrow['achieved']} GB/s")
+    valu = valu_roofline(roof, narrow)
     if valu:
         progress(f"valu roofline: {valu['achieved']} of {valu['peak']} T butterfly/s")
     served = None
@@ -460,6 +471,7 @@ def main():
         "hoisted_rotations": rot,
         "roofline": roof,
         "valu_roofline": valu,
+        "narrow_ntt_roofline": narrow,
         "cpu_baseline": cpu,
         "bit_exact_vs_oracle": exact,
         "served_stream": served,

@@ -48,6 +48,11 @@ int ntt_hip_abi_version(void);
  * src/cuda/ntt_kernels.cu:57-75 DeviceGuard). ---- */
 int ntt_hip_device_count(int *count);
 int ntt_hip_device_name(int device, char *name, size_t capacity);
+/* Architecture name (gcnArchName), device memory, compute units, driver and
+ * runtime versions: the device description of the reference CLI's info and
+ * JSONL workload lines (src/cuda/ntt_kernels.cu:491-506, :664-680). */
+int ntt_hip_device_properties(int device, char *arch, size_t arch_capacity, uint64_t *memory_bytes,
+                              int *compute_units, int *driver_version, int *runtime_version);
 int ntt_hip_malloc(int device, void **ptr, size_t bytes);
 int ntt_hip_free(int device, void *ptr);
 int ntt_hip_host_alloc(void **ptr, size_t bytes); /* pinned host staging */

@@ -168,7 +168,7 @@ struct ntt_hip_context {
   int device = 0;
   uint32_t n = 0, log_n = 0;
   std::vector<uint64_t> q, psi;
-  bool lazy = true;
+  int arith = kLazy;  // NTT arithmetic mode for every tower of the basis
   uint32_t ntt_chunk = 0;      // towers per NTT pass pair (0 = all), set by the autotuner
   uint32_t conv_targets = 0;   // basis-conversion targets per workgroup (0 = from the grid size)
   hipStream_t stream = nullptr;
@@ -411,6 +411,22 @@ int ntt_hip_device_name(int device, char *name, size_t capacity) {
   });
 }
 
+int ntt_hip_device_properties(int device, char *arch, size_t arch_capacity, uint64_t *memory_bytes,
+                              int *compute_units, int *driver_version, int *runtime_version) {
+  return guarded([&] {
+    require(arch && arch_capacity > 0 && memory_bytes && compute_units && driver_version && runtime_version,
+            "null argument");
+    hipDeviceProp_t p;
+    HIP_CHECK(hipGetDeviceProperties(&p, device));
+    HIP_CHECK(hipDriverGetVersion(driver_version));
+    HIP_CHECK(hipRuntimeGetVersion(runtime_version));
+    std::snprintf(arch, arch_capacity, "%s", p.gcnArchName);
+    *memory_bytes = p.totalGlobalMem;
+    *compute_units = p.multiProcessorCount;
+    return NTT_HIP_OK;
+  });
+}
+
 int ntt_hip_malloc(int device, void **ptr, size_t bytes) {
   return guarded([&] {
     require(ptr != nullptr && bytes > 0, "invalid allocation request");
@@ -551,8 +567,8 @@ int ntt_hip_context_create(ntt_hip_context **out, int device, uint64_t n, const 
     while ((1u << c->log_n) < n) ++c->log_n;
     c->q.assign(q, q + towers);
     c->psi.assign(psi, psi + towers);
-    // wide-lazy butterflies keep residues below 8q < 2^64
-    c->lazy = std::all_of(c->q.begin(), c->q.end(), [](uint64_t v) { return v < (uint64_t{1} << 61); });
+    // 32-bit butterflies when every q < 2^30; wide-lazy (residues below 8q < 2^64) when every q < 2^61
+    c->arith = arith_for(*std::max_element(c->q.begin(), c->q.end()));
     // Host tables: fwd[k] = {psi^bitrev(k), Shoup}, inv[k] = {psi^-bitrev(k), Shoup}
     // (reference transform_stage.hpp:19-30), one thread per group of towers.
     std::vector<ulonglong2> host(towers * 2 * (size_t)n);
@@ -598,6 +614,9 @@ int ntt_hip_context_create(ntt_hip_context **out, int device, uint64_t n, const 
       p.last_w = mulmod(w1, p.ninv, qt);
       p.last_wp = shoup_of(p.last_w, qt);
       p.r2 = (uint64_t)(((u128)mont_of(1, qt) << 64) % qt);  // 2^128 mod q
+      const u128 mu = ~(u128)0 / qt;  // = floor(2^128 / q): q is odd, so it does not divide 2^128
+      p.mu_lo = (uint64_t)mu;
+      p.mu_hi = (uint64_t)(mu >> 64);
       p.fwd = dt + (t * 2) * (size_t)n;
       p.inv = dt + (t * 2 + 1) * (size_t)n;
     }
@@ -637,7 +656,7 @@ int ntt_hip_transform_batch(ntt_hip_context *ctx, uint64_t *data, const uint64_t
             "polynomial stride shorter than its towers");
     DeviceGuard g(ctx->device);
     const TowerMap m = affine(data, src, ctx->n, towers, first, poly_stride, src_poly_stride);
-    hip_ok(launch_ntt(m, ctx->dprimes(), ctx->log_n, (uint32_t)(polys * towers), inverse != 0, ctx->lazy,
+    hip_ok(launch_ntt(m, ctx->dprimes(), ctx->log_n, (uint32_t)(polys * towers), inverse != 0, ctx->arith,
                       ctx->pick(stream), ctx->ntt_chunk),
            "ntt launch");
     return NTT_HIP_OK;
@@ -881,7 +900,7 @@ void ks_modup(const ntt_hip_keyswitch &ks, const uint64_t *in, size_t B, const K
   const size_t L = ks.L, T = ks.L + ks.K, n = c.n;
   const PrimeConst *pr = c.dprimes();
   hip_ok(launch_ntt(affine(w.coeff, in, n, L, 0, ks.coeff_words(), L * n), pr, c.log_n, (uint32_t)(B * L), true,
-                    c.lazy, s, c.ntt_chunk),
+                    c.arith, s, c.ntt_chunk),
          "intt");
   for (uint32_t d = 0; d < ks.beta; ++d) {
     const auto &plan = *ks.up[d];
@@ -906,7 +925,7 @@ void ks_modup(const ntt_hip_keyswitch &ks, const uint64_t *in, size_t B, const K
   m.tower_stride = n;
   m.poly_stride = ks.digit_words();
   m.towers_per_poly = (uint32_t)ks.jobs_per_poly;
-  hip_ok(launch_ntt(m, pr, c.log_n, (uint32_t)(B * ks.jobs_per_poly), false, c.lazy, s, c.ntt_chunk), "modup ntt");
+  hip_ok(launch_ntt(m, pr, c.log_n, (uint32_t)(B * ks.jobs_per_poly), false, c.arith, s, c.ntt_chunk), "modup ntt");
 }
 
 // Key inner product over all digits of `plan`'s workspace with `key`'s
@@ -945,7 +964,7 @@ void ks_moddown(const ntt_hip_keyswitch &ks, const uint64_t *z, size_t B, uint64
   const size_t L = ks.L, K = ks.K, n = c.n;
   const PrimeConst *pr = c.dprimes();
   hip_ok(launch_ntt(affine(w.pc, z + L * n, n, K, L, ks.pc_words(), ks.z_words()), pr, c.log_n,
-                    (uint32_t)(2 * B * K), true, c.lazy, s, c.ntt_chunk),
+                    (uint32_t)(2 * B * K), true, c.arith, s, c.ntt_chunk),
          "moddown intt");
   BasisLaunch a{};
   a.plan = ks.down->dev;
@@ -963,7 +982,7 @@ void ks_moddown(const ntt_hip_keyswitch &ks, const uint64_t *z, size_t B, uint64
   TowerMap fin = with_correction(affine(w.corr, nullptr, n, L, 0, ks.corr_words(), 0), z, ks.z_words(), out_b, out_a,
                                  (uint32_t)B, L * n, ks.pinv);
   fin.epi.accumulate = accumulate;
-  hip_ok(launch_ntt(fin, pr, c.log_n, (uint32_t)(2 * B * L), false, c.lazy, s, c.ntt_chunk),
+  hip_ok(launch_ntt(fin, pr, c.log_n, (uint32_t)(2 * B * L), false, c.arith, s, c.ntt_chunk),
          "moddown ntt + correction");
 }
 }  // namespace
@@ -1202,18 +1221,18 @@ int ntt_hip_modup_run(ntt_hip_keyswitch *ks, const uint64_t *in, int in_eval, si
     DeviceGuard g(c.device);
     if (in_eval) {
       // Q part of the output is the input itself; coefficients for the lift
-      hip_ok(launch_ntt(affine(coeff, in, n, L, 0, L * n, L * n), pr, c.log_n, (uint32_t)(batch * L), true, c.lazy, s, c.ntt_chunk),
+      hip_ok(launch_ntt(affine(coeff, in, n, L, 0, L * n, L * n), pr, c.log_n, (uint32_t)(batch * L), true, c.arith, s, c.ntt_chunk),
              "modup intt");
       for (size_t b = 0; b < batch; ++b)
         HIP_CHECK(hipMemcpyAsync(out + b * T * n, in + b * L * n, L * n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
       launch_basis(*ks->full_up, c.conv_targets, coeff, out + L * n, L * n, T * n, (uint32_t)n, (uint32_t)batch, s);
     } else {
-      hip_ok(launch_ntt(affine(out, in, n, L, 0, T * n, L * n), pr, c.log_n, (uint32_t)(batch * L), false, c.lazy, s, c.ntt_chunk),
+      hip_ok(launch_ntt(affine(out, in, n, L, 0, T * n, L * n), pr, c.log_n, (uint32_t)(batch * L), false, c.arith, s, c.ntt_chunk),
              "modup ntt q");
       launch_basis(*ks->full_up, c.conv_targets, in, out + L * n, L * n, T * n, (uint32_t)n, (uint32_t)batch, s);
     }
     hip_ok(launch_ntt(affine(out + L * n, nullptr, n, K, L, T * n, 0), pr, c.log_n, (uint32_t)(batch * K), false,
-                      c.lazy, s, c.ntt_chunk),
+                      c.arith, s, c.ntt_chunk),
            "modup ntt p");
     return NTT_HIP_OK;
   });
@@ -1239,13 +1258,13 @@ int ntt_hip_moddown_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batch,
     uint64_t *pc = (uint64_t *)workspace;  // batch * K * n
     uint64_t *corr = pc + batch * K * n;   // batch * L * n
     DeviceGuard g(c.device);
-    hip_ok(launch_ntt(affine(pc, in + L * n, n, K, L, K * n, T * n), pr, c.log_n, (uint32_t)(batch * K), true, c.lazy,
+    hip_ok(launch_ntt(affine(pc, in + L * n, n, K, L, K * n, T * n), pr, c.log_n, (uint32_t)(batch * K), true, c.arith,
                       s, c.ntt_chunk),
            "moddown intt");
     launch_basis(*ks->down, c.conv_targets, pc, corr, K * n, L * n, (uint32_t)n, (uint32_t)batch, s);
     hip_ok(launch_ntt(with_correction(affine(corr, nullptr, n, L, 0, L * n, 0), in, T * n, out, out, (uint32_t)batch,
                                       L * n, ks->pinv),
-                      pr, c.log_n, (uint32_t)(batch * L), false, c.lazy, s, c.ntt_chunk),
+                      pr, c.log_n, (uint32_t)(batch * L), false, c.arith, s, c.ntt_chunk),
            "moddown ntt + correction");
     return NTT_HIP_OK;
   });
@@ -1323,8 +1342,8 @@ int ntt_hip_autotune(ntt_hip_context *ctx, size_t towers, char *report, size_t c
       if (chunk && chunk >= towers) continue;
       TowerMap m = affine(a, nullptr, n, 1, 0, n, 0);
       const float ms = time_ms(c, [&] {
-        hip_ok(launch_ntt(m, c.dprimes(), c.log_n, (uint32_t)towers, false, c.lazy, c.stream, chunk), "tune ntt");
-        hip_ok(launch_ntt(m, c.dprimes(), c.log_n, (uint32_t)towers, true, c.lazy, c.stream, chunk), "tune ntt");
+        hip_ok(launch_ntt(m, c.dprimes(), c.log_n, (uint32_t)towers, false, c.arith, c.stream, chunk), "tune ntt");
+        hip_ok(launch_ntt(m, c.dprimes(), c.log_n, (uint32_t)towers, true, c.arith, c.stream, chunk), "tune ntt");
       }, 3);
       rep += "\"" + std::to_string(chunk) + "\":" + std::to_string(ms) + ",";
       if (ms < best * 0.98f) {  // prefer fewer launches unless clearly faster

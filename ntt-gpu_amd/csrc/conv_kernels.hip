@@ -261,7 +261,7 @@ __global__ __launch_bounds__(256) void pointwise_kernel(uint64_t *out, const uin
   const uint64_t idx = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (idx >= (uint64_t)n * towers) return;
   const PrimeConst &pc = primes[first_prime + idx / n];
-  out[idx] = mont_mul(mont_mul(out[idx], right[idx], pc.q, pc.qinv_neg), pc.r2, pc.q, pc.qinv_neg);
+  out[idx] = barrett_mul(out[idx], right[idx], pc.q, pc.mu_lo, pc.mu_hi);
 }
 
 hipError_t launch_pointwise(uint64_t *out, const uint64_t *right, const PrimeConst *primes, uint32_t first_prime,

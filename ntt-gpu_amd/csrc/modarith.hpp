@@ -177,4 +177,97 @@ __host__ __device__ __forceinline__ uint64_t canon8(uint64_t x, const Lazy8 &m) 
   return sub_if_ge_neg(sub_if_ge_neg(sub_if_ge_neg(x, m.nq4), m.nq2), m.nq);
 }
 
+// ---------------------------------------------------------------- Barrett
+// Variable x variable products a * b mod q (the pointwise product of two
+// residue arrays; the reference's candidate form is include/ntt/barrett.hpp:6-49)
+// with mu = floor(2^128 / q): one reduction instead of two
+// Montgomery steps (the product would otherwise need a conversion into the
+// Montgomery domain). With x = ab = xh 2^64 + xl and mu = mh 2^64 + ml,
+//   floor(x mu / 2^128) = xh mh + hi(xh ml) + hi(xl mh) + c,
+// c the carry out of lo(xh ml) + lo(xl mh) + hi(xl ml), in {0, 1, 2}.
+// floor(x mu / 2^128) is floor(x / q) or one less (x < 2^128), so with the
+// carry the remainder x - e q is in [0, 2q); without it (e short by up to 2
+// more) in [0, 4q), which fits a word only when q < 2^62.
+__host__ __device__ __forceinline__ uint64_t mulhi64_hd(uint64_t a, uint64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __umul64hi(a, b);
+#else
+  return (uint64_t)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+// q < 2^62: carries dropped, two conditional subtracts.
+__host__ __device__ __forceinline__ uint64_t barrett_mul_lazy(uint64_t a, uint64_t b, uint64_t q, uint64_t mu_lo,
+                                                              uint64_t mu_hi) {
+  const uint64_t xl = a * b, xh = mulhi64_hd(a, b);
+  const uint64_t e = xh * mu_hi + mulhi64_hd(xh, mu_lo) + mulhi64_hd(xl, mu_hi);
+  const uint64_t r = xl - e * q;  // [0, 4q)
+  const uint64_t r2 = r >= 2 * q ? r - 2 * q : r;
+  return r2 >= q ? r2 - q : r2;
+}
+
+// q < 2^63: every carry kept, one conditional subtract.
+__host__ __device__ __forceinline__ uint64_t barrett_mul_exact(uint64_t a, uint64_t b, uint64_t q, uint64_t mu_lo,
+                                                               uint64_t mu_hi) {
+  const uint64_t xl = a * b, xh = mulhi64_hd(a, b);
+  const uint64_t s1 = xh * mu_lo, s2 = xl * mu_hi, s3 = mulhi64_hd(xl, mu_lo);
+  const uint64_t t = s1 + s2;
+  const uint64_t u = t + s3;
+  const uint64_t c = (uint64_t)(t < s1) + (uint64_t)(u < t);
+  const uint64_t e = xh * mu_hi + mulhi64_hd(xh, mu_lo) + mulhi64_hd(xl, mu_hi) + c;
+  const uint64_t r = xl - e * q;  // [0, 2q)
+  return r >= q ? r - q : r;
+}
+
+__host__ __device__ __forceinline__ uint64_t barrett_mul(uint64_t a, uint64_t b, uint64_t q, uint64_t mu_lo,
+                                                         uint64_t mu_hi) {
+  return q < (uint64_t{1} << 62) ? barrett_mul_lazy(a, b, q, mu_lo, mu_hi) : barrett_mul_exact(a, b, q, mu_lo, mu_hi);
+}
+
+// ---------------------------------------------------------------- narrow towers
+// q < 2^30: residues fit 32-bit registers with the Harvey lazy ranges
+// ([0, 4q) forward, [0, 2q) inverse, 4q < 2^32), so a butterfly is one
+// 32-bit high product and two 32-bit low products instead of 64-bit chains.
+// The Shoup companion is floor(w 2^32 / q) = the high word of the 64-bit
+// companion floor(w 2^64 / q) (a floor of a floor by 2^32), so the 64-bit
+// twiddle tables serve both widths.
+struct Narrow {
+  uint32_t q, q2;
+};
+
+__host__ __device__ __forceinline__ Narrow narrow_of(uint64_t q) { return {(uint32_t)q, (uint32_t)(2 * q)}; }
+
+// x < 2c, c <= 2^31 -> x mod c: x - c wraps above x exactly when x < c.
+__host__ __device__ __forceinline__ uint32_t min_sub(uint32_t x, uint32_t c) {
+  const uint32_t d = x - c;
+  return d < x ? d : x;
+}
+
+// y * w mod q in [0, 2q) for any y < 2^32, w < q < 2^31, wp = floor(w 2^32 / q).
+__host__ __device__ __forceinline__ uint32_t shoup32(uint32_t y, uint32_t w, uint32_t wp, uint32_t q) {
+  const uint32_t qh = (uint32_t)(((uint64_t)y * wp) >> 32);
+  return y * w - qh * q;
+}
+
+// Forward (Cooley-Tukey): x, y in [0, 4q) -> x + yW, x - yW in [0, 4q).
+__host__ __device__ __forceinline__ void ct32(uint32_t &x, uint32_t &y, uint32_t w, uint32_t wp, const Narrow &m) {
+  const uint32_t a = min_sub(x, m.q2);      // [0, 2q)
+  const uint32_t t = shoup32(y, w, wp, m.q);  // [0, 2q)
+  x = a + t;
+  y = a - t + m.q2;
+}
+
+// Inverse (Gentleman-Sande): x, y in [0, 2q) -> x + y, (x - y) W in [0, 2q).
+__host__ __device__ __forceinline__ void gs32(uint32_t &x, uint32_t &y, uint32_t w, uint32_t wp, const Narrow &m) {
+  const uint32_t s = x + y;          // [0, 4q)
+  const uint32_t d = x - y + m.q2;   // (0, 4q)
+  x = min_sub(s, m.q2);
+  y = shoup32(d, w, wp, m.q);
+}
+
+// [0, 4q) -> [0, q).
+__host__ __device__ __forceinline__ uint32_t canon4_32(uint32_t x, const Narrow &m) {
+  return min_sub(min_sub(x, m.q2), m.q);
+}
+
 }  // namespace nttamd

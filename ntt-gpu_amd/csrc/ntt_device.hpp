@@ -14,6 +14,7 @@ struct PrimeConst {
   uint64_t last_w, last_wp;          // psi^-bitrev(1) * n^-1 and Shoup (inverse last stage)
   uint64_t r2;                       // 2^128 mod q (to enter Montgomery form)
   uint64_t nq, nq2, nq4;             // 2^64 - q, 2^64 - 2q, 2^64 - 4q (precomputed: see Lazy8)
+  uint64_t mu_lo, mu_hi;             // floor(2^128 / q) (Barrett, variable x variable products)
   const ulonglong2 *fwd;
   const ulonglong2 *inv;
 };
@@ -58,7 +59,13 @@ struct TowerMap {
   } epi;
 };
 
+// Arithmetic of a launch, chosen per context from its largest modulus.
+enum Arith : int { kExact = 0, kLazy = 1, kNarrow = 2 };
+inline int arith_for(uint64_t max_q) {
+  return max_q < (uint64_t{1} << 30) ? kNarrow : max_q < (uint64_t{1} << 61) ? kLazy : kExact;
+}
+
 hipError_t launch_ntt(const TowerMap &map, const PrimeConst *primes, uint32_t log_n, uint32_t towers, bool inverse,
-                      bool lazy, hipStream_t stream, uint32_t chunk = 0);
+                      int arith, hipStream_t stream, uint32_t chunk = 0);
 
 }  // namespace nttamd

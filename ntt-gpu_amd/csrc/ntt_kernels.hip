@@ -21,6 +21,8 @@
 //
 // Pass A for R < 256 (N = 2^9..2^15) and whole transforms for N <= 2^12 use
 // a plain LDS-staged kernel (one radix-2 stage per barrier).
+#include <type_traits>
+
 #include "ntt_device.hpp"
 
 namespace nttamd {
@@ -40,12 +42,20 @@ typedef const __attribute__((address_space(1))) tw2 *TwPtr;
 
 __device__ __forceinline__ TwPtr tw_table(const ulonglong2 *p) { return (TwPtr)(const void *)p; }
 
-// Lazy: q < 2^61, forward residues in [0, 8q), inverse in [0, 4q) (ct8/gs8).
-// Exact: 2^61 <= q < 2^63, canonical throughout.
-template <bool Lazy>
-__device__ __forceinline__ void ct_bfly(uint64_t &x, uint64_t &y, tw2 w, const Lazy8 &m) {
-  if constexpr (Lazy) {
+// Arithmetic modes (the A template argument):
+//   kLazy   q < 2^61: 64-bit wide-lazy (forward residues in [0, 8q), inverse in [0, 4q); ct8/gs8);
+//   kExact  2^61 <= q < 2^63: canonical throughout;
+//   kNarrow q < 2^30: 32-bit registers and LDS words (ct32/gs32, Harvey ranges).
+// Global memory holds uint64 residues in every mode.
+template <int A>
+using Val = std::conditional_t<A == kNarrow, uint32_t, uint64_t>;
+
+template <int A>
+__device__ __forceinline__ void ct_bfly(Val<A> &x, Val<A> &y, tw2 w, const Lazy8 &m) {
+  if constexpr (A == kLazy) {
     ct8(x, y, w.x, w.y, m);
+  } else if constexpr (A == kNarrow) {
+    ct32(x, y, (uint32_t)w.x, (uint32_t)(w.y >> 32), narrow_of(m.q));
   } else {
     const uint64_t t = shoup(y, w.x, w.y, m.q);
     y = sub_mod(x, t, m.q);
@@ -53,10 +63,12 @@ __device__ __forceinline__ void ct_bfly(uint64_t &x, uint64_t &y, tw2 w, const L
   }
 }
 
-template <bool Lazy>
-__device__ __forceinline__ void gs_bfly(uint64_t &x, uint64_t &y, tw2 w, const Lazy8 &m) {
-  if constexpr (Lazy) {
+template <int A>
+__device__ __forceinline__ void gs_bfly(Val<A> &x, Val<A> &y, tw2 w, const Lazy8 &m) {
+  if constexpr (A == kLazy) {
     gs8(x, y, w.x, w.y, m);
+  } else if constexpr (A == kNarrow) {
+    gs32(x, y, (uint32_t)w.x, (uint32_t)(w.y >> 32), narrow_of(m.q));
   } else {
     const uint64_t s = add_mod(x, y, m.q);
     y = shoup(sub_mod(x, y, m.q), w.x, w.y, m.q);
@@ -65,14 +77,19 @@ __device__ __forceinline__ void gs_bfly(uint64_t &x, uint64_t &y, tw2 w, const L
 }
 
 // Final inverse stage with n^-1 folded in: x' = (x+y) n^-1, y' = (x-y) W n^-1,
-// both canonical (exact Shoup quotient; lazy inputs in [0, 4q)).
-template <bool Lazy>
-__device__ __forceinline__ void gs_bfly_last(uint64_t &x, uint64_t &y, const PrimeConst &pc, const Lazy8 &m) {
-  if constexpr (Lazy) {
+// both canonical (lazy inputs in [0, 4q), narrow inputs in [0, 2q)).
+template <int A>
+__device__ __forceinline__ void gs_bfly_last(Val<A> &x, Val<A> &y, const PrimeConst &pc, const Lazy8 &m) {
+  if constexpr (A == kLazy) {
     const uint64_t s = x + y;
     const uint64_t d = x + m.q4 - y;
     x = reduce_once(shoup_lazy(s, pc.ninv, pc.ninv_shoup, pc.q), pc.q);
     y = reduce_once(shoup_lazy(d, pc.last_w, pc.last_wp, pc.q), pc.q);
+  } else if constexpr (A == kNarrow) {
+    const Narrow nm = narrow_of(pc.q);
+    const uint32_t s = x + y, d = x - y + nm.q2;  // [0, 4q), (0, 4q)
+    x = min_sub(shoup32(s, (uint32_t)pc.ninv, (uint32_t)(pc.ninv_shoup >> 32), nm.q), nm.q);
+    y = min_sub(shoup32(d, (uint32_t)pc.last_w, (uint32_t)(pc.last_wp >> 32), nm.q), nm.q);
   } else {
     const uint64_t s = add_mod(x, y, pc.q);
     const uint64_t d = sub_mod(x, y, pc.q);
@@ -81,17 +98,28 @@ __device__ __forceinline__ void gs_bfly_last(uint64_t &x, uint64_t &y, const Pri
   }
 }
 
-template <bool Lazy>
-__device__ __forceinline__ uint64_t canon(uint64_t x, const Lazy8 &m) {
-  if constexpr (Lazy) return canon8(x, m);
-  return x;
+// Forward output -> canonical.
+template <int A>
+__device__ __forceinline__ Val<A> canon(Val<A> x, const Lazy8 &m) {
+  if constexpr (A == kLazy) return canon8(x, m);
+  else if constexpr (A == kNarrow) return canon4_32(x, narrow_of(m.q));
+  else return x;
+}
+
+// ModDown correction operand: sub - NTT value, congruent mod q, below 2^64
+// (the exact Shoup product that follows reduces it).
+template <int A>
+__device__ __forceinline__ uint64_t epi_diff(uint64_t sub, Val<A> x, const Lazy8 &m, uint64_t q) {
+  if constexpr (A == kLazy) return sub + m.q4 - sub_if_ge_neg(x, m.nq4);
+  else if constexpr (A == kNarrow) return sub_mod(sub, canon4_32(x, narrow_of(q)), q);
+  else return sub_mod(sub, x, q);
 }
 
 // Four register stages on v[0..15]. H runs 8,4,2,1 (forward) or 1,2,4,8
 // (inverse). Twiddle index = ml * row_factor + group, ml = (8/h) * ml_scale,
 // group = g_base * (8/h) + k / (2h).
-template <bool Inverse, bool Lazy>
-__device__ __forceinline__ void reg_stages(uint64_t (&v)[16], TwPtr tw, uint32_t ml_scale, uint32_t row_factor,
+template <bool Inverse, int A>
+__device__ __forceinline__ void reg_stages(Val<A> (&v)[16], TwPtr tw, uint32_t ml_scale, uint32_t row_factor,
                                            uint32_t g_base, const Lazy8 &m) {
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
@@ -103,8 +131,8 @@ __device__ __forceinline__ void reg_stages(uint64_t (&v)[16], TwPtr tw, uint32_t
       if (k & h) continue;
       const uint32_t idx = ml * row_factor + gb + (uint32_t)(k / (2 * h));
       const tw2 w = tw[idx];
-      if constexpr (Inverse) gs_bfly<Lazy>(v[k], v[k + h], w, m);
-      else ct_bfly<Lazy>(v[k], v[k + h], w, m);
+      if constexpr (Inverse) gs_bfly<A>(v[k], v[k + h], w, m);
+      else ct_bfly<A>(v[k], v[k + h], w, m);
     }
   }
 }
@@ -142,9 +170,9 @@ __device__ __forceinline__ uint64_t *epi_dest(const TowerMap &m, uint32_t k, uin
 }
 
 // ---------------------------------------------------------------- pass A, R = 256
-template <bool Inverse, bool Lazy>
+template <bool Inverse, int A>
 __global__ __launch_bounds__(256) void ntt_pass_a256(TowerMap map, const PrimeConst *__restrict__ primes) {
-  __shared__ uint64_t lds[16 * kLdsRow];
+  __shared__ Val<A> lds[16 * kLdsRow];
   uint64_t *data;
   const uint64_t *src;
   uint32_t prime, tk, tpoly;
@@ -154,23 +182,23 @@ __global__ __launch_bounds__(256) void ntt_pass_a256(TowerMap map, const PrimeCo
   const Lazy8 m = lazy8_of(pc.q, pc.nq, pc.nq2, pc.nq4);
   const int t = threadIdx.x, cl = t & 15, u = t >> 4;
   const uint32_t c = blockIdx.x * 16 + cl;  // column
-  uint64_t v[16];
+  Val<A> v[16];
   if constexpr (!Inverse) {
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = src[(uint64_t)(u + 16 * k) * kRow + c];
-    reg_stages<false, Lazy>(v, tw, 1, 1, 0, m);
+    reg_stages<false, A>(v, tw, 1, 1, 0, m);
 #pragma unroll
     for (int k = 0; k < 16; ++k) lds[lds_idx(cl, u + 16 * k)] = v[k];
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = lds[lds_idx(cl, 16 * u + k)];
-    reg_stages<false, Lazy>(v, tw, 16, 1, u, m);
+    reg_stages<false, A>(v, tw, 16, 1, u, m);
 #pragma unroll
     for (int k = 0; k < 16; ++k) data[(uint64_t)(16 * u + k) * kRow + c] = v[k];
   } else {
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = data[(uint64_t)(16 * u + k) * kRow + c];
-    reg_stages<true, Lazy>(v, tw, 16, 1, u, m);
+    reg_stages<true, A>(v, tw, 16, 1, u, m);
 #pragma unroll
     for (int k = 0; k < 16; ++k) lds[lds_idx(cl, 16 * u + k)] = v[k];
     __syncthreads();
@@ -184,11 +212,11 @@ __global__ __launch_bounds__(256) void ntt_pass_a256(TowerMap map, const PrimeCo
       for (int k = 0; k < 16; ++k) {
         if (k & h) continue;
         const tw2 w = tw[(8u / h) + (uint32_t)(k / (2 * h))];
-        gs_bfly<Lazy>(v[k], v[k + h], w, m);
+        gs_bfly<A>(v[k], v[k + h], w, m);
       }
     }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) gs_bfly_last<Lazy>(v[k], v[k + 8], pc, m);
+    for (int k = 0; k < 8; ++k) gs_bfly_last<A>(v[k], v[k + 8], pc, m);
 #pragma unroll
     for (int k = 0; k < 16; ++k) data[(uint64_t)(u + 16 * k) * kRow + c] = v[k];
   }
@@ -199,9 +227,9 @@ __global__ __launch_bounds__(256) void ntt_pass_a256(TowerMap map, const PrimeCo
 // Epi (forward only): the ModDown correction epilogue, applied after the
 // exchange back to the coalesced layout (measured faster than loading the
 // subtrahend up front at the uncoalesced positions).
-template <bool Inverse, bool Lazy, bool Epi>
+template <bool Inverse, int A, bool Epi>
 __global__ __launch_bounds__(256) void ntt_pass_b(TowerMap map, const PrimeConst *__restrict__ primes, uint32_t R) {
-  __shared__ uint64_t lds[16 * kLdsRow];
+  __shared__ Val<A> lds[16 * kLdsRow];
   uint64_t *data;
   const uint64_t *src;
   uint32_t prime, tk, tpoly;
@@ -214,17 +242,17 @@ __global__ __launch_bounds__(256) void ntt_pass_b(TowerMap map, const PrimeConst
   uint64_t *row = data + (uint64_t)r * kRow;
   const uint64_t *src_row = src + (uint64_t)r * kRow;
   const uint32_t rf = R + r;
-  uint64_t v[16];
+  Val<A> v[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) v[k] = src_row[u + 16 * k];
   if constexpr (Epi) {
-    reg_stages<false, Lazy>(v, tw, 1, rf, 0, m);
+    reg_stages<false, A>(v, tw, 1, rf, 0, m);
 #pragma unroll
     for (int k = 0; k < 16; ++k) lds[lds_idx(rl, u + 16 * k)] = v[k];
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = lds[lds_idx(rl, 16 * u + k)];
-    reg_stages<false, Lazy>(v, tw, 16, rf, u, m);
+    reg_stages<false, A>(v, tw, 16, rf, u, m);
 #pragma unroll
     for (int k = 0; k < 16; ++k) lds[lds_idx(rl, 16 * u + k)] = v[k];
     __syncthreads();
@@ -238,27 +266,27 @@ __global__ __launch_bounds__(256) void ntt_pass_b(TowerMap map, const PrimeConst
     if (acc) {
 #pragma unroll
       for (int k = 0; k < 16; ++k) {
-        const uint64_t d = Lazy ? sub[u + 16 * k] + m.q4 - sub_if_ge_neg(v[k], m.nq4) : sub_mod(sub[u + 16 * k], v[k], pc.q);
+        const uint64_t d = epi_diff<A>(sub[u + 16 * k], v[k], m, pc.q);
         out[u + 16 * k] = add_mod(out[u + 16 * k], shoup(d, c.x, c.y, pc.q), pc.q);
       }
       return;
     }
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-      const uint64_t d = Lazy ? sub[u + 16 * k] + m.q4 - sub_if_ge_neg(v[k], m.nq4) : sub_mod(sub[u + 16 * k], v[k], pc.q);
+      const uint64_t d = epi_diff<A>(sub[u + 16 * k], v[k], m, pc.q);
       out[u + 16 * k] = shoup(d, c.x, c.y, pc.q);
     }
     return;
   } else if constexpr (!Inverse) {
-    reg_stages<false, Lazy>(v, tw, 1, rf, 0, m);
+    reg_stages<false, A>(v, tw, 1, rf, 0, m);
 #pragma unroll
     for (int k = 0; k < 16; ++k) lds[lds_idx(rl, u + 16 * k)] = v[k];
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = lds[lds_idx(rl, 16 * u + k)];
-    reg_stages<false, Lazy>(v, tw, 16, rf, u, m);
+    reg_stages<false, A>(v, tw, 16, rf, u, m);
 #pragma unroll
-    for (int k = 0; k < 16; ++k) v[k] = canon<Lazy>(v[k], m);
+    for (int k = 0; k < 16; ++k) v[k] = canon<A>(v[k], m);
 #pragma unroll
     for (int k = 0; k < 16; ++k) lds[lds_idx(rl, 16 * u + k)] = v[k];
     __syncthreads();
@@ -270,13 +298,13 @@ __global__ __launch_bounds__(256) void ntt_pass_b(TowerMap map, const PrimeConst
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = lds[lds_idx(rl, 16 * u + k)];
-    reg_stages<true, Lazy>(v, tw, 16, rf, u, m);
+    reg_stages<true, A>(v, tw, 16, rf, u, m);
 #pragma unroll
     for (int k = 0; k < 16; ++k) lds[lds_idx(rl, 16 * u + k)] = v[k];
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = lds[lds_idx(rl, u + 16 * k)];
-    reg_stages<true, Lazy>(v, tw, 1, rf, 0, m);
+    reg_stages<true, A>(v, tw, 1, rf, 0, m);
   }
 #pragma unroll
   for (int k = 0; k < 16; ++k) row[u + 16 * k] = v[k];
@@ -289,11 +317,11 @@ __global__ __launch_bounds__(256) void ntt_pass_b(TowerMap map, const PrimeConst
 // (stride 1, one sub-transform) and for pass A when R < 256 (stride 256,
 // 4096/R columns per workgroup). Forward runs ml = 1..L/2, inverse the mirror
 // with n^-1 folded into the last stage when `last` is set.
-template <bool Inverse, bool Lazy>
+template <bool Inverse, int A>
 __global__ __launch_bounds__(256) void ntt_lds_generic(TowerMap map, const PrimeConst *__restrict__ primes,
                                                        uint32_t log_len, uint32_t stride, uint32_t cols_per_block,
                                                        uint32_t finish, uint32_t first_pass) {
-  __shared__ uint64_t lds[4096];
+  __shared__ Val<A> lds[4096];
   uint64_t *data;
   const uint64_t *src;
   uint32_t prime, tk, tpoly;
@@ -320,12 +348,12 @@ __global__ __launch_bounds__(256) void ntt_lds_generic(TowerMap map, const Prime
       const uint32_t col = p / (L / 2), pp = p % (L / 2);
       const uint32_t g = pp >> lg_d;
       const uint32_t j = pp + (g << lg_d);
-      uint64_t x = lds[col * L + j], y = lds[col * L + j + (1u << lg_d)];
+      Val<A> x = lds[col * L + j], y = lds[col * L + j + (1u << lg_d)];
       if constexpr (Inverse) {
-        if (last) gs_bfly_last<Lazy>(x, y, pc, m);
-        else gs_bfly<Lazy>(x, y, tw[ml + g], m);
+        if (last) gs_bfly_last<A>(x, y, pc, m);
+        else gs_bfly<A>(x, y, tw[ml + g], m);
       } else {
-        ct_bfly<Lazy>(x, y, tw[ml + g], m);
+        ct_bfly<A>(x, y, tw[ml + g], m);
       }
       lds[col * L + j] = x;
       lds[col * L + j + (1u << lg_d)] = y;
@@ -344,8 +372,9 @@ __global__ __launch_bounds__(256) void ntt_lds_generic(TowerMap map, const Prime
   for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
     const uint32_t col = e % cols_per_block, j = e / cols_per_block;
     const uint64_t at = (uint64_t)j * stride + col0 + col;
-    uint64_t x = lds[col * L + j];
-    if (!Inverse && finish) x = canon<Lazy>(x, m);
+    Val<A> xv = lds[col * L + j];
+    if (!Inverse && finish) xv = canon<A>(xv, m);
+    uint64_t x = xv;
     if (epilogue) {
       x = shoup(sub_mod(sub[at], x, pc.q), c.x, c.y, pc.q);
       if (acc) x = add_mod(out[at], x, pc.q);
@@ -355,30 +384,30 @@ __global__ __launch_bounds__(256) void ntt_lds_generic(TowerMap map, const Prime
 }
 
 // ---------------------------------------------------------------- launcher
-template <bool Inverse, bool Lazy>
+template <bool Inverse, int A>
 static hipError_t launch_t(const TowerMap &map, const PrimeConst *primes, uint32_t log_n, uint32_t towers,
                            hipStream_t stream) {
   const uint32_t n = 1u << log_n;
   if (log_n <= 12) {
-    hipLaunchKernelGGL((ntt_lds_generic<Inverse, Lazy>), dim3(1, towers), dim3(256), 0, stream, map, primes, log_n,
+    hipLaunchKernelGGL((ntt_lds_generic<Inverse, A>), dim3(1, towers), dim3(256), 0, stream, map, primes, log_n,
                        1u, 1u, 1u, 1u);
     return hipGetLastError();
   }
   const uint32_t R = n / kRow, log_r = log_n - 8;
   auto pass_a = [&]() {
     if (R == 256) {
-      hipLaunchKernelGGL((ntt_pass_a256<Inverse, Lazy>), dim3(kRow / 16, towers), dim3(256), 0, stream, map, primes);
+      hipLaunchKernelGGL((ntt_pass_a256<Inverse, A>), dim3(kRow / 16, towers), dim3(256), 0, stream, map, primes);
     } else {
       const uint32_t cols = 4096 / R;
-      hipLaunchKernelGGL((ntt_lds_generic<Inverse, Lazy>), dim3(kRow / cols, towers), dim3(256), 0, stream, map,
+      hipLaunchKernelGGL((ntt_lds_generic<Inverse, A>), dim3(kRow / cols, towers), dim3(256), 0, stream, map,
                          primes, log_r, (uint32_t)kRow, cols, (uint32_t)Inverse, (uint32_t)!Inverse);
     }
   };
   auto pass_b = [&]() {
     if (!Inverse && map.epi.sub)
-      hipLaunchKernelGGL((ntt_pass_b<false, Lazy, true>), dim3(R / 16, towers), dim3(256), 0, stream, map, primes, R);
+      hipLaunchKernelGGL((ntt_pass_b<false, A, true>), dim3(R / 16, towers), dim3(256), 0, stream, map, primes, R);
     else
-      hipLaunchKernelGGL((ntt_pass_b<Inverse, Lazy, false>), dim3(R / 16, towers), dim3(256), 0, stream, map, primes,
+      hipLaunchKernelGGL((ntt_pass_b<Inverse, A, false>), dim3(R / 16, towers), dim3(256), 0, stream, map, primes,
                          R);
   };
   if (!Inverse) {
@@ -395,7 +424,7 @@ static hipError_t launch_t(const TowerMap &map, const PrimeConst *primes, uint32
 // one group before the next group), so a group's intermediate hand-off stays
 // in L2 / Infinity Cache instead of a full HBM round trip.
 hipError_t launch_ntt(const TowerMap &map, const PrimeConst *primes, uint32_t log_n, uint32_t towers, bool inverse,
-                      bool lazy, hipStream_t stream, uint32_t chunk) {
+                      int arith, hipStream_t stream, uint32_t chunk) {
   if (towers == 0) return hipSuccess;
   if (map.towers_per_poly == 0 || towers % map.towers_per_poly != 0) return hipErrorInvalidValue;
   const uint32_t step = chunk ? chunk : towers;
@@ -405,10 +434,15 @@ hipError_t launch_ntt(const TowerMap &map, const PrimeConst *primes, uint32_t lo
     m.y_offset = map.y_offset + first;
     const uint32_t count = towers - first < step ? towers - first : step;
     hipError_t e;
-    if (inverse) e = lazy ? launch_t<true, true>(m, primes, log_n, count, stream)
-                          : launch_t<true, false>(m, primes, log_n, count, stream);
-    else e = lazy ? launch_t<false, true>(m, primes, log_n, count, stream)
-                  : launch_t<false, false>(m, primes, log_n, count, stream);
+    switch (arith * 2 + (inverse ? 1 : 0)) {
+      case kLazy * 2: e = launch_t<false, kLazy>(m, primes, log_n, count, stream); break;
+      case kLazy * 2 + 1: e = launch_t<true, kLazy>(m, primes, log_n, count, stream); break;
+      case kNarrow * 2: e = launch_t<false, kNarrow>(m, primes, log_n, count, stream); break;
+      case kNarrow * 2 + 1: e = launch_t<true, kNarrow>(m, primes, log_n, count, stream); break;
+      case kExact * 2: e = launch_t<false, kExact>(m, primes, log_n, count, stream); break;
+      case kExact * 2 + 1: e = launch_t<true, kExact>(m, primes, log_n, count, stream); break;
+      default: return hipErrorInvalidValue;
+    }
     if (e != hipSuccess) return e;
   }
   return hipSuccess;

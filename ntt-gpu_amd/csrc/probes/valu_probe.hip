@@ -2,7 +2,8 @@
 // built from, and the NTT's own wide-lazy butterflies (modarith.hpp ct8/gs8)
 // in dependent chains with enough independent chains per thread and waves per
 // CU to saturate issue. The butterfly rates are the VALU roof of the NTT:
-// a transform of N = 2^16 points is 16 stages x 32768 butterflies.
+// a transform of N = 2^16 points is 16 stages x 32768 butterflies. The
+// 32-bit forms (ct32/gs32) are the roof of towers below 2^30.
 //
 // Usage: valu_probe [--json]   (one line per probe; exits non-zero on a HIP error)
 #include <hip/hip_runtime.h>
@@ -87,6 +88,31 @@ __global__ void p_gs8(uint64_t *out, uint64_t q, uint64_t w, uint64_t wp, uint64
   out[blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
+// narrow towers (q < 2^30): 32-bit butterflies, forward residues in [0, 4q), inverse in [0, 2q)
+__global__ void p_ct32(uint64_t *out, uint32_t q, uint32_t w, uint32_t wp) {
+  const Narrow m = narrow_of(q);
+  uint32_t x[kChains], y[kChains];
+  for (int i = 0; i < kChains; ++i) x[i] = (threadIdx.x * 131 + i) % q, y[i] = (threadIdx.x * 17 + 5 * i) % q;
+  for (int it = 0; it < kIter / 4; ++it)
+#pragma unroll
+    for (int i = 0; i < kChains; ++i) ct32(x[i], y[i], w, wp, m);
+  uint64_t s = 0;
+  for (int i = 0; i < kChains; ++i) s ^= x[i] ^ y[i];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+__global__ void p_gs32(uint64_t *out, uint32_t q, uint32_t w, uint32_t wp) {
+  const Narrow m = narrow_of(q);
+  uint32_t x[kChains], y[kChains];
+  for (int i = 0; i < kChains; ++i) x[i] = (threadIdx.x * 131 + i) % q, y[i] = (threadIdx.x * 17 + 5 * i) % q;
+  for (int it = 0; it < kIter / 4; ++it)
+#pragma unroll
+    for (int i = 0; i < kChains; ++i) gs32(x[i], y[i], w, wp, m);
+  uint64_t s = 0;
+  for (int i = 0; i < kChains; ++i) s ^= x[i] ^ y[i];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
 template <class Launch>
 float time_ms(Launch launch) {
   hipEvent_t a, b;
@@ -115,6 +141,8 @@ int main(int argc, char **argv) {
   const uint64_t q = 1152921504606584833ull;  // 60-bit NTT prime (2^60 - 2^18 + 1 class)
   const uint64_t w = 123456789012345ull % q;
   const uint64_t wp = (uint64_t)(((unsigned __int128)w << 64) / q);
+  const uint32_t q32 = 1073479681u;  // 30-bit NTT prime (class 1 mod 2^17)
+  const uint32_t w32 = 123456789u % q32, wp32 = (uint32_t)(((uint64_t)w32 << 32) / q32);
   const double lanes = (double)blocks * threads;
   struct Row {
     const char *name, *unit;
@@ -129,6 +157,10 @@ int main(int argc, char **argv) {
       {"ct8_forward_butterfly", "butterfly", time_ms([&] { p_ct8<<<blocks, threads>>>(buf, q, w, wp, 0 - q, 0 - 2 * q, 0 - 4 * q); }),
        lanes * kIter / 4 * kChains},
       {"gs8_inverse_butterfly", "butterfly", time_ms([&] { p_gs8<<<blocks, threads>>>(buf, q, w, wp, 0 - q, 0 - 2 * q, 0 - 4 * q); }),
+       lanes * kIter / 4 * kChains},
+      {"ct32_forward_butterfly", "butterfly", time_ms([&] { p_ct32<<<blocks, threads>>>(buf, q32, w32, wp32); }),
+       lanes * kIter / 4 * kChains},
+      {"gs32_inverse_butterfly", "butterfly", time_ms([&] { p_gs32<<<blocks, threads>>>(buf, q32, w32, wp32); }),
        lanes * kIter / 4 * kChains},
   };
   PROBE_CHECK(hipGetLastError());

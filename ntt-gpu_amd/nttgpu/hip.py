@@ -29,6 +29,8 @@ SIGNATURES = {
     "ntt_hip_abi_version": [],
     "ntt_hip_device_count": [ctypes.POINTER(INT)],
     "ntt_hip_device_name": [INT, ctypes.c_char_p, SZ],
+    "ntt_hip_device_properties": [INT, ctypes.c_char_p, SZ, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(INT),
+                                  ctypes.POINTER(INT), ctypes.POINTER(INT)],
     "ntt_hip_malloc": [INT, ctypes.POINTER(VP), SZ],
     "ntt_hip_free": [INT, VP],
     "ntt_hip_host_alloc": [ctypes.POINTER(VP), SZ],

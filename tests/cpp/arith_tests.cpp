@@ -107,6 +107,59 @@ int main() {
     const U128 got = combine_limbs(p[0], p[1], p[2], p[3], p[4], p[5]);
     check(got.lo == (uint64_t)want && got.hi == (uint64_t)(want >> 64), "limb-split dot product is exact");
   }
+  // Barrett variable x variable products: lazy form below 2^62, exact form below 2^63
+  for (int i = 0; i < 3000000; ++i) {
+    uint64_t q = (rng() >> (1 + i % 40)) | 1;
+    if (i % 11 == 0) q = (uint64_t{1} << 63) - 1 - 2 * (rng() & 0xffff);
+    if (i % 13 == 0) q = (uint64_t{1} << 62) - 1 - 2 * (rng() & 0xffff);
+    if (q < 3) q = 3;
+    const u128 mu = ~(u128)0 / q;
+    const uint64_t ml = (uint64_t)mu, mh = (uint64_t)(mu >> 64);
+    const uint64_t a = (i % 17 == 0) ? q - 1 : rng() % q, b = (i % 19 == 0) ? q - 1 : rng() % q;
+    const uint64_t want = (uint64_t)(((u128)a * b) % q);
+    check(barrett_mul_exact(a, b, q, ml, mh) == want, "barrett_mul_exact");
+    if (q < (uint64_t{1} << 62)) check(barrett_mul_lazy(a, b, q, ml, mh) == want, "barrett_mul_lazy");
+    check(barrett_mul(a, b, q, ml, mh) == want, "barrett_mul");
+  }
+  // narrow towers (q < 2^30): 32-bit Shoup with the high word of the 64-bit
+  // companion, Harvey ranges of ct32 / gs32, canonicalisation
+  const uint32_t narrow_primes[] = {17, 7681, 12289, 786433, 998244353, 1073479681, 1073741789, 1073707009};
+  for (int i = 0; i < 3000000; ++i) {
+    const uint32_t q = (i % 4 == 0) ? narrow_primes[i % 8] : (uint32_t)((rng() >> 34) | 1) % ((1u << 30) - 3) + 3;
+    const uint32_t w = (uint32_t)(rng() % q);
+    const uint64_t wp64 = (uint64_t)(((u128)w << 64) / q);
+    const uint32_t wp = (uint32_t)(wp64 >> 32);
+    check(wp == (uint32_t)(((uint64_t)w << 32) / q), "narrow companion = high word of the 64-bit companion");
+    const Narrow m = narrow_of(q);
+    const uint32_t y = (i % 3 == 0) ? (uint32_t)rng() : (uint32_t)(rng() % (4ull * q));  // any 32-bit input
+    const uint32_t t = shoup32(y, w, wp, q);
+    check(t < 2 * q && t % q == (uint32_t)(((uint64_t)y * w) % q), "shoup32 in [0, 2q) and congruent");
+    const uint32_t fx = (uint32_t)(rng() % (4ull * q)), fy = (uint32_t)(rng() % (4ull * q));
+    uint32_t cx = fx, cy = fy;
+    ct32(cx, cy, w, wp, m);
+    const uint64_t prod = ((uint64_t)(fy % q) * w) % q;
+    check(cx < 4 * q && cy < 4 * q, "ct32 outputs below 4q");
+    check(cx % q == (fx % q + prod) % q && cy % q == (fx % q + q - prod) % q, "ct32 congruent");
+    const uint32_t ix = (uint32_t)(rng() % (2ull * q)), iy = (uint32_t)(rng() % (2ull * q));
+    uint32_t gx = ix, gy = iy;
+    gs32(gx, gy, w, wp, m);
+    check(gx < 2 * q && gy < 2 * q, "gs32 outputs below 2q");
+    check(gx % q == (ix % q + iy % q) % q && gy % q == ((uint64_t)((ix % q + q - iy % q) % q) * w) % q,
+          "gs32 congruent");
+    check(canon4_32(fx, m) == fx % q, "canon4_32");
+    const uint32_t c = (uint32_t)(rng() >> 33) | 1, x = (uint32_t)(rng() % (2ull * c));
+    check(min_sub(x, c) == (x >= c ? x - c : x), "min_sub");
+  }
+  for (const uint32_t q : narrow_primes)
+    for (const uint32_t x : {0u, q - 1, q, 2 * q - 1, 4 * q - 1}) {
+      check(canon4_32(x, narrow_of(q)) == x % q, "canon4_32 at the edges");
+      for (const uint32_t w : {0u, 1u, q - 1}) {
+        const uint32_t wp = (uint32_t)(((uint64_t)w << 32) / q);
+        check(shoup32(x, w, wp, q) % q == (uint32_t)(((uint64_t)x * w) % q) && shoup32(x, w, wp, q) < 2 * q,
+              "shoup32 at the edges");
+        check(shoup32(0xffffffffu, w, wp, q) < 2 * q, "shoup32 on the largest input");
+      }
+    }
   std::printf("%zu ARITH CHECKS PASSED\n", checks);
   return 0;
 }

@@ -199,6 +199,17 @@ def test_keyswitch_wide_lazy_boundary_primes():
     _check_keyswitch(hp, 3, 2, 660)
 
 
+@pytest.mark.parametrize("n,L,batch", [(4096, 6, 2), (4096, 4, 2), (65536, 6, 1)])
+def test_keyswitch_narrow_primes(n, L, batch):
+    """Every modulus below 2^30 (q0 = LastPrime(30), scaling primes near 2^28,
+    P below q0): the NTTs run the 32-bit butterflies, including the ModDown
+    correction fused into the last forward pass; bit-exact vs the oracle."""
+    hp = params.openfhe_like(n, 6, 2, 2, first_bits=30, scale_bits=28)
+    assert max(hp.q + hp.p) < 1 << 30
+    log_case(f"keyswitch narrow primes n={n} L={L} batch={batch}")
+    _check_keyswitch(hp, L, batch, 7100 + L)
+
+
 def test_keyswitch_north_star_shape():
     """N=2^16, L=24 (~50-bit, first 60-bit), K=6 (60-bit), dnum=4 (alpha=6), batch of 2."""
     hp = params.openfhe_like(65536, 24, 6, 6)

@@ -127,6 +127,49 @@ def test_wide_lazy_boundary_primes(n):
         assert np.array_equal(_gpu_round(ctx, x, 0, 2, True), o.ntt_towers(x, n, qs, rs, inverse=True))
 
 
+@pytest.mark.parametrize("n", [2, 256, 4096, 16384, 65536])
+def test_narrow_primes(n):
+    """Towers below 2^30 run the 32-bit butterflies (every launch shape: whole
+    transform in LDS, column pass for R < 256, the R = 256 column pass): the
+    largest NTT prime below 2^30 (4q just under 2^32), 30-bit primes from the
+    reference generator and the smallest class member, with all-(q-1) and
+    seeded inputs, both directions."""
+    m = 2 * max(n, 2)
+    qs = [params.last_prime(30, m)] + params.find_primes(max(n, 2), 2, 30)
+    small = next(q for q in range(m + 1, 1 << 30, m) if params.is_prime(q))
+    qs.append(small)
+    rs = [params.find_psi(q, n) for q in qs]
+    ctx = hip.Context(n, qs, rs)
+    log_case(f"ntt narrow primes n={n} q={qs}")
+    T = len(qs)
+    for x in (np.concatenate([np.full(n, q - 1, np.uint64) for q in qs]), uniform(n, qs, 30)):
+        f = _gpu_round(ctx, x, 0, T, False)
+        assert np.array_equal(f, o.ntt_towers(x, n, qs, rs))
+        assert np.array_equal(_gpu_round(ctx, f, 0, T, True), x)
+        assert np.array_equal(_gpu_round(ctx, x, 0, T, True), o.ntt_towers(x, n, qs, rs, inverse=True))
+
+
+def test_pointwise_every_width():
+    """Barrett variable x variable products (lazy form below 2^62, exact form
+    up to 2^63) on primes of 30, 50, 60, 61.9, 62.5 and ~63 bits: all-(q-1),
+    ones and seeded operands against the oracle's 128-bit remainder."""
+    n = 4096
+    qs = [params.last_prime(30, 2 * n), params.last_prime(50, 2 * n), params.last_prime(60, 2 * n),
+          params.last_prime(62, 2 * n), params.find_primes(n, 1, 63)[0], params.last_prime(63, 2 * n)]
+    rs = [params.find_psi(q, n) for q in qs]
+    ctx = hip.Context(n, qs, rs)
+    log_case(f"pointwise widths q={qs}")
+    T = len(qs)
+    cases = [(np.concatenate([np.full(n, q - 1, np.uint64) for q in qs]),) * 2,
+             (np.concatenate([np.full(n, q - 1, np.uint64) for q in qs]), np.ones(T * n, np.uint64)),
+             (uniform(n, qs, 3), uniform(n, qs, 4))]
+    for x, y in cases:
+        a, b = hip.DeviceBuffer(x.size).upload(x), hip.DeviceBuffer(y.size).upload(y)
+        ctx.pointwise(a.ptr, b.ptr, 0, T)
+        ctx.synchronize()
+        assert np.array_equal(a.download(), o.pointwise_towers(x, y, n, qs))
+
+
 def test_out_of_place_and_subrange():
     qs = params.mixed_width_primes(N16, 6)
     rs = [params.min_root(q, N16) for q in qs]

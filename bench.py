@@ -337,6 +337,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--roofline-only", action="store_true")
+    ap.add_argument("--roofline-bits", type=int, default=59)
     ap.add_argument("--serve-rates", type=str, default="200,1000")
     ap.add_argument("--serve-device-rates", type=str, default="1000,5000,20000")
     ap.add_argument("--rotations", type=int, default=8)
@@ -354,7 +355,7 @@ def main():
         # For counter runs: the NTT roofline launches plus a plain copy kernel
         # of known size (1 GiB read + 1 GiB written) to calibrate the counters.
         s = hip.Stream(0)
-        roof = ntt_roofline(args.ntt_towers, 4, s.handle)
+        roof = ntt_roofline(args.ntt_towers, 4, s.handle, bits=args.roofline_bits)
         words = 1 << 27
         a, b = hip.DeviceBuffer(words), hip.DeviceBuffer(words)
         e0, e1 = hip.Event(), hip.Event()

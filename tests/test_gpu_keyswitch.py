@@ -210,6 +210,22 @@ def test_keyswitch_narrow_primes(n, L, batch):
     _check_keyswitch(hp, L, batch, 7100 + L)
 
 
+@pytest.mark.parametrize("n,bits,L,K,alpha,batch", [
+    (8192, 63, 4, 2, 1, 2),    # canonical arithmetic, 4 digits
+    (16384, 60, 5, 2, 2, 2),   # wide-lazy, shortened last digit
+    (32768, 50, 12, 1, 1, 2),  # 12 single-tower digits
+    (8192, 30, 6, 2, 3, 3),    # 32-bit butterflies
+])
+def test_keyswitch_ring_sizes_and_widths(n, bits, L, K, alpha, batch):
+    """Rings 2^13..2^15 (two-pass transforms with a column pass of R < 256
+    rows) in every NTT arithmetic mode and with up to 12 digits, bit-exact vs
+    the oracle."""
+    qs = params.find_primes(n, L + K, bits)
+    hp = params.HybridParams(n=n, q=qs[:L], p=qs[L:], alpha=alpha)
+    log_case(f"keyswitch n={n} {bits}-bit L={L} K={K} alpha={alpha} batch={batch}")
+    _check_keyswitch(hp, L, batch, 9000 + n % 97 + L)
+
+
 def test_keyswitch_north_star_shape():
     """N=2^16, L=24 (~50-bit, first 60-bit), K=6 (60-bit), dnum=4 (alpha=6), batch of 2."""
     hp = params.openfhe_like(65536, 24, 6, 6)

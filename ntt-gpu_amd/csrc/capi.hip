@@ -382,6 +382,36 @@ struct ntt_hip_keyswitch {
   std::unique_ptr<DeviceBlob> key;                // beta*(L+K)*n * 2 (b then a), Montgomery form
   bool key_loaded = false;
   bool mac_per_term = false;
+  // Batches split into `lanes` sub-batches on concurrent streams (fork/join
+  // with events), so one lane's VALU-bound conversion overlaps another's
+  // memory-bound transform passes. Streams and events are created on first use.
+  uint32_t lanes = kDefaultLanes;
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> joins;
+  hipEvent_t fork = nullptr;
+  static constexpr uint32_t kDefaultLanes = 2;
+  static constexpr size_t kMinLaneBatch = 4;  // smaller sub-batches are not split
+
+  ~ntt_hip_keyswitch() {
+    try {
+      DeviceGuard g(ctx->device);
+      for (hipStream_t st : side) (void)hipStreamDestroy(st);
+      for (hipEvent_t e : joins) (void)hipEventDestroy(e);
+      if (fork) (void)hipEventDestroy(fork);
+    } catch (...) {
+    }
+  }
+  void ensure_lanes(uint32_t count) {
+    if (!fork) HIP_CHECK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    while (side.size() + 1 < count) {
+      hipStream_t st = nullptr;
+      HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));  // default priority
+      side.push_back(st);
+      hipEvent_t e = nullptr;
+      HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      joins.push_back(e);
+    }
+  }
 
   size_t n() const { return ctx->n; }
   // workspace layout (elements per component)
@@ -1017,11 +1047,36 @@ int ntt_hip_keyswitch_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batc
     require(ks->key_loaded, "evaluation key not uploaded");
     require(batch >= 1 && batch <= ks->max_batch, "batch exceeds the plan capacity");
     const hipStream_t s = ks->ctx->pick(stream);
-    const KsBuffers w = ks_buffers(*ks, workspace, batch);
     DeviceGuard g(ks->ctx->device);
-    ks_modup(*ks, in, batch, w, s);
-    ks_mac(*ks, *ks, in, batch, w, w.z, s);
-    ks_moddown(*ks, w.z, batch, out_b, out_a, w, s);
+    auto chain = [&](size_t first, size_t count, hipStream_t st) {
+      const size_t off = first * ks->L * ks->n();
+      const KsBuffers w = ks_buffers(*ks, (uint64_t *)workspace + first * ks->per_component(), count);
+      ks_modup(*ks, in + off, count, w, st);
+      ks_mac(*ks, *ks, in + off, count, w, w.z, st);
+      ks_moddown(*ks, w.z, count, out_b + off, out_a + off, w, st);
+    };
+    const uint32_t lanes = (uint32_t)std::max<size_t>(1, std::min<size_t>(ks->lanes, batch / ntt_hip_keyswitch::kMinLaneBatch));
+    if (lanes <= 1) {
+      chain(0, batch, s);
+      return NTT_HIP_OK;
+    }
+    // lane j runs components [first_j, first_j + count_j) in its own part of
+    // the workspace; lane 0 on the caller's stream, the others forked from it
+    // and joined back, so the caller's stream (or its capture) orders the whole batch
+    ks->ensure_lanes(lanes);
+    HIP_CHECK(hipEventRecord(ks->fork, s));
+    size_t first = 0;
+    for (uint32_t j = 0; j < lanes; ++j) {
+      const size_t count = batch / lanes + (j < batch % lanes ? 1 : 0);
+      const hipStream_t st = j == 0 ? s : ks->side[j - 1];
+      if (j) HIP_CHECK(hipStreamWaitEvent(st, ks->fork, 0));
+      chain(first, count, st);
+      first += count;
+    }
+    for (uint32_t j = 1; j < lanes; ++j) {
+      HIP_CHECK(hipEventRecord(ks->joins[j - 1], ks->side[j - 1]));
+      HIP_CHECK(hipStreamWaitEvent(s, ks->joins[j - 1], 0));
+    }
     return NTT_HIP_OK;
   });
 }

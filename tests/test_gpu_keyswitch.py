@@ -269,6 +269,45 @@ def test_keyswitch_batch_equals_alone():
         assert np.array_equal(oa.download(), ga[c * L * n:(c + 1) * L * n])
 
 
+def test_keyswitch_lanes_and_graph_replays():
+    """A batch of 11 runs as two concurrent lanes (6 + 5 components on two
+    streams, forked from and joined back to the caller's stream): equal to each
+    component switched alone, eagerly and through a captured graph replayed
+    three times with eager work and synchronizations between the replays."""
+    hp = params.openfhe_like(4096, 8, 3, 3)
+    L, n, B = 8, 4096, 11
+    log_case("keyswitch lanes + graph n=4096 L=8 batch=11")
+    x, kb, ka, gb, ga = _keyswitch(hp, L, B, 9500)
+    ctx = hip.Context(n, hp.q + hp.p, hp.q_psi + hp.p_psi)
+    one = hip.KeySwitch(ctx, L, hp.K, hp.alpha, 1)
+    one.upload_key(kb, ka, hp.M, hp.q)
+    ws1 = hip.DeviceBuffer(one.workspace_words(1))
+    for c in (0, 5, 6, 10):  # both ends of both lanes
+        sl = slice(c * L * n, (c + 1) * L * n)
+        di, ob, oa = _dev(x[sl]), hip.DeviceBuffer(L * n), hip.DeviceBuffer(L * n)
+        one.run(di.ptr, 1, ob.ptr, oa.ptr, ws1.ptr)
+        ctx.synchronize()
+        assert np.array_equal(ob.download(), gb[sl]) and np.array_equal(oa.download(), ga[sl]), f"component {c}"
+    wb, wa = o.key_switch(x[:L * n], n, hp.q[:L], hp.q_psi[:L], hp.M, hp.p, hp.p_psi, hp.alpha, kb, ka)
+    assert np.array_equal(gb[:L * n], wb) and np.array_equal(ga[:L * n], wa)
+    ks = hip.KeySwitch(ctx, L, hp.K, hp.alpha, B)
+    ks.upload_key(kb, ka, hp.M, hp.q)
+    d_in, d_b, d_a = _dev(x), hip.DeviceBuffer(B * L * n), hip.DeviceBuffer(B * L * n)
+    ws = hip.DeviceBuffer(ks.workspace_words(B))
+    st = hip.Stream(0)
+    graph = ks.capture(d_in.ptr, B, d_b.ptr, d_a.ptr, ws.ptr, st.handle)
+    zeros = np.zeros(B * L * n, np.uint64)
+    e_in, e_b, e_a = _dev(x[:L * n]), hip.DeviceBuffer(L * n), hip.DeviceBuffer(L * n)
+    for replay in range(3):
+        d_b.upload(zeros)
+        d_a.upload(zeros)
+        one.run(e_in.ptr, 1, e_b.ptr, e_a.ptr, ws1.ptr)  # eager work between replays
+        ctx.synchronize()
+        graph.launch(st.handle)
+        st.synchronize()
+        assert np.array_equal(d_b.download(), gb) and np.array_equal(d_a.download(), ga), f"replay {replay}"
+
+
 @pytest.mark.parametrize("evaluation", [True, False])
 def test_batched_modup_moddown_config_c3(evaluation):
     """N=2^16, Q: 24 towers (60 + 50-bit), P: 6 towers (60-bit): batched ModUp

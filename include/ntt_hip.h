@@ -178,7 +178,8 @@ int ntt_hip_graph_destroy(void *graph);
  * All components are batch * L * n evaluation residues on the device (the
  * caller converts coefficient-form components with the forward NTT first, as
  * the adapter's SetFormat(EVALUATION) does). The additions are fused into the
- * ModDown's last pass. Workspace: ntt_hip_keyswitch_workspace_bytes. */
+ * ModDown's last pass. Workspace: ntt_hip_keyswitch_workspace_bytes. The
+ * components must be distinct buffers (NTT_HIP_EINVAL otherwise). */
 int ntt_hip_keyswitch_in_place(ntt_hip_keyswitch *ks, uint64_t *c0, uint64_t *c1, const uint64_t *c2, size_t batch,
                                void *workspace, void *stream);
 /* Relinearization (reference openfhe_hybrid_adapter.hpp:250-286

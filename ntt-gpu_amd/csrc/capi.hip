@@ -1085,6 +1085,7 @@ int ntt_hip_keyswitch_in_place(ntt_hip_keyswitch *ks, uint64_t *c0, uint64_t *c1
                                void *workspace, void *stream) {
   return guarded([&] {
     require(ks && c0 && c1 && workspace, "null argument");
+    require(c0 != c1 && (!c2 || (c2 != c0 && c2 != c1)), "ciphertext components must not alias");
     require(ks->key_loaded, "evaluation key not uploaded");
     require(batch >= 1 && batch <= ks->max_batch, "batch exceeds the plan capacity");
     const hipStream_t s = ks->ctx->pick(stream);
@@ -1104,6 +1105,7 @@ int ntt_hip_relinearize(ntt_hip_keyswitch *const *keys, size_t extra, uint64_t *
                         const uint64_t *const *higher, size_t batch, void *workspace, void *stream) {
   return guarded([&] {
     require(c0 && c1 && workspace && (extra == 0 || (keys && higher)), "null argument");
+    require(c0 != c1, "ciphertext components must not alias");
     if (extra == 0) return NTT_HIP_OK;
     const ntt_hip_keyswitch &plan = *keys[0];
     require(batch >= 1 && batch <= plan.max_batch, "batch exceeds the plan capacity");
@@ -1113,6 +1115,7 @@ int ntt_hip_relinearize(ntt_hip_keyswitch *const *keys, size_t extra, uint64_t *
               "relinearization keys must share the context and the (L, K, alpha) plan");
       require(k->key_loaded, "relinearization key not uploaded");
       require(higher[j], "null higher component");
+      require(higher[j] != c0 && higher[j] != c1, "ciphertext components must not alias");
     }
     const hipStream_t s = plan.ctx->pick(stream);
     const KsBuffers w = ks_buffers(plan, workspace, batch);

@@ -504,11 +504,19 @@ def test_relinearize_rejections():
     ks.upload_key(kb, ka, hp.M, hp.q)
     d = hip.DeviceBuffer(6 * hp.n)
     ws = hip.DeviceBuffer(ks.workspace_words(1))
+    d2 = hip.DeviceBuffer(6 * hp.n)
     with pytest.raises(hip.ContractError, match="not uploaded"):
-        other.in_place(d.ptr, d.ptr, 1, ws.ptr)
+        other.in_place(d.ptr, d2.ptr, 1, ws.ptr)
+    with pytest.raises(hip.ContractError, match="alias"):
+        ks.in_place(d.ptr, d.ptr, 1, ws.ptr)
+    with pytest.raises(hip.ContractError, match="alias"):
+        ks.in_place(d.ptr, d2.ptr, 1, ws.ptr, c2_ptr=d2.ptr)
+    with pytest.raises(hip.ContractError, match="alias"):
+        hip.KeySwitch.relinearize([ks], d.ptr, d2.ptr, [d2.ptr], 1, ws.ptr)
+    d3 = hip.DeviceBuffer(6 * hp.n)
     with pytest.raises(hip.ContractError, match="share the context"):
-        hip.KeySwitch.relinearize([ks, other], d.ptr, d.ptr, [d.ptr, d.ptr], 1, ws.ptr)
+        hip.KeySwitch.relinearize([ks, other], d.ptr, d2.ptr, [d3.ptr, d3.ptr], 1, ws.ptr)
     with pytest.raises(hip.ContractError, match="capacity"):
-        ks.in_place(d.ptr, d.ptr, 2, ws.ptr)
+        ks.in_place(d.ptr, d2.ptr, 2, ws.ptr)
     with pytest.raises(hip.ContractError, match="one relinearization key"):
-        hip.KeySwitch.relinearize([ks], d.ptr, d.ptr, [], 1, ws.ptr)
+        hip.KeySwitch.relinearize([ks], d.ptr, d2.ptr, [], 1, ws.ptr)

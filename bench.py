@@ -304,6 +304,50 @@ def served_stream(requests, rates, device_rates, seed=0xC5):
             "mix": "40% KeySwitch, 30% NTT/INTT, 30% ModUp/ModDown; N in 2^13..2^16; L in [3,30]; dnum in {2,3,4}"}
 
 
+def served_keyswitch(hp, kb, ka, d_in, d_b, d_a, B, requests, rates, seed=0x5E):
+    """Continuous batching on one parameter set: Poisson arrivals of
+    device-resident KeySwitch requests (N=2^16, L=24, K=6, dnum=4; input i is
+    component i mod B of the batched run) into the server (max batch B, four
+    streams). Requests that arrive while a batch runs are gathered into the
+    next one; every sampled response equals the batched run's component."""
+    from nttgpu import serve
+    comp = L * N
+    srv = serve.Server(streams=4, max_batch=B, hbm_budget_bytes=96 << 30)
+    pid = srv.add_params(hp, kb, ka)
+    outs = [(hip.DeviceBuffer(comp), hip.DeviceBuffer(comp)) for _ in range(requests)]
+    for i in range(2):  # plan, key upload and arenas outside the timed streams
+        srv.wait_device(srv.submit_device(pid, serve.KEYSWITCH, L, d_in.at(0), outs[i][0].ptr, outs[i][1].ptr))
+    results = []
+    for rate in rates:
+        gaps = np.random.default_rng(seed + int(rate)).exponential(1.0 / rate, requests)
+        tickets, t0 = [], time.perf_counter()
+        due = t0
+        for i, g in enumerate(gaps):
+            due += g
+            while time.perf_counter() < due:
+                pass
+            tickets.append(srv.submit_device(pid, serve.KEYSWITCH, L, d_in.at((i % B) * comp), outs[i][0].ptr,
+                                             outs[i][1].ptr))
+        waited = [srv.wait_device(t) for t in tickets]
+        wall = time.perf_counter() - t0
+        sample = range(0, requests, max(1, requests // 8))
+        exact = all(np.array_equal(outs[i][0].download(), d_b.download(comp, (i % B) * comp)) and
+                    np.array_equal(outs[i][1].download(), d_a.download(comp, (i % B) * comp)) for i in sample)
+        lat = [w[0] for w in waited]
+        results.append({"offered_rate_per_s": rate, "requests": requests,
+                        "throughput_per_s": round(requests / wall, 1), "p50_ms": round(pct(lat, 0.5), 3),
+                        "p99_ms": round(pct(lat, 0.99), 3), "mean_batch": round(statistics.mean(w[1] for w in waited), 2),
+                        "sampled_equal_batched": exact})
+        progress(f"served keyswitch {rate}/s: {results[-1]['throughput_per_s']} KS/s, mean batch "
+                 f"{results[-1]['mean_batch']}, p50 {results[-1]['p50_ms']} ms, exact={exact}")
+    srv.close()
+    for b_, a_ in outs:
+        b_.free()
+        a_.free()
+    return {"workload": "KeySwitch requests on one parameter set, N=65536, L=24, K=6, dnum=4, device-resident",
+            "max_batch": B, "rates": results}
+
+
 def cpu_baseline(hp, x0, kb, ka, gpu_b, gpu_a, samples):
     """The oracle's key switch (reference algorithm, 1 thread, towers in sequence)
     on component 0 of the same batch; also checks the GPU output bit-exactly."""
@@ -341,6 +385,8 @@ def main():
     ap.add_argument("--roofline-bits", type=int, default=59)
     ap.add_argument("--serve-rates", type=str, default="200,1000")
     ap.add_argument("--serve-device-rates", type=str, default="1000,5000,20000")
+    ap.add_argument("--serve-keyswitch-requests", type=int, default=256)
+    ap.add_argument("--serve-keyswitch-rates", type=str, default="2000,6000,12000")
     ap.add_argument("--rotations", type=int, default=8)
     ap.add_argument("--session-requests", type=int, default=16)
     ap.add_argument("--rotation-batch", type=int, default=16)
@@ -437,6 +483,10 @@ def main():
     valu = valu_roofline(roof, narrow)
     if valu:
         progress(f"valu roofline: {valu['achieved']} of {valu['peak']} T butterfly/s")
+    served_ks = None
+    if args.serve_keyswitch_requests:
+        served_ks = served_keyswitch(hp, kb, ka, d_in, d_b, d_a, B, args.serve_keyswitch_requests,
+                                     [float(r) for r in args.serve_keyswitch_rates.split(",")])
     served = None
     if args.serve_requests:
         served = served_stream(args.serve_requests, [float(r) for r in args.serve_rates.split(",")],
@@ -477,6 +527,7 @@ def main():
         "cpu_baseline": cpu,
         "bit_exact_vs_oracle": exact,
         "served_stream": served,
+        "served_keyswitch": served_ks,
         "device": hip.device_name(0),
     }
     print(json.dumps(out), flush=True)

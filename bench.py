@@ -216,7 +216,7 @@ def session_plain(hp, requests):
 
 
 def served_stream(requests, rates, device_rates, seed=0xC5):
-    """Config C5: Poisson arrivals of mixed N/L/dnum/op requests at swept rates
+    """The mixed served stream: Poisson arrivals of mixed N/L/dnum/op requests at swept rates
     into the continuous-batching server; p50/p99 latency and throughput per
     rate, and every response compared with the same request served alone.
     Two modes: "host" (requests carry host arrays: upload, compute, download

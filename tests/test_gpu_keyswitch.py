@@ -3,7 +3,7 @@ automorphism and the fused batched Hybrid key switch against the CPU oracle.
 
 Bit-exact on full arrays: the Driver-seam operations one by one
 (src/cuda/resident_adapter.inc:204-280), then the whole KeySwitchHybrid core
-(include/ntt/openfhe_hybrid_adapter.hpp:163-181) at the north-star shape
+(include/ntt/openfhe_hybrid_adapter.hpp:163-181) at the headline shape
 N=2^16, L=24, K=6, dnum=4 and at reduced levels (key offset i + (M - L)).
 """
 import numpy as np
@@ -226,7 +226,7 @@ def test_keyswitch_ring_sizes_and_widths(n, bits, L, K, alpha, batch):
     _check_keyswitch(hp, L, batch, 9000 + n % 97 + L)
 
 
-def test_keyswitch_north_star_shape():
+def test_keyswitch_n65536_l24_k6_dnum4():
     """N=2^16, L=24 (~50-bit, first 60-bit), K=6 (60-bit), dnum=4 (alpha=6), batch of 2."""
     hp = params.openfhe_like(65536, 24, 6, 6)
     log_case("keyswitch n=65536 L=24 K=6 dnum=4 batch=2")
@@ -309,7 +309,7 @@ def test_keyswitch_lanes_and_graph_replays():
 
 
 @pytest.mark.parametrize("evaluation", [True, False])
-def test_batched_modup_moddown_config_c3(evaluation):
+def test_batched_modup_moddown_l24_k6_alpha6(evaluation):
     """N=2^16, Q: 24 towers (60 + 50-bit), P: 6 towers (60-bit): batched ModUp
     (evaluation or coefficient input) and ModDown, each bit-exact vs the oracle."""
     hp = params.openfhe_like(65536, 24, 6, 6)
@@ -317,7 +317,7 @@ def test_batched_modup_moddown_config_c3(evaluation):
     ctx = hip.Context(n, hp.q + hp.p, hp.q_psi + hp.p_psi)
     ks = hip.KeySwitch(ctx, L, K, hp.alpha, B)
     x = np.concatenate([uniform(n, hp.q, 300 + c) for c in range(B)])
-    log_case(f"modup/moddown c3 evaluation={evaluation}")
+    log_case(f"batched modup/moddown L=24 K=6 alpha=6 evaluation={evaluation}")
     din = _dev(x)
     up = hip.DeviceBuffer(B * (L + K) * n)
     ws = hip.DeviceBuffer(max(ks.modup_workspace_words(B), ks.moddown_workspace_words(B)))

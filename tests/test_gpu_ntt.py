@@ -1,7 +1,7 @@
 """GPU parity of the gfx950 NTT and element-wise kernels against the CPU oracle.
 
 Bit-exact on full arrays (integer work): the reference's seven patterns on the
-pinned OpenFHE QP towers (tests/cuda_openfhe_tests.cpp:67-155), the north-star
+pinned OpenFHE QP towers (tests/cuda_openfhe_tests.cpp:67-155), the headline
 NTT configurations, every ring dimension 2..2^16, and 62-63-bit primes that
 take the canonical (non-lazy) butterfly.
 """
@@ -54,25 +54,25 @@ def test_seven_patterns_pinned_towers(pinned_ctx, subset):
         assert np.array_equal(a.download(), want_prod), f"ring product pattern {pattern}"
 
 
-def test_single_tower_config_c1():
+def test_single_tower_n65536_59bit():
     """One NTT-friendly prime q ~ 2^59, uniform seeded input, forward then inverse."""
     q = params.last_prime(59, 2 * N16)
     psi = params.min_root(q, N16)
     ctx = hip.Context(N16, [q], [psi])
     x = lcg_values(N16, [q])
-    log_case("ntt c1 single tower")
+    log_case("ntt single tower n=65536, 59-bit prime")
     f = _gpu_round(ctx, x, 0, 1, False)
     assert np.array_equal(f, o.ntt_forward(x, q, psi))
     assert np.array_equal(_gpu_round(ctx, f, 0, 1, True), x)
 
 
-def test_batched_rns_config_c2():
+def test_batched_rns_24_primes_2_polys():
     """24 distinct 50-60-bit primes, 2 polynomials = 48 tower transforms in one launch."""
     qs = params.mixed_width_primes(N16, 24)
     rs = [params.min_root(q, N16) for q in qs]
     ctx = hip.Context(N16, qs, rs)
     x = uniform(N16, qs + qs, SEED)
-    log_case("ntt c2 batched 48 towers")
+    log_case("ntt batched 24 primes x 2 polynomials (48 towers)")
     buf = hip.DeviceBuffer(x.size).upload(x)
     ctx.transform_batch(buf.ptr, 2, 24 * N16, 0, 24, False)
     ctx.synchronize()

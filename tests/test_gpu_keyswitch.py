@@ -226,6 +226,31 @@ def test_keyswitch_ring_sizes_and_widths(n, bits, L, K, alpha, batch):
     _check_keyswitch(hp, L, batch, 9000 + n % 97 + L)
 
 
+def _random_shapes(count, seed):
+    rng = np.random.default_rng(seed)
+    shapes = []
+    for _ in range(count):
+        log_n = int(rng.integers(10, 17))
+        bits = int(rng.choice([30, 45, 50, 55, 60, 61, 62]))
+        L = int(rng.integers(1, 13))
+        K = int(rng.integers(1, 5))
+        alpha = int(rng.integers(1, L + 2))
+        batch = int(rng.integers(1, 10)) if log_n <= 13 else int(rng.integers(1, 3))
+        shapes.append((1 << log_n, bits, L, K, alpha, batch))
+    return shapes
+
+
+@pytest.mark.parametrize("n,bits,L,K,alpha,batch", _random_shapes(10, 0xC0FFEE))
+def test_keyswitch_random_shapes(n, bits, L, K, alpha, batch):
+    """Seeded random shapes: ring 2^10..2^16, prime width 30..62 bits (every
+    arithmetic mode), 1..12 Q towers, 1..4 P towers, any digit size (also
+    wider than L), batches up to 9 (two lanes from 8)."""
+    qs = params.find_primes(n, L + K, bits)
+    hp = params.HybridParams(n=n, q=qs[:L], p=qs[L:], alpha=alpha)
+    log_case(f"keyswitch random shape n={n} {bits}-bit L={L} K={K} alpha={alpha} batch={batch}")
+    _check_keyswitch(hp, L, batch, 12000 + n % 1000 + L * 10 + K)
+
+
 def test_keyswitch_n65536_l24_k6_dnum4():
     """N=2^16, L=24 (~50-bit, first 60-bit), K=6 (60-bit), dnum=4 (alpha=6), batch of 2."""
     hp = params.openfhe_like(65536, 24, 6, 6)

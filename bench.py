@@ -348,6 +348,64 @@ def served_keyswitch(hp, kb, ka, d_in, d_b, d_a, B, requests, rates, seed=0x5E):
             "max_batch": B, "rates": results}
 
 
+# The reference's largest published workload (N=65536, 40 towers of 60-bit
+# primes from its generator, zero coefficients in the vector file; operands
+# from its fixed-seed stream), RTX 4080 SUPER medians in ms
+# (docs/batch40-native-analysis.md:27-34, docs/paired-ntt-native-analysis.md:130-134,182-187).
+REFERENCE_PUBLISHED = {
+    ("events", "gpu_forward"): 0.552, ("events", "gpu_inverse"): 0.598, ("events", "gpu_pointwise"): 0.048,
+    ("events", "gpu_fused_multiply"): 1.723,
+    ("public", "gpu_forward"): 8.079, ("public", "gpu_inverse"): 8.045, ("public", "gpu_pointwise"): 11.858,
+    ("public", "gpu_fused_multiply"): 12.431,
+    ("conversion", "resident_modup_moddown"): 3.776, ("conversion", "resident_conversion_host_boundary"): 20.989,
+}
+CLI = os.path.join(ROOT, "ntt-gpu_amd", "bin", "ntt_hip_cli")
+CONVERSION_BENCH = os.path.join(ROOT, "ntt-gpu_amd", "bin", "session_conversion_bench")
+
+
+def reference_workloads():
+    """The reference's own benchmark commands on its largest workload, run
+    through this build's drop-in tools: ntt_hip_cli events-jsonl (resident,
+    device events) and gpu-bench-jsonl (public host-vector calls), and the
+    resident ModUp -> ModDown pair through the include/ntt Session. Each tool
+    checks its results against the CPU reference NTT before reporting."""
+    import tempfile
+    if not (os.path.exists(CLI) and os.path.exists(CONVERSION_BENCH)):
+        return None
+    n, towers = 65536, 40
+    q = params.find_primes(n, towers, 60)
+    lines = [f"{n} {towers}"] + [f"{m} {params.find_psi(m, n)}" for m in q] + ["COEFFS"] + ["0"] * n
+    rows = []
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, f"benchmark-vector-n{n}-k{towers}-b60.txt")
+        with open(path, "w") as f:
+            f.write("\n".join(lines) + "\n")
+        for scope, cmd in (("events", [CLI, "events-jsonl", path, "5", "20"]),
+                           ("public", [CLI, "gpu-bench-jsonl", path, "2", "10"]),
+                           ("conversion", [CONVERSION_BENCH, path, "2", "10"])):
+            try:
+                r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+            except (OSError, subprocess.TimeoutExpired):
+                return None
+            if r.returncode != 0:
+                progress(f"reference workload {scope} failed: {r.stderr.strip()[-200:]}")
+                return None
+            for line in r.stdout.splitlines():
+                rec = json.loads(line)
+                if rec.get("type") != "summary":
+                    continue
+                ref = REFERENCE_PUBLISHED.get((scope, rec["operation"]))
+                rows.append({"scope": scope, "operation": rec["operation"], "median_ms": round(rec["median_ms"], 4),
+                             "p95_ms": round(rec["p95_ms"], 4), "reference_ms": ref,
+                             "reference_over_this": round(ref / rec["median_ms"], 2) if ref else None})
+    progress("reference workloads: " + ", ".join(f"{r['scope']}/{r['operation']} {r['median_ms']} ms" for r in rows
+                                                  if r["reference_ms"]))
+    return {"workload": "N=65536, 40 towers of 60-bit primes (reference generator), fixed-seed operands; "
+                        "conversion: + 3 P primes, 3 towers per digit",
+            "reference_hardware": "RTX 4080 SUPER (sm_89), the reference's published medians",
+            "rows": rows}
+
+
 def cpu_baseline(hp, x0, kb, ka, gpu_b, gpu_a, samples):
     """The oracle's key switch (reference algorithm, 1 thread, towers in sequence)
     on component 0 of the same batch; also checks the GPU output bit-exactly."""
@@ -386,6 +444,7 @@ def main():
     ap.add_argument("--serve-rates", type=str, default="200,1000")
     ap.add_argument("--serve-device-rates", type=str, default="1000,5000,20000")
     ap.add_argument("--serve-keyswitch-requests", type=int, default=256)
+    ap.add_argument("--no-reference-workloads", action="store_true")
     ap.add_argument("--serve-keyswitch-rates", type=str, default="2000,6000,12000")
     ap.add_argument("--rotations", type=int, default=8)
     ap.add_argument("--session-requests", type=int, default=16)
@@ -483,6 +542,7 @@ def main():
     valu = valu_roofline(roof, narrow)
     if valu:
         progress(f"valu roofline: {valu['achieved']} of {valu['peak']} T butterfly/s")
+    refw = reference_workloads() if not args.no_reference_workloads else None
     served_ks = None
     if args.serve_keyswitch_requests:
         served_ks = served_keyswitch(hp, kb, ka, d_in, d_b, d_a, B, args.serve_keyswitch_requests,
@@ -528,6 +588,7 @@ def main():
         "bit_exact_vs_oracle": exact,
         "served_stream": served,
         "served_keyswitch": served_ks,
+        "reference_workloads": refw,
         "device": hip.device_name(0),
     }
     print(json.dumps(out), flush=True)

@@ -7,7 +7,9 @@ CPU: the vector-file parser and decimal reduction against Python big
 integers, the percentile / JSON / operand-stream helpers against numpy,
 json and a Python restatement, and the CLI's preflight errors (raised before
 any device call). GPU: transform / identity / verify / bench and the three
-JSONL modes, every transform output against the oracle.
+JSONL modes, every transform output against the oracle; the resident
+conversion benchmark (reference src/resident_conversion_bench.cpp:97-300) on
+the HIP Session.
 """
 import json
 import os
@@ -22,6 +24,7 @@ from common import log_case
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CLI = os.path.join(ROOT, "ntt-gpu_amd", "bin", "ntt_hip_cli")
+CONVERSION = os.path.join(ROOT, "ntt-gpu_amd", "bin", "session_conversion_bench")
 TOOL = os.path.join(ROOT, "tests", "cpp", "build", "vector_io_tool")
 
 
@@ -160,6 +163,20 @@ def test_cli_usage_and_preflight(tmp_path):
     assert r.returncode == 2
 
 
+def test_conversion_bench_preflight(tmp_path):
+    """The resident conversion benchmark rejects bad counts and moduli before any device call."""
+    n = 16
+    q, psi = _params(n, 2)
+    good = _vector_file(tmp_path / "p.txt", n, q, psi, [0] * n)
+    r = _run(CONVERSION, good, 0, 10)
+    assert r.returncode == 1 and "warmups" in r.stderr
+    q61, psi61 = _params(n, 1, 61)
+    wide = _vector_file(tmp_path / "w.txt", n, q61, psi61, [0] * n)
+    r = _run(CONVERSION, wide, 1, 2)
+    assert r.returncode == 1 and "below 2^60" in r.stderr
+    assert _run(CONVERSION, good).returncode == 1
+
+
 # ---------------------------------------------------------------- GPU
 @pytest.mark.gpu
 @pytest.mark.parametrize("n,count,bits", [(4, 1, 0), (1024, 3, 46), (65536, 2, 60)])
@@ -227,3 +244,22 @@ def test_cli_bench_table():
     rows = [line for line in r.stdout.splitlines() if line.startswith("| 2^")]
     assert [row.split()[1] for row in rows] == ["2^10", "2^11"]
     assert all(float(c) > 0 for row in rows for c in row.strip("|").split("|")[2:])
+
+
+@pytest.mark.gpu
+def test_conversion_bench_resident_pair(tmp_path):
+    """The reference's resident conversion benchmark on the HIP Session: ModUp,
+    ModDown, the pair and the host-boundary pair, every sample checked inside
+    the tool against the CPU reference (exit status 1 on a mismatch)."""
+    n = 4096
+    q, psi = _params(n, 5, 60)
+    log_case(f"session conversion bench n={n} towers=5")
+    path = _vector_file(tmp_path / "p.txt", n, q, psi, [0] * n)
+    r = _run(CONVERSION, path, 1, 3)
+    assert r.returncode == 0, r.stderr
+    lines = [json.loads(x) for x in r.stdout.splitlines()]
+    assert lines[0]["type"] == "workload" and lines[0]["towers"] == 5 and lines[0]["p_towers"] == 3
+    names = [x["operation"] for x in lines if x["type"] == "summary"]
+    assert names == ["resident_modup_coefficient", "resident_modup_evaluation", "resident_moddown_evaluation",
+                     "resident_modup_moddown", "resident_conversion_host_boundary"]
+    assert lines[-1] == {"type": "complete", "operations": 5}

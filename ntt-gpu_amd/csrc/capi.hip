@@ -405,10 +405,13 @@ struct ntt_hip_keyswitch {
     if (!fork) HIP_CHECK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
     while (side.size() + 1 < count) {
       hipStream_t st = nullptr;
-      HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));  // default priority
-      side.push_back(st);
       hipEvent_t e = nullptr;
-      HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));  // default priority
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+        (void)hipStreamDestroy(st);
+        throw RuntimeError("HIP: lane event creation failed");
+      }
+      side.push_back(st);  // streams and join events stay paired
       joins.push_back(e);
     }
   }

@@ -203,6 +203,47 @@ void transform_class_patterns() {
     }
   }
 }
+// The synchronous class on 16 towers (over 2^20 residues: the parallel
+// check-and-stage path): bit-exact forward / inverse against the host
+// reference, and the contract errors (noncanonical residue, wrong size,
+// a bad right operand) thrown with the caller's vectors untouched.
+void transform_class_staging_and_rejections() {
+  const auto q = ntt::find_primes(N, 16, 60);
+  std::vector<uint64_t> r;
+  for (uint64_t m : q) r.push_back(ntt::find_psi(m, N));
+  ntt::cuda::CudaNttTransform gpu(N, q, r);
+  std::vector<uint64_t> in(N * q.size());
+  uint64_t state = 0x5eed;
+  for (size_t i = 0; i < in.size(); ++i) in[i] = lcg(state) % q[i / N];
+  auto v = in;
+  gpu.forward(v);
+  for (size_t k : {size_t{0}, size_t{7}, size_t{15}}) {
+    std::vector<uint64_t> row(in.begin() + k * N, in.begin() + (k + 1) * N);
+    gpu.ref(k).forward(row);
+    check(std::equal(row.begin(), row.end(), v.begin() + k * N), "staged forward, tower " + std::to_string(k));
+  }
+  gpu.inverse(v);
+  check(v == in, "staged round trip");
+  auto bad = in;
+  bad.back() = q.back();  // q itself is not a canonical residue
+  auto left = in;
+  rejects([&] { gpu.forward(bad); }, "noncanonical residue in the last tower");
+  check(bad.back() == q.back(), "rejected vector untouched");
+  rejects([&] { gpu.pointwise_multiply(left, bad); }, "noncanonical right operand");
+  check(left == in, "left operand untouched when the right one is rejected");
+  std::vector<uint64_t> shorter(in.begin(), in.end() - 1);
+  bool length = false;
+  try {
+    gpu.forward(shorter);
+  } catch (const std::length_error &) {
+    length = true;
+  }
+  check(length, "wrong length rejected");
+  auto again = in;
+  gpu.forward(again);  // the transform still works after the rejections
+  gpu.inverse(again);
+  check(again == in, "round trip after rejections");
+}
 }  // namespace
 
 int main() {
@@ -212,6 +253,7 @@ int main() {
     session_key_switch(5);
     session_key_switch(4);
     transform_class_patterns();
+    transform_class_staging_and_rejections();
     std::cout << checks << " HIP RESIDENT CHECKS PASSED (device)\n";
   } catch (const std::exception &e) {
     std::cerr << "FAIL: " << e.what() << '\n';

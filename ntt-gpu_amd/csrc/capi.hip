@@ -379,6 +379,12 @@ struct ntt_hip_keyswitch {
   std::unique_ptr<DeviceBlob> tower_prime;        // identity map 0..L+K
   std::unique_ptr<DeviceBlob> jobs;               // complement NTT jobs for max_batch
   size_t jobs_per_poly = 0;
+  std::vector<uint32_t> tower_job_begin;          // [T + 1]: jobs of tower t are [begin[t], begin[t + 1])
+  // ModUp complement transforms and the key inner product interleaved by
+  // groups of this many towers (0: one transform launch, then one MAC), so
+  // the MAC reads a group's digits while they are still in the Infinity Cache.
+  uint32_t mac_group = kDefaultMacGroup;
+  static constexpr uint32_t kDefaultMacGroup = 2;
   std::unique_ptr<DeviceBlob> key;                // beta*(L+K)*n * 2 (b then a), Montgomery form
   bool key_loaded = false;
   bool mac_per_term = false;
@@ -856,6 +862,9 @@ int ntt_hip_keyswitch_create(ntt_hip_keyswitch **out, ntt_hip_context *ctx, size
         jobs.push_back(j);
       }
     ks->jobs_per_poly = jobs.size();
+    ks->tower_job_begin.assign(T + 1, 0);
+    for (const TowerJob &j : jobs) ++ks->tower_job_begin[j.prime + 1];
+    for (size_t t = 0; t < T; ++t) ks->tower_job_begin[t + 1] += ks->tower_job_begin[t];
     ks->jobs = std::make_unique<DeviceBlob>(ctx->device, jobs.size() * sizeof(TowerJob));
     // Montgomery single-reduction bound for the MAC: beta * max q < 2^64
     u128 bound = 0;
@@ -940,7 +949,29 @@ KsBuffers ks_buffers(const ntt_hip_keyswitch &ks, void *workspace, size_t B) {
 // ModUp of B components (evaluation Q input, stride L*n): coefficient form of
 // every Q tower, per digit the unreduced CRT into its complement, then the
 // forward NTT of every complement tower (one launch, jobs sorted by prime).
+TowerMap complement_jobs(const ntt_hip_keyswitch &ks, const KsBuffers &w) {
+  TowerMap m{};
+  m.base = w.digits;
+  m.jobs = (const TowerJob *)ks.jobs->ptr;
+  m.tower_stride = ks.n();
+  m.poly_stride = ks.digit_words();
+  m.towers_per_poly = (uint32_t)ks.jobs_per_poly;
+  return m;
+}
+
+// INTT of the Q towers and, per digit, the unreduced CRT into its complement
+// (coefficient form, in the digit buffers).
+void ks_modup_convert(const ntt_hip_keyswitch &ks, const uint64_t *in, size_t B, const KsBuffers &w, hipStream_t s);
+
 void ks_modup(const ntt_hip_keyswitch &ks, const uint64_t *in, size_t B, const KsBuffers &w, hipStream_t s) {
+  const ntt_hip_context &c = *ks.ctx;
+  ks_modup_convert(ks, in, B, w, s);
+  hip_ok(launch_ntt(complement_jobs(ks, w), c.dprimes(), c.log_n, (uint32_t)(B * ks.jobs_per_poly), false, c.arith,
+                    s, c.ntt_chunk),
+         "modup ntt");
+}
+
+void ks_modup_convert(const ntt_hip_keyswitch &ks, const uint64_t *in, size_t B, const KsBuffers &w, hipStream_t s) {
   const ntt_hip_context &c = *ks.ctx;
   const size_t L = ks.L, T = ks.L + ks.K, n = c.n;
   const PrimeConst *pr = c.dprimes();
@@ -964,19 +995,12 @@ void ks_modup(const ntt_hip_keyswitch &ks, const uint64_t *in, size_t B, const K
     a.skip_len = count;
     hip_ok(launch_basis_switch(a, plan.per_term, (uint32_t)B, s), "modup conversion");
   }
-  TowerMap m{};
-  m.base = w.digits;
-  m.jobs = (const TowerJob *)ks.jobs->ptr;
-  m.tower_stride = n;
-  m.poly_stride = ks.digit_words();
-  m.towers_per_poly = (uint32_t)ks.jobs_per_poly;
-  hip_ok(launch_ntt(m, pr, c.log_n, (uint32_t)(B * ks.jobs_per_poly), false, c.arith, s, c.ntt_chunk), "modup ntt");
 }
 
 // Key inner product over all digits of `plan`'s workspace with `key`'s
 // resident key, both channels, into z ([2][B] * z_words).
 void ks_mac(const ntt_hip_keyswitch &key, const ntt_hip_keyswitch &plan, const uint64_t *in, size_t B,
-            const KsBuffers &w, uint64_t *z, hipStream_t s) {
+            const KsBuffers &w, uint64_t *z, hipStream_t s, uint32_t first_tower = 0, uint32_t towers = 0) {
   const size_t T = plan.L + plan.K, n = plan.n();
   MacLaunch m{};
   m.own = in;
@@ -995,7 +1019,33 @@ void ks_mac(const ntt_hip_keyswitch &key, const ntt_hip_keyswitch &plan, const u
   m.n = (uint32_t)n;
   m.towers = (uint32_t)T;
   m.digits_count = plan.beta;
+  m.first_tower = first_tower;
+  m.launch_towers = towers;
   hip_ok(launch_mac_fused(m, key.mac_per_term, (uint32_t)B, s), "mac");
+}
+
+// ModUp and the key inner product of `key` into z, interleaved by tower
+// groups (mac_group) when the plan asks for it.
+constexpr size_t kMinGroupBatch = 8;
+void ks_modup_mac(const ntt_hip_keyswitch &key, const ntt_hip_keyswitch &plan, const uint64_t *in, size_t B,
+                  const KsBuffers &w, uint64_t *z, hipStream_t s) {
+  const ntt_hip_context &c = *plan.ctx;
+  const uint32_t T = plan.L + plan.K, G = plan.mac_group;
+  if (G == 0 || G >= T || B < kMinGroupBatch) {  // small batches: fewer, larger launches
+    ks_modup(plan, in, B, w, s);
+    ks_mac(key, plan, in, B, w, z, s);
+    return;
+  }
+  ks_modup_convert(plan, in, B, w, s);
+  const TowerMap m = complement_jobs(plan, w);
+  for (uint32_t t0 = 0; t0 < T; t0 += G) {
+    const uint32_t t1 = std::min(T, t0 + G);
+    const uint32_t k0 = plan.tower_job_begin[t0], k1 = plan.tower_job_begin[t1];
+    hip_ok(launch_ntt_span(m, c.dprimes(), c.log_n, (uint32_t)B, k0 * (uint32_t)B, (k1 - k0) * (uint32_t)B, false,
+                           c.arith, s),
+           "modup ntt group");
+    ks_mac(key, plan, in, B, w, z, s, t0, t1 - t0);
+  }
 }
 
 // ModDown of both channels of z ([2][B] * z_words, evaluation QP): INTT(P) ->
@@ -1054,8 +1104,7 @@ int ntt_hip_keyswitch_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batc
     auto chain = [&](size_t first, size_t count, hipStream_t st) {
       const size_t off = first * ks->L * ks->n();
       const KsBuffers w = ks_buffers(*ks, (uint64_t *)workspace + first * ks->per_component(), count);
-      ks_modup(*ks, in + off, count, w, st);
-      ks_mac(*ks, *ks, in + off, count, w, w.z, st);
+      ks_modup_mac(*ks, *ks, in + off, count, w, w.z, st);
       ks_moddown(*ks, w.z, count, out_b + off, out_a + off, w, st);
     };
     const uint32_t lanes = (uint32_t)std::max<size_t>(1, std::min<size_t>(ks->lanes, batch / ntt_hip_keyswitch::kMinLaneBatch));
@@ -1095,8 +1144,7 @@ int ntt_hip_keyswitch_in_place(ntt_hip_keyswitch *ks, uint64_t *c0, uint64_t *c1
     const KsBuffers w = ks_buffers(*ks, workspace, batch);
     const uint64_t *in = c2 ? c2 : c1;
     DeviceGuard g(ks->ctx->device);
-    ks_modup(*ks, in, batch, w, s);
-    ks_mac(*ks, *ks, in, batch, w, w.z, s);
+    ks_modup_mac(*ks, *ks, in, batch, w, w.z, s);
     // c1 is read only by the ModUp and the MAC above, so the two-component
     // case may overwrite it in the ModDown's last pass (stream order)
     ks_moddown(*ks, w.z, batch, c0, c1, w, s, c2 ? 3u : 1u);
@@ -1124,8 +1172,7 @@ int ntt_hip_relinearize(ntt_hip_keyswitch *const *keys, size_t extra, uint64_t *
     const KsBuffers w = ks_buffers(plan, workspace, batch);
     DeviceGuard g(plan.ctx->device);
     for (size_t j = 0; j < extra; ++j) {
-      ks_modup(plan, higher[j], batch, w, s);
-      ks_mac(*keys[j], plan, higher[j], batch, w, w.z, s);
+      ks_modup_mac(*keys[j], plan, higher[j], batch, w, w.z, s);
       ks_moddown(plan, w.z, batch, c0, c1, w, s, 3u);
     }
     return NTT_HIP_OK;

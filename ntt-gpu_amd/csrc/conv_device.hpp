@@ -44,6 +44,8 @@ struct MacLaunch {
   const uint32_t *tower_prime;    // [towers]
   uint64_t digit_poly_stride, out_poly_stride;
   uint32_t n, towers, digits_count;
+  // towers [first_tower, first_tower + launch_towers) only (0 = all towers)
+  uint32_t first_tower, launch_towers;
 };
 
 struct CorrectLaunch {

@@ -180,7 +180,7 @@ hipError_t launch_mac_digit(uint64_t *b, uint64_t *a, const uint64_t *digit, con
 template <bool PerTerm>
 __global__ __launch_bounds__(256) void mac_fused_kernel(MacLaunch m, uint32_t polys) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  const uint32_t tower = blockIdx.y / polys, poly = blockIdx.y - tower * polys;
+  const uint32_t tower = m.first_tower + blockIdx.y / polys, poly = blockIdx.y % polys;
   if (i >= m.n) return;
   const PrimeConst &pc = m.primes[m.tower_prime[tower]];
   const uint64_t *dg = m.digits + (uint64_t)poly * m.digit_poly_stride + (uint64_t)tower * m.n + i;
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(256) void mac_fused_kernel(MacLaunch m, uint32_t po
 }
 
 hipError_t launch_mac_fused(const MacLaunch &m, bool per_term, uint32_t polys, hipStream_t stream) {
-  dim3 grid((m.n + 255) / 256, m.towers * polys);
+  dim3 grid((m.n + 255) / 256, (m.launch_towers ? m.launch_towers : m.towers) * polys);
   if (per_term) hipLaunchKernelGGL(mac_fused_kernel<true>, grid, dim3(256), 0, stream, m, polys);
   else hipLaunchKernelGGL(mac_fused_kernel<false>, grid, dim3(256), 0, stream, m, polys);
   return hipGetLastError();

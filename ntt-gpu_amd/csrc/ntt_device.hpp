@@ -68,4 +68,10 @@ inline int arith_for(uint64_t max_q) {
 hipError_t launch_ntt(const TowerMap &map, const PrimeConst *primes, uint32_t log_n, uint32_t towers, bool inverse,
                       int arith, hipStream_t stream, uint32_t chunk = 0);
 
+// Launch indices [y0, y0 + count) of a map over `polys` polynomials (both
+// passes): a range of the tower-major index space, e.g. the jobs of a few
+// towers for every polynomial.
+hipError_t launch_ntt_span(const TowerMap &map, const PrimeConst *primes, uint32_t log_n, uint32_t polys, uint32_t y0,
+                           uint32_t count, bool inverse, int arith, hipStream_t stream);
+
 }  // namespace nttamd

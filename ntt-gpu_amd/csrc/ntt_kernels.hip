@@ -458,6 +458,24 @@ static hipError_t launch_t(const TowerMap &map, const PrimeConst *primes, uint32
   return hipGetLastError();
 }
 
+hipError_t launch_ntt_span(const TowerMap &map, const PrimeConst *primes, uint32_t log_n, uint32_t polys, uint32_t y0,
+                           uint32_t count, bool inverse, int arith, hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  if (polys == 0) return hipErrorInvalidValue;
+  TowerMap m = map;
+  m.polys = polys;
+  m.y_offset = map.y_offset + y0;
+  switch (arith * 2 + (inverse ? 1 : 0)) {
+    case kLazy * 2: return launch_t<false, kLazy>(m, primes, log_n, count, stream);
+    case kLazy * 2 + 1: return launch_t<true, kLazy>(m, primes, log_n, count, stream);
+    case kNarrow * 2: return launch_t<false, kNarrow>(m, primes, log_n, count, stream);
+    case kNarrow * 2 + 1: return launch_t<true, kNarrow>(m, primes, log_n, count, stream);
+    case kExact * 2: return launch_t<false, kExact>(m, primes, log_n, count, stream);
+    case kExact * 2 + 1: return launch_t<true, kExact>(m, primes, log_n, count, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 // `chunk` > 0 splits the towers into groups run pass after pass (A then B for
 // one group before the next group), so a group's intermediate hand-off stays
 // in L2 / Infinity Cache instead of a full HBM round trip.

@@ -102,6 +102,28 @@ struct Context {
     return e;
   }
   void *stream() const { return ntt_hip_context_stream(ctx); }
+
+  // Session::transform and pointwise copy their input into a fresh output and
+  // then work on that output in place. The driver holds such a copy back until
+  // its next call: when that call transforms the copy's destination, the copy
+  // becomes the transform's out-of-place first pass (one pass over the data
+  // instead of two); any other call, and the release of either buffer,
+  // enqueues it first, so stream order is unchanged.
+  struct PendingCopy {
+    uint64_t *dst = nullptr;
+    const uint64_t *src = nullptr;
+    size_t n = 0;
+  } pending;
+  void flush_pending() {
+    if (!pending.dst) return;
+    const PendingCopy p = pending;
+    pending = {};
+    ok(ntt_hip_memcpy(device, p.dst, p.src, p.n * sizeof(uint64_t), NTT_HIP_D2D, stream()), "copy");
+  }
+  bool pending_touches(const uint64_t *block, size_t elements) const {
+    const auto in = [&](const uint64_t *p) { return p >= block && p < block + elements; };
+    return pending.dst && (in(pending.dst) || in(pending.src));
+  }
   // Position of a contiguous layout inside the session basis.
   size_t first(const PolynomialLayout &p) const {
     validate(p);
@@ -137,7 +159,17 @@ struct Buffer final : transport::Buffer {
     if (context->cleanup_failed) throw std::runtime_error("resident HIP cleanup failed");
     data = static_cast<uint64_t *>(context->take_block(bytes()));
   }
-  ~Buffer() override { context->give_block(data, bytes()); }
+  ~Buffer() override {
+    if (context->pending_touches(data, elements)) {
+      try {
+        context->flush_pending();  // enqueued before the block can be reused
+      } catch (...) {
+        context->pending = {};
+        context->cleanup_failed = true;
+      }
+    }
+    context->give_block(data, bytes());
+  }
   size_t bytes() const { return std::max<size_t>(1, elements) * sizeof(uint64_t); }
 };
 
@@ -167,6 +199,7 @@ class Driver final : public transport::Driver {
   void healthy() const {
     if (c_->cleanup_failed) throw std::runtime_error("resident HIP cleanup failed");
   }
+  void flush() { c_->flush_pending(); }
 
  public:
   Driver(PolynomialLayout p, int ordinal) : c_(std::make_shared<Context>(std::move(p), ordinal)) {}
@@ -181,19 +214,22 @@ class Driver final : public transport::Driver {
     const auto &a = buf(out);
     require(a.elements == src.size(), "HIP upload extent");
     healthy();
+    flush();
     ok(ntt_hip_memcpy(c_->device, a.data, src.data(), src.size_bytes(), NTT_HIP_H2D, c_->stream()), "upload");
   }
   void copy(transport::Buffer &out, const transport::Buffer &in, size_t offset, size_t n) override {
     const auto &a = buf(out), &b = buf(in);
     require(a.elements == n && offset <= b.elements && n <= b.elements - offset, "HIP copy extent");
     healthy();
-    ok(ntt_hip_memcpy(c_->device, a.data, b.data + offset, n * sizeof(uint64_t), NTT_HIP_D2D, c_->stream()), "copy");
+    flush();
+    c_->pending = {a.data, b.data + offset, n};
   }
   void copy_range(transport::Buffer &out, size_t dst, const transport::Buffer &in, size_t src, size_t n) override {
     const auto &a = buf(out), &b = buf(in);
     require(dst <= a.elements && n <= a.elements - dst && src <= b.elements && n <= b.elements - src,
             "HIP range copy extent");
     healthy();
+    flush();
     ok(ntt_hip_memcpy(c_->device, a.data + dst, b.data + src, n * sizeof(uint64_t), NTT_HIP_D2D, c_->stream()),
        "copy_range");
   }
@@ -201,8 +237,14 @@ class Driver final : public transport::Driver {
     const auto &a = buf(out);
     require(a.elements == bytes(p) / sizeof(uint64_t), "HIP transform extent");
     healthy();
-    ok(ntt_hip_transform(c_->ctx, a.data, nullptr, c_->first(p), p.towers.size(), inverse, c_->stream()),
-       "transform");
+    const uint64_t *src = nullptr;
+    if (c_->pending.dst == a.data && c_->pending.n == a.elements) {
+      src = c_->pending.src;  // the held-back copy becomes the first pass's input
+      c_->pending = {};
+    } else {
+      flush();
+    }
+    ok(ntt_hip_transform(c_->ctx, a.data, src, c_->first(p), p.towers.size(), inverse, c_->stream()), "transform");
   }
   void pointwise(transport::Buffer &out, const transport::Buffer &right, size_t offset,
                  const PolynomialLayout &p) override {
@@ -210,6 +252,7 @@ class Driver final : public transport::Driver {
     require(a.elements == bytes(p) / sizeof(uint64_t) && offset <= b.elements && a.elements <= b.elements - offset,
             "HIP pointwise extent");
     healthy();
+    flush();
     ok(ntt_hip_pointwise(c_->ctx, a.data, b.data + offset, c_->first(p), p.towers.size(), c_->stream()),
        "pointwise");
   }
@@ -222,6 +265,7 @@ class Driver final : public transport::Driver {
                 k2.elements == count,
             "HIP Hybrid MAC extent");
     healthy();
+    flush();
     ok(ntt_hip_hybrid_mac(c_->ctx, b.data, a.data, d.data, k1.data, k2.data, c_->first(layout), layout.towers.size(),
                           first_digit, c_->stream()),
        "hybrid_mac");
@@ -236,6 +280,7 @@ class Driver final : public transport::Driver {
                 table.elements == layout.active_q,
             "HIP Hybrid rotation extent");
     healthy();
+    flush();
     const auto p = c_->p_positions();
     ok(ntt_hip_hybrid_rotate(c_->ctx, b.data, a.data, xb.data, xa.data, first.data + c0_offset, c_->first(layout),
                              layout.towers.size(), layout.active_q, p.data(), p.size(), automorphism, c_->stream()),
@@ -250,6 +295,7 @@ class Driver final : public transport::Driver {
                 table.elements == 3 * s + t + s * t,
             "HIP basis extent");
     healthy();
+    flush();
     // The context derives (and caches) its Montgomery-form plan for the same
     // source/target primes; the session's packed table is the shape check.
     const auto src = c_->positions(source.towers), dst = c_->positions(target.towers);
@@ -265,6 +311,7 @@ class Driver final : public transport::Driver {
                 a.elements <= c.elements - right_offset && table.elements == 2 * p.towers.size(),
             "HIP correction extent");
     healthy();
+    flush();
     const auto pp = c_->p_positions();
     ok(ntt_hip_correct(c_->ctx, a.data, b.data + left_offset, c.data + right_offset, c_->first(p), p.towers.size(),
                        pp.data(), pp.size(), c_->stream()),
@@ -274,12 +321,14 @@ class Driver final : public transport::Driver {
     const auto &a = buf(in);
     require(offset <= a.elements && target.size() <= a.elements - offset, "HIP download extent");
     healthy();
+    flush();
     ok(ntt_hip_memcpy(c_->device, target.data(), a.data + offset, target.size_bytes(), NTT_HIP_D2H, c_->stream()),
        "download");
   }
   void record(transport::Event &e) override {
     auto &out = evt(e);
     healthy();
+    flush();  // the event covers every command issued before it
     ok(ntt_hip_event_record(c_->device, out.event, c_->stream()), "record");
     out.recorded = true;
   }
@@ -300,6 +349,7 @@ class Driver final : public transport::Driver {
   }
   bool quiesce() noexcept override {
     try {
+      flush();
       return ntt_hip_stream_synchronize(c_->device, c_->stream()) == NTT_HIP_OK;
     } catch (...) {
       return false;

@@ -320,6 +320,14 @@ void conversion_chain_and_failures() {
   ntt::ApproximateBasisPlan ref({ps[0].q, ps[1].q}, {qs[0].q, qs[1].q, qs[2].q});
   equal(s.download(switched).wait_values(), ref.convert(8, std::span<const uint64_t>(values).subspan(24)),
         "basis switch honours a nonzero view offset");
+  // Tables are built once per session but still staged and uploaded per call.
+  const size_t tables_before = s.stats().parameter_uploads;
+  auto red_again = s.mod_down(ext);
+  check(s.stats().parameter_uploads == tables_before + 2, "a repeated ModDown uploads both of its tables again");
+  const auto red_first = s.download(red);  // owns the staging its values point into
+  const auto first_values = red_first.wait_values();
+  equal(s.download(red_again).wait_values(), {first_values.begin(), first_values.end()},
+        "a repeated ModDown reuses its tables with the same result");
   const size_t before = d->trace->calls;
   rejects([&] { s.mod_down(up); }, "ModDown rejects coefficient input");
   rejects([&] { s.mod_up(up); }, "ModUp rejects QP input");

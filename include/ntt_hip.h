@@ -123,6 +123,13 @@ int ntt_hip_correct(ntt_hip_context *ctx, uint64_t *out, const uint64_t *left, c
 /* Driver::hybrid_mac (resident_adapter.inc:214-231): b,a = (first ? 0 : b,a) + digit * key_{b,a} */
 int ntt_hip_hybrid_mac(ntt_hip_context *ctx, uint64_t *b, uint64_t *a, const uint64_t *digit, const uint64_t *key_b,
                        const uint64_t *key_a, size_t first, size_t towers, int first_digit, void *stream);
+/* Consecutive Driver::hybrid_mac calls on the same outputs (src/resident.cpp:530-570
+ * issues one per digit) in one pass: b,a = (first ? 0 : b,a) + sum_j digits[j] *
+ * key_{b,a}[j], equal to calling ntt_hip_hybrid_mac for j = 0..count-1 in order
+ * (first_digit applying to j = 0). */
+int ntt_hip_hybrid_mac_digits(ntt_hip_context *ctx, uint64_t *b, uint64_t *a, const uint64_t *const *digits,
+                              const uint64_t *const *key_b, const uint64_t *const *key_a, size_t count, size_t first,
+                              size_t towers, int first_digit, void *stream);
 /* Driver::hybrid_rotate (resident_adapter.inc:232-253): sigma_k(B + P*c0), sigma_k(A)
  * on evaluation QP towers [first, first+towers) whose first active_q are Q;
  * P = product of context towers p[0..k). */

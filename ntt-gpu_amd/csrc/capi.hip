@@ -782,6 +782,37 @@ int ntt_hip_hybrid_mac(ntt_hip_context *ctx, uint64_t *b, uint64_t *a, const uin
   });
 }
 
+int ntt_hip_hybrid_mac_digits(ntt_hip_context *ctx, uint64_t *b, uint64_t *a, const uint64_t *const *digits,
+                              const uint64_t *const *key_b, const uint64_t *const *key_a, size_t count, size_t first,
+                              size_t towers, int first_digit, void *stream) {
+  return guarded([&] {
+    require(ctx && b && a && digits && key_b && key_a, "null argument");
+    require(count >= 1, "no digits");
+    for (size_t j = 0; j < count; ++j) require(digits[j] && key_b[j] && key_a[j], "null digit or key row");
+    ctx->check_range(first, towers);
+    DeviceGuard g(ctx->device);
+    // Launches of at most kMacDigitsPerLaunch digits; each after the first accumulates.
+    for (size_t j0 = 0; j0 < count; j0 += kMacDigitsPerLaunch) {
+      MacDigitsLaunch m{};
+      m.b = b;
+      m.a = a;
+      m.count = (uint32_t)std::min<size_t>(kMacDigitsPerLaunch, count - j0);
+      for (uint32_t j = 0; j < m.count; ++j) {
+        m.digits[j] = digits[j0 + j];
+        m.key_b[j] = key_b[j0 + j];
+        m.key_a[j] = key_a[j0 + j];
+      }
+      m.primes = ctx->dprimes();
+      m.first_prime = (uint32_t)first;
+      m.n = ctx->n;
+      m.towers = (uint32_t)towers;
+      m.first_digit = (first_digit != 0 && j0 == 0) ? 1 : 0;
+      hip_ok(launch_mac_digits(m, ctx->pick(stream)), "hybrid mac launch");
+    }
+    return NTT_HIP_OK;
+  });
+}
+
 int ntt_hip_hybrid_rotate(ntt_hip_context *ctx, uint64_t *out_b, uint64_t *out_a, const uint64_t *ext_b,
                           const uint64_t *ext_a, const uint64_t *c0, size_t first, size_t towers, size_t active_q,
                           const size_t *p, size_t k, uint32_t automorphism, void *stream) {

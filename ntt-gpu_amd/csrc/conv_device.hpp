@@ -73,6 +73,18 @@ hipError_t launch_mac_digit(uint64_t *b, uint64_t *a, const uint64_t *digit, con
                             const uint64_t *ka, const PrimeConst *primes, uint32_t first_prime, uint32_t n,
                             uint32_t towers, bool first_digit, hipStream_t stream);
 hipError_t launch_mac_fused(const MacLaunch &m, bool per_term, uint32_t polys, hipStream_t stream);
+
+// Several per-digit MAC steps of the resident driver in one pass: digit j at
+// digits[j], its canonical key rows at key_b[j], key_a[j] (same extent as b, a).
+constexpr uint32_t kMacDigitsPerLaunch = 8;
+struct MacDigitsLaunch {
+  uint64_t *b, *a;
+  const uint64_t *digits[kMacDigitsPerLaunch], *key_b[kMacDigitsPerLaunch], *key_a[kMacDigitsPerLaunch];
+  const PrimeConst *primes;
+  uint32_t first_prime, n, towers, count;
+  int first_digit;  // 1: b, a = sum; 0: b, a += sum
+};
+hipError_t launch_mac_digits(const MacDigitsLaunch &m, hipStream_t stream);
 hipError_t launch_to_mont(uint64_t *data, const PrimeConst *primes, const uint32_t *tower_prime, uint32_t n,
                           uint32_t towers, hipStream_t stream);
 hipError_t launch_correct(const CorrectLaunch &c, uint32_t polys, hipStream_t stream);

@@ -171,6 +171,30 @@ hipError_t launch_mac_digit(uint64_t *b, uint64_t *a, const uint64_t *digit, con
   return hipGetLastError();
 }
 
+// The same accumulation over m.count digits in one pass (each product a
+// Barrett reduction, the sum kept canonical), so b and a are read and written
+// once instead of once per digit; results equal the per-digit sequence.
+__global__ __launch_bounds__(256) void mac_digits_kernel(MacDigitsLaunch m) {
+  const uint64_t idx = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (uint64_t)m.n * m.towers) return;
+  const PrimeConst &pc = m.primes[m.first_prime + idx / m.n];
+  uint64_t rb = m.first_digit ? 0 : m.b[idx], ra = m.first_digit ? 0 : m.a[idx];
+  for (uint32_t j = 0; j < m.count; ++j) {
+    const uint64_t d = m.digits[j][idx];
+    rb = add_mod(rb, barrett_mul(d, m.key_b[j][idx], pc.q, pc.mu_lo, pc.mu_hi), pc.q);
+    ra = add_mod(ra, barrett_mul(d, m.key_a[j][idx], pc.q, pc.mu_lo, pc.mu_hi), pc.q);
+  }
+  m.b[idx] = rb;
+  m.a[idx] = ra;
+}
+
+hipError_t launch_mac_digits(const MacDigitsLaunch &m, hipStream_t stream) {
+  if (m.count == 0 || m.count > kMacDigitsPerLaunch) return hipErrorInvalidValue;
+  const uint64_t total = (uint64_t)m.n * m.towers;
+  hipLaunchKernelGGL(mac_digits_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, stream, m);
+  return hipGetLastError();
+}
+
 // Fused inner product over all digits with resident Montgomery-form keys:
 // Z[t][i] = sum_d D_d[t][i] * K_d[t][i] for both channels, one reduction per output.
 // digits: poly-major (poly, digit, tower, i); keys: (digit, tower, i) in Montgomery form.

@@ -114,15 +114,45 @@ struct Context {
     const uint64_t *src = nullptr;
     size_t n = 0;
   } pending;
-  void flush_pending() {
+  void flush_copy() {
     if (!pending.dst) return;
     const PendingCopy p = pending;
     pending = {};
     ok(ntt_hip_memcpy(device, p.dst, p.src, p.n * sizeof(uint64_t), NTT_HIP_D2D, stream()), "copy");
   }
+
+  // Session::hybrid_accumulate issues one hybrid_mac per digit into the same
+  // two outputs, back to back. The driver holds them back the same way and
+  // enqueues the run as one pass over all its digits (b and a read and
+  // written once instead of once per digit), on the first call that is not
+  // the next digit of the run, or at the release of any buffer it reads.
+  struct PendingMac {
+    uint64_t *b = nullptr, *a = nullptr;
+    size_t first = 0, towers = 0;
+    bool first_digit = false;
+    std::vector<const uint64_t *> digits, key_b, key_a;
+  } mac;
+  void flush_mac() {
+    if (!mac.b) return;
+    PendingMac m = std::move(mac);
+    mac = {};
+    ok(ntt_hip_hybrid_mac_digits(ctx, m.b, m.a, m.digits.data(), m.key_b.data(), m.key_a.data(), m.digits.size(),
+                                 m.first, m.towers, m.first_digit, stream()),
+       "hybrid_mac");
+  }
+  // Everything held back, in stream order (at most one of the two is pending).
+  void flush_pending() {
+    flush_copy();
+    flush_mac();
+  }
   bool pending_touches(const uint64_t *block, size_t elements) const {
     const auto in = [&](const uint64_t *p) { return p >= block && p < block + elements; };
-    return pending.dst && (in(pending.dst) || in(pending.src));
+    if (pending.dst && (in(pending.dst) || in(pending.src))) return true;
+    if (!mac.b) return false;
+    if (in(mac.b) || in(mac.a)) return true;
+    for (size_t j = 0; j < mac.digits.size(); ++j)
+      if (in(mac.digits[j]) || in(mac.key_b[j]) || in(mac.key_a[j])) return true;
+    return false;
   }
   // Position of a contiguous layout inside the session basis.
   size_t first(const PolynomialLayout &p) const {
@@ -165,6 +195,7 @@ struct Buffer final : transport::Buffer {
         context->flush_pending();  // enqueued before the block can be reused
       } catch (...) {
         context->pending = {};
+        context->mac = {};
         context->cleanup_failed = true;
       }
     }
@@ -265,10 +296,22 @@ class Driver final : public transport::Driver {
                 k2.elements == count,
             "HIP Hybrid MAC extent");
     healthy();
-    flush();
-    ok(ntt_hip_hybrid_mac(c_->ctx, b.data, a.data, d.data, k1.data, k2.data, c_->first(layout), layout.towers.size(),
-                          first_digit, c_->stream()),
-       "hybrid_mac");
+    c_->flush_copy();
+    const size_t first = c_->first(layout);
+    auto &m = c_->mac;
+    const bool next_digit = m.b == b.data && m.a == a.data && m.first == first && m.towers == layout.towers.size() &&
+                            !first_digit;
+    if (!next_digit) {
+      c_->flush_mac();
+      m.b = b.data;
+      m.a = a.data;
+      m.first = first;
+      m.towers = layout.towers.size();
+      m.first_digit = first_digit;
+    }
+    m.digits.push_back(d.data);
+    m.key_b.push_back(k1.data);
+    m.key_a.push_back(k2.data);
   }
   void hybrid_rotate(transport::Buffer &ob, transport::Buffer &oa, const transport::Buffer &eb,
                      const transport::Buffer &ea, const transport::Buffer &c0, size_t c0_offset,

@@ -60,6 +60,8 @@ SIGNATURES = {
     "ntt_hip_switch_basis": [VP, VP, VP, SZP, SZ, SZP, SZ, VP],
     "ntt_hip_correct": [VP, VP, VP, VP, SZ, SZ, SZP, SZ, VP],
     "ntt_hip_hybrid_mac": [VP, VP, VP, VP, VP, VP, SZ, SZ, INT, VP],
+    "ntt_hip_hybrid_mac_digits": [VP, VP, VP, ctypes.POINTER(VP), ctypes.POINTER(VP), ctypes.POINTER(VP), SZ, SZ, SZ,
+                                  INT, VP],
     "ntt_hip_hybrid_rotate": [VP, VP, VP, VP, VP, VP, SZ, SZ, SZ, SZP, SZ, ctypes.c_uint32, VP],
     "ntt_hip_keyswitch_create": [ctypes.POINTER(VP), VP, SZ, SZ, SZ, SZ],
     "ntt_hip_keyswitch_destroy": [VP],
@@ -277,6 +279,13 @@ class Context:
     def hybrid_mac(self, b_ptr, a_ptr, digit_ptr, kb_ptr, ka_ptr, first, towers, first_digit, stream=None):
         check(lib().ntt_hip_hybrid_mac(self.handle, b_ptr, a_ptr, digit_ptr, kb_ptr, ka_ptr, first, towers,
                                        int(first_digit), stream), "hybrid_mac")
+
+    def hybrid_mac_digits(self, b_ptr, a_ptr, digit_ptrs, kb_ptrs, ka_ptrs, first, towers, first_digit,
+                          stream=None):
+        count = len(digit_ptrs)
+        arr = lambda ptrs: (VP * max(1, count))(*ptrs)
+        check(lib().ntt_hip_hybrid_mac_digits(self.handle, b_ptr, a_ptr, arr(digit_ptrs), arr(kb_ptrs), arr(ka_ptrs),
+                                              count, first, towers, int(first_digit), stream), "hybrid_mac_digits")
 
     def hybrid_rotate(self, ob_ptr, oa_ptr, eb_ptr, ea_ptr, c0_ptr, first, towers, active_q, p, automorphism,
                       stream=None):

@@ -99,6 +99,28 @@ def test_correct_mac_rotate_pointwise(small):
     wb, wa = o.hybrid_mac(np.concatenate(digits), beta, np.concatenate(kb), np.concatenate(ka), n, hp.q, hp.p, L)
     assert np.array_equal(b.download(), wb) and np.array_equal(a.download(), wa)
 
+    # the same digits in one pass (the driver's merged run), then 11 digits
+    # (two launches of at most 8) accumulating onto existing outputs, against
+    # the per-digit entry point
+    log_case("hybrid_mac_digits")
+    ptrs = [keep[3 * d:3 * d + 3] for d in range(beta)]
+    b2, a2 = hip.DeviceBuffer(T * n), hip.DeviceBuffer(T * n)
+    ctx.hybrid_mac_digits(b2.ptr, a2.ptr, [p[0].ptr for p in ptrs], [p[1].ptr for p in ptrs],
+                          [p[2].ptr for p in ptrs], 0, T, True)
+    ctx.synchronize()
+    assert np.array_equal(b2.download(), wb) and np.array_equal(a2.download(), wa)
+    many = [[_dev(uniform(n, mods, 50 + 3 * d + j)) for j in range(3)] for d in range(11)]
+    b3, a3 = _dev(wb), _dev(wa)
+    for d in range(11):
+        ctx.hybrid_mac(b3.ptr, a3.ptr, many[d][0].ptr, many[d][1].ptr, many[d][2].ptr, 0, T, False)
+    ctx.hybrid_mac_digits(b2.ptr, a2.ptr, [m[0].ptr for m in many], [m[1].ptr for m in many],
+                          [m[2].ptr for m in many], 0, T, False)
+    ctx.synchronize()
+    assert np.array_equal(b2.download(), b3.download()) and np.array_equal(a2.download(), a3.download())
+    assert not np.array_equal(b3.download(), wb)
+    with pytest.raises(hip.NttHipError):
+        ctx.hybrid_mac_digits(b2.ptr, a2.ptr, [], [], [], 0, T, True)
+
     log_case("hybrid_rotate")
     c0 = uniform(n, hp.q, 40)
     dc0 = _dev(c0)

@@ -355,6 +355,51 @@ def test_keyswitch_lanes_and_graph_replays():
         assert np.array_equal(d_b.download(), gb) and np.array_equal(d_a.download(), ga), f"replay {replay}"
 
 
+def test_keyswitch_full_batch_64_graph():
+    """The measured configuration: N=2^16, L=24, K=6, dnum=4, 64 components
+    per captured-graph launch (two lanes of 32, ModUp transforms and MAC
+    interleaved by tower groups). Replayed twice; the components at both ends
+    of both lanes equal the same component switched alone, and component 0
+    equals the oracle's key switch."""
+    hp = params.openfhe_like(65536, 24, 6, 6)
+    L, n, B = 24, 65536, 64
+    log_case("keyswitch n=65536 L=24 K=6 dnum=4 batch=64 graph")
+    ctx = hip.Context(n, hp.q + hp.p, hp.q_psi + hp.p_psi)
+    kb, ka = _key(hp, 9900)
+    ks = hip.KeySwitch(ctx, L, hp.K, hp.alpha, B)
+    ks.upload_key(kb, ka, hp.M, hp.q)
+    x = np.concatenate([uniform(n, hp.q, 9901 + c) for c in range(B)])
+    d_in, d_b, d_a = _dev(x), hip.DeviceBuffer(B * L * n), hip.DeviceBuffer(B * L * n)
+    ws = hip.DeviceBuffer(ks.workspace_words(B))
+    st = hip.Stream(0)
+    graph = ks.capture(d_in.ptr, B, d_b.ptr, d_a.ptr, ws.ptr, st.handle)
+    one = hip.KeySwitch(ctx, L, hp.K, hp.alpha, 1)
+    one.upload_key(kb, ka, hp.M, hp.q)
+    ws1 = hip.DeviceBuffer(one.workspace_words(1))
+    alone = {}
+    for c in (0, 31, 32, 63):
+        sl = slice(c * L * n, (c + 1) * L * n)
+        di, ob, oa = _dev(x[sl]), hip.DeviceBuffer(L * n), hip.DeviceBuffer(L * n)
+        one.run(di.ptr, 1, ob.ptr, oa.ptr, ws1.ptr)
+        ctx.synchronize()
+        alone[c] = (ob.download(), oa.download())
+    wb, wa = o.key_switch(x[:L * n], n, hp.q, hp.q_psi, hp.M, hp.p, hp.p_psi, hp.alpha, kb, ka)
+    assert np.array_equal(alone[0][0], wb) and np.array_equal(alone[0][1], wa)
+    zeros = np.zeros(B * L * n, np.uint64)
+    for replay in range(2):
+        d_b.upload(zeros)
+        d_a.upload(zeros)
+        graph.launch(st.handle)
+        st.synchronize()
+        gb, ga = d_b.download(), d_a.download()
+        for c, (eb, ea) in alone.items():
+            sl = slice(c * L * n, (c + 1) * L * n)
+            assert np.array_equal(gb[sl], eb) and np.array_equal(ga[sl], ea), f"replay {replay} component {c}"
+    graph.close()
+    for buf in (d_in, d_b, d_a, ws, ws1):
+        buf.free()
+
+
 @pytest.mark.parametrize("evaluation", [True, False])
 def test_batched_modup_moddown_l24_k6_alpha6(evaluation):
     """N=2^16, Q: 24 towers (60 + 50-bit), P: 6 towers (60-bit): batched ModUp

@@ -66,6 +66,32 @@ def test_single_tower_n65536_59bit():
     assert np.array_equal(_gpu_round(ctx, f, 0, 1, True), x)
 
 
+@pytest.mark.parametrize("bits", [59, 30])
+def test_full_size_2048_towers(bits):
+    """The roofline workload at full size: 2048 towers of one prime (1 GiB) in
+    one standalone call, which runs them in Infinity-Cache-sized groups (four
+    of 448 and one of 256). The towers on both sides of every group boundary
+    are bit-exact against the oracle, and forward then inverse returns every
+    tower."""
+    q = params.last_prime(bits, 2 * N16)
+    psi = params.min_root(q, N16)
+    ctx = hip.Context(N16, [q], [psi])
+    T = 2048
+    x = np.random.default_rng(bits).integers(0, q, T * N16, dtype=np.uint64)
+    log_case(f"ntt full size: 2048 towers, {bits}-bit prime")
+    buf = hip.DeviceBuffer(x.size).upload(x)
+    ctx.transform_batch(buf.ptr, T, N16, 0, 1, False)
+    ctx.synchronize()
+    f = buf.download()
+    for t in (0, 447, 448, 895, 896, 1343, 1344, 1791, 1792, 2047):
+        sl = slice(t * N16, (t + 1) * N16)
+        assert np.array_equal(f[sl], o.ntt_forward(x[sl], q, psi)), f"tower {t}"
+    ctx.transform_batch(buf.ptr, T, N16, 0, 1, True)
+    ctx.synchronize()
+    assert np.array_equal(buf.download(), x)
+    buf.free()
+
+
 def test_batched_rns_24_primes_2_polys():
     """24 distinct 50-60-bit primes, 2 polynomials = 48 tower transforms in one launch."""
     qs = params.mixed_width_primes(N16, 24)

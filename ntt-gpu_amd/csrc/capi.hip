@@ -893,6 +893,12 @@ int ntt_hip_keyswitch_create(ntt_hip_keyswitch **out, ntt_hip_context *ctx, size
         jobs.push_back(j);
       }
     ks->jobs_per_poly = jobs.size();
+    // A launch of the plan takes at most this many workgroup rows (grid y) per
+    // component: the complement transforms, both ModDown channels, or
+    // conversion chunks of up to 40 targets for both channels. HIP allows
+    // 65536 rows per launch.
+    const size_t rows_per_component = std::max({ks->jobs_per_poly, 2 * T, size_t{80}});
+    require(max_batch <= 65535 / rows_per_component, "batch capacity exceeds the rows of one launch");
     ks->tower_job_begin.assign(T + 1, 0);
     for (const TowerJob &j : jobs) ++ks->tower_job_begin[j.prime + 1];
     for (size_t t = 0; t < T; ++t) ks->tower_job_begin[t + 1] += ks->tower_job_begin[t];

@@ -612,3 +612,8 @@ def test_relinearize_rejections():
         ks.in_place(d.ptr, d2.ptr, 2, ws.ptr)
     with pytest.raises(hip.ContractError, match="one relinearization key"):
         hip.KeySwitch.relinearize([ks], d.ptr, d2.ptr, [], 1, ws.ptr)
+    # every launch must fit the 65536 workgroup rows HIP allows: 80 rows per
+    # component here (T = 8, 3 digits x 6 complement towers = 18 jobs)
+    assert hip.KeySwitch(ctx, 6, 2, 2, 65535 // 80).max_batch == 65535 // 80
+    with pytest.raises(hip.ContractError, match="rows of one launch"):
+        hip.KeySwitch(ctx, 6, 2, 2, 65535 // 80 + 1)

@@ -457,7 +457,11 @@ int ntt_hip_device_name(int device, char *name, size_t capacity) {
     require(name && capacity > 0, "null name buffer");
     hipDeviceProp_t p;
     HIP_CHECK(hipGetDeviceProperties(&p, device));
-    std::snprintf(name, capacity, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    // some drivers leave the marketing name empty: then the ordinal stands in for it
+    char fallback[32];
+    std::snprintf(fallback, sizeof fallback, "HIP device %d", device);
+    std::snprintf(name, capacity, "%s (%s, %d CUs)", p.name[0] ? p.name : fallback, p.gcnArchName,
+                  p.multiProcessorCount);
     return NTT_HIP_OK;
   });
 }

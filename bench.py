@@ -193,6 +193,34 @@ def hoisted_rotations(ctx, hp, x, stream, R, B, steps=5):
             "note": "one ModUp of c1 shared by the R rotations; per rotation MAC, automorphism, ModDown"}
 
 
+def in_place_and_relinearize(ks, x, stream, B, steps=5):
+    """The caller-facing wrappers on the batched path: KeySwitchInPlace on B
+    two-component ciphertexts (c1 switched; c0 += B, c1 = A, additions fused
+    into ModDown's last pass) and relinearization of B three-component
+    ciphertexts with one key (c2 switched, c0 += B, c1 += A). Each call
+    updates its ciphertexts in place, so the timed calls run back to back on
+    the same buffers; KeySwitch/s counts switched components."""
+    comp = L * N
+    c = [hip.DeviceBuffer(B * comp).upload(x[: B * comp]) for _ in range(3)]
+    ws = hip.DeviceBuffer(ks.workspace_words(B))
+    e0, e1 = hip.Event(), hip.Event()
+    out = {}
+    for name, call in (("in_place", lambda: ks.in_place(c[0].ptr, c[1].ptr, B, ws.ptr, stream=stream)),
+                       ("relinearize", lambda: hip.KeySwitch.relinearize([ks], c[0].ptr, c[1].ptr, [c[2].ptr], B,
+                                                                         ws.ptr, stream))):
+        call()
+        e0.record(stream)
+        for _ in range(steps):
+            call()
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / steps
+        out[name] = {"batch": B, "ms_per_call": round(ms, 3), "keyswitch_per_s": round(B / (ms * 1e-3), 1)}
+    for b in c + [ws]:
+        b.free()
+    return out
+
+
 def session_plain(hp, requests):
     """The reference-API plain path: KeySwitchHybrid one request at a time
     through the include/ntt Session (hybrid_digits, hybrid_accumulate, two
@@ -535,6 +563,9 @@ def main():
         progress(f"session plain: {sess['keyswitch_per_s']} KS/s")
     rot = hoisted_rotations(ctx, hp, x, s, args.rotations, min(B, args.rotation_batch))
     progress(f"hoisted rotations: {rot['rotations_per_s']} rotations/s")
+    wrappers = in_place_and_relinearize(ks, x, s, B)
+    progress(f"in-place key switch: {wrappers['in_place']['keyswitch_per_s']} KS/s, "
+             f"relinearize: {wrappers['relinearize']['keyswitch_per_s']} KS/s")
     roof = ntt_roofline(args.ntt_towers, 12, s)
     progress(f"ntt roofline done: {roof['achieved']} GB/s")
     narrow = ntt_roofline(args.ntt_towers, 12, s, bits=30)
@@ -581,6 +612,7 @@ def main():
         "plain_session": sess,
         "batched_over_plain_session": round(batched_kss / sess["keyswitch_per_s"], 2) if sess else None,
         "hoisted_rotations": rot,
+        "keyswitch_wrappers": wrappers,
         "roofline": roof,
         "valu_roofline": valu,
         "narrow_ntt_roofline": narrow,

@@ -163,6 +163,21 @@ void session_key_switch(size_t L) {
         "oracle ran");
   same(ob.wait_values(), wb, N, "device Session key switch B equals the oracle, L=" + std::to_string(L));
   same(oa.wait_values(), wa, N, "device Session key switch A equals the oracle, L=" + std::to_string(L));
+  // Two more accumulations queued back to back (the driver merges each one's
+  // per-digit MACs into one pass): a second input, then the first input again.
+  const auto c2 = seeded(source, 177 + L);
+  const auto z2 = s.hybrid_accumulate(s.hybrid_digits(s.upload(source, c2)), key);
+  const auto z3 = s.hybrid_accumulate(s.hybrid_digits(s.upload(source, c)), key);
+  const auto ob2 = s.download(s.mod_down(z2.b)), oa2 = s.download(s.mod_down(z2.a));
+  const auto ob3 = s.download(s.mod_down(z3.b)), oa3 = s.download(s.mod_down(z3.a));
+  std::vector<uint64_t> wb2(L * N), wa2(L * N);
+  check(or_key_switch(c2.data(), N, qm.data(), qr.data(), L, M, pm.data(), pr.data(), K, alpha, kb.data(), ka.data(),
+                      wb2.data(), wa2.data()) == 0,
+        "oracle ran");
+  same(ob2.wait_values(), wb2, N, "second Session key switch B equals the oracle");
+  same(oa2.wait_values(), wa2, N, "second Session key switch A equals the oracle");
+  same(ob3.wait_values(), wb, N, "repeated Session key switch B unchanged");
+  same(oa3.wait_values(), wa, N, "repeated Session key switch A unchanged");
   s.close();
 }
 

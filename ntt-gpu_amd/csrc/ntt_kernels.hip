@@ -13,8 +13,8 @@
 // In both passes a thread keeps 16 residues in registers and runs 4 radix-2
 // stages on them, then trades through LDS for the other 4 ("P1" layout:
 // j = u + 16k, "P2" layout: j = 16u + k, u = lane within the sub-transform).
-// The row pass of wide-lazy towers at N = 2^16 instead keeps 8 residues per
-// thread and exchanges within waves (ntt_pass_b8_fwd / ntt_pass_b8_inv).
+// The row pass of wide-lazy and 32-bit towers at N = 2^16 instead keeps 8
+// residues per thread and exchanges within waves (ntt_pass_b8_fwd / _inv).
 // Butterflies for q < 2^61 are wide-lazy (forward residues in [0, 8q),
 // inverse in [0, 4q), a Shoup quotient short by at most 2: modarith.hpp) with
 // twiddles {W, floor(W 2^64/q)} read as one 16-byte load; a canonical variant
@@ -351,13 +351,14 @@ __global__ __launch_bounds__(256) void ntt_pass_b(TowerMap map, const PrimeConst
 }
 
 // ---------------------------------------------------------------- row pass, 8 residues per thread
-// The row pass of the wide-lazy towers (q < 2^61) at R = 256: 32 threads per
-// 256-element row, 8 rows per workgroup and 2 per wave, so every exchange
-// stays inside one wave and needs no workgroup barrier (LDS operations of one
-// wave complete in issue order; the wavefront fences below only keep the
-// compiler from reordering them). With 8 residues a thread needs 62-72 VGPRs,
-// 7-8 waves per SIMD, against 98 VGPRs and 4 waves for 16 residues; measured
-// +3.9% batched KeySwitch/s and an 8.5% faster inverse transform.
+// The row pass of wide-lazy (q < 2^61) and 32-bit (q < 2^30) towers at
+// R = 256: 32 threads per 256-element row, 8 rows per workgroup and 2 per
+// wave, so every exchange stays inside one wave and needs no workgroup
+// barrier (LDS operations of one wave complete in issue order; the wavefront
+// fences below only keep the compiler from reordering them). With 8 residues
+// a 64-bit thread needs 62-72 VGPRs, 7-8 waves per SIMD, against 98 VGPRs and
+// 4 waves for 16 residues. Measured: +3.9% batched KeySwitch/s, inverse
+// transform 9% faster; 32-bit towers 7-8% faster in both directions.
 // Forward layouts of row index j: P1 j = u + 32k; P2 j = (u >> 2) << 5 | k << 2
 // | (u & 3); P3 j = u << 3 | k. Stages 0-2 in P1, 3-5 in P2, 6-7 in P3, then
 // back to P1 for the store (every global access a 256-byte run per row).
@@ -370,7 +371,7 @@ __device__ __forceinline__ int pad16(int j) { return j + (j >> 4); }
 
 template <int A, bool Epi>
 __global__ __launch_bounds__(256) void ntt_pass_b8_fwd(TowerMap map, const PrimeConst *__restrict__ primes, uint32_t R) {
-  __shared__ uint64_t lds[8 * kLdsRow];
+  __shared__ Val<A> lds[8 * kLdsRow];
   uint64_t *data;
   const uint64_t *src;
   uint32_t prime, tk, tpoly;
@@ -382,10 +383,10 @@ __global__ __launch_bounds__(256) void ntt_pass_b8_fwd(TowerMap map, const Prime
   const uint32_t r = blockIdx.x * 8 + rl;
   uint64_t *row = data + (uint64_t)r * kRow;
   const uint32_t rf = R + r;
-  uint64_t *buf = lds + rl * kLdsRow;
-  uint64_t v[8];
+  Val<A> *buf = lds + rl * kLdsRow;
+  Val<A> v[8];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) v[k] = row[u + 32 * k];
+  for (int k = 0; k < 8; ++k) v[k] = (Val<A>)row[u + 32 * k];
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
     const int h = 4 >> s;
@@ -469,7 +470,7 @@ __global__ __launch_bounds__(256) void ntt_pass_b8_fwd(TowerMap map, const Prime
 // j = (u >> 3) << 6 | k << 3 | (u & 7) (distances 8, 16, 32), P1 (64, 128).
 template <int A>
 __global__ __launch_bounds__(256) void ntt_pass_b8_inv(TowerMap map, const PrimeConst *__restrict__ primes, uint32_t R) {
-  __shared__ uint64_t lds[8 * kLdsRow];
+  __shared__ Val<A> lds[8 * kLdsRow];
   uint64_t *data;
   const uint64_t *src;
   uint32_t prime, tk, tpoly;
@@ -482,10 +483,10 @@ __global__ __launch_bounds__(256) void ntt_pass_b8_inv(TowerMap map, const Prime
   uint64_t *row = data + (uint64_t)r * kRow;
   const uint64_t *src_row = src + (uint64_t)r * kRow;
   const uint32_t rf = R + r;
-  uint64_t *buf = lds + rl * kLdsRow;
-  uint64_t v[8];
+  Val<A> *buf = lds + rl * kLdsRow;
+  Val<A> v[8];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) v[k] = src_row[u + 32 * k];
+  for (int k = 0; k < 8; ++k) v[k] = (Val<A>)src_row[u + 32 * k];
 #pragma unroll
   for (int k = 0; k < 8; ++k) buf[pad16(u + 32 * k)] = v[k];
   wave_sync8();
@@ -634,13 +635,13 @@ static hipError_t launch_t(const TowerMap &map, const PrimeConst *primes, uint32
     }
   };
   auto pass_b = [&]() {
-    if constexpr (Inverse && A == kLazy) {
+    if constexpr (Inverse && (A == kLazy || A == kNarrow)) {
       if (R == 256) {
         hipLaunchKernelGGL((ntt_pass_b8_inv<A>), dim3(R / 8, towers), dim3(256), 0, stream, map, primes, R);
         return;
       }
     }
-    if constexpr (!Inverse && A == kLazy) {
+    if constexpr (!Inverse && (A == kLazy || A == kNarrow)) {
       if (R == 256) {
         if (map.epi.sub)
           hipLaunchKernelGGL((ntt_pass_b8_fwd<A, true>), dim3(R / 8, towers), dim3(256), 0, stream, map, primes, R);

@@ -88,7 +88,7 @@ def ntt_roofline(towers, reps, stream, bits=59):
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": measured_traffic(towers) if bits > 30 else None,
-        "kernel": f"forward NTT (ntt_pass_a256 + ntt_pass_b), N=65536, {bits}-bit prime",
+        "kernel": f"forward NTT (ntt_pass_a256 + ntt_pass_b8_fwd), N=65536, {bits}-bit prime",
         "towers_per_launch": towers,
         "launch_ms": round(ms, 4),
         "inverse_launch_ms": round(statistics.mean(inv), 4),

@@ -116,7 +116,14 @@ __host__ __device__ __forceinline__ uint64_t mad_lo(uint64_t x, uint64_t y, uint
   const uint32_t h0 = (uint32_t)qh, h1 = (uint32_t)(qh >> 32), n0 = (uint32_t)nq, n1 = (uint32_t)(nq >> 32);
   uint64_t r = (uint64_t)y0 * w0 + x;
   r = (uint64_t)h0 * n0 + r;
-  const uint32_t hi = (uint32_t)(r >> 32) + y0 * w1 + y1 * w0 + h0 * n1 + h1 * n0;
+  uint32_t hi = (uint32_t)(r >> 32) + y0 * w1 + y1 * w0 + h0 * n1 + h1 * n0;
+#if defined(__HIP_DEVICE_COMPILE__)
+  // Opaque to the optimizer: otherwise it folds the cross products into a
+  // 64-bit add of (sum << 32) onto r (a v_mov of zero and a v_lshl_add_u64)
+  // instead of one 32-bit add into the high word. 2% fewer instructions in
+  // the forward row pass, forward 59-bit transform 1.8% faster.
+  asm("" : "+v"(hi));
+#endif
   return ((uint64_t)hi << 32) | (uint32_t)r;
 }
 

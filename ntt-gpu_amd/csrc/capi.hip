@@ -826,6 +826,8 @@ int ntt_hip_hybrid_rotate(ntt_hip_context *ctx, uint64_t *out_b, uint64_t *out_a
     require(active_q >= 1 && active_q <= towers, "invalid active Q count");
     require((automorphism & 1) && automorphism < 2 * ctx->n, "invalid negacyclic automorphism index");
     std::vector<size_t> pv(p, p + k);
+    // P towers are context towers outside the rotated Q range (the same rule as ntt_hip_correct)
+    for (size_t j : pv) require(j < ctx->q.size() && (j < first || j >= first + active_q), "invalid P tower");
     RotateLaunch r{};
     r.out_b = out_b;
     r.out_a = out_a;
@@ -1199,6 +1201,7 @@ int ntt_hip_relinearize(ntt_hip_keyswitch *const *keys, size_t extra, uint64_t *
     require(c0 && c1 && workspace && (extra == 0 || (keys && higher)), "null argument");
     require(c0 != c1, "ciphertext components must not alias");
     if (extra == 0) return NTT_HIP_OK;
+    require(keys[0] != nullptr, "null relinearization key");
     const ntt_hip_keyswitch &plan = *keys[0];
     require(batch >= 1 && batch <= plan.max_batch, "batch exceeds the plan capacity");
     for (size_t j = 0; j < extra; ++j) {
@@ -1234,6 +1237,8 @@ int ntt_hip_keyswitch_rotate(ntt_hip_keyswitch *const *keys, size_t rotations, c
                              uint64_t *const *out_a, void *workspace, void *stream) {
   return guarded([&] {
     require(keys && automorphisms && c0 && c1 && out_b && out_a && workspace && rotations >= 1, "null argument");
+    require(keys[0] != nullptr, "null rotation key");
+    require(c0 != c1, "ciphertext components must not alias");
     ntt_hip_keyswitch &plan = *keys[0];
     require(batch >= 1 && batch <= plan.max_batch, "batch exceeds the plan capacity");
     for (size_t r = 0; r < rotations; ++r) {
@@ -1242,6 +1247,9 @@ int ntt_hip_keyswitch_rotate(ntt_hip_keyswitch *const *keys, size_t rotations, c
               "rotation keys must share the context and the (L, K, alpha) plan");
       require(k->key_loaded, "rotation key not uploaded");
       require(out_b[r] && out_a[r], "null rotation output");
+      // every rotation re-reads c0 and c1: an output over either would corrupt the later ones
+      require(out_b[r] != out_a[r] && out_b[r] != c0 && out_b[r] != c1 && out_a[r] != c0 && out_a[r] != c1,
+              "rotation outputs must not alias each other or the ciphertext");
       require((automorphisms[r] & 1) && automorphisms[r] < 2 * plan.ctx->n, "invalid negacyclic automorphism index");
     }
     ntt_hip_context &c = *plan.ctx;
@@ -1492,9 +1500,14 @@ int ntt_hip_autotune(ntt_hip_context *ctx, size_t towers, char *report, size_t c
     ntt_hip_context &c = *ctx;
     const size_t n = c.n;
     DeviceGuard g(c.device);
-    DeviceBlob scratch(c.device, towers * n * sizeof(uint64_t) * 2);
-    uint64_t *a = (uint64_t *)scratch.ptr, *b = a + towers * n;
-    HIP_CHECK(hipMemsetAsync(a, 0, towers * n * sizeof(uint64_t) * 2, c.stream));
+    // conversion sweep shape (first half of the basis -> the rest, <= 6 -> <= 24)
+    const size_t cs = std::min<size_t>(6, c.q.size() / 2), ct = std::min<size_t>(24, c.q.size() - cs);
+    const size_t cpolys = cs ? std::max<size_t>(1, std::min(towers / ct, towers / cs)) : 0;
+    // operand a: the NTT's towers and the conversion's sources; b: its targets
+    const size_t a_words = std::max(towers, cpolys * cs) * n, b_words = std::max<size_t>(1, cpolys * ct) * n;
+    DeviceBlob scratch(c.device, (a_words + b_words) * sizeof(uint64_t));
+    uint64_t *a = (uint64_t *)scratch.ptr, *b = a + a_words;
+    HIP_CHECK(hipMemsetAsync(a, 0, (a_words + b_words) * sizeof(uint64_t), c.stream));
     // NTT chunk: every tower uses context tower 0 (one prime, as a batch does)
     std::string rep = "{\"ntt\":{";
     uint32_t best_chunk = 0;
@@ -1516,12 +1529,12 @@ int ntt_hip_autotune(ntt_hip_context *ctx, size_t towers, char *report, size_t c
     // conversion geometry: first half of the basis -> second half (<= 6 -> <= 24)
     uint32_t best_targets = c.conv_targets;
     if (c.q.size() >= 2) {
-      const size_t s = std::min<size_t>(6, c.q.size() / 2), t = std::min<size_t>(24, c.q.size() - s);
+      const size_t s = cs, t = ct;
       std::vector<size_t> src, dst;
       for (size_t i = 0; i < s; ++i) src.push_back(i);
       for (size_t i = 0; i < t; ++i) dst.push_back(s + i);
       auto plan = basis_plan(c, src, dst);
-      const uint32_t polys = (uint32_t)std::max<size_t>(1, std::min(towers / t, towers / s));
+      const uint32_t polys = (uint32_t)cpolys;  // polys * s <= a_words / n, polys * t <= b_words / n
       rep += ",\"conversion\":{";
       best = 1e30f;
       for (uint32_t tpb : {0u, 1u, 2u, 4u, 6u, 8u, 12u, 16u, 24u}) {

@@ -133,11 +133,11 @@ hipError_t launch_basis_switch(const BasisLaunch &in, bool per_term, uint32_t po
   }
   const size_t lds = (size_t)a.plan_host_s * 256 * sizeof(uint64_t);
   if (lds > 64 * 1024) {
-    // up to 40 sources: 80 KiB of the 160 KiB gfx950 LDS
-    (void)hipFuncSetAttribute((const void *)basis_switch_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds);
-    (void)hipFuncSetAttribute((const void *)basis_switch_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds);
+    // up to 40 sources: 80 KiB of the 160 KiB gfx950 LDS; a refused opt-in is
+    // reported here rather than as a failed launch
+    const void *kernel = per_term ? (const void *)basis_switch_kernel<true> : (const void *)basis_switch_kernel<false>;
+    const hipError_t e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
   }
   if (per_term) hipLaunchKernelGGL(basis_switch_kernel<true>, grid, dim3(256), lds, stream, a);
   else hipLaunchKernelGGL(basis_switch_kernel<false>, grid, dim3(256), lds, stream, a);

@@ -51,6 +51,7 @@ struct Request {
   Result result;
   std::string error;
   bool done = false;
+  bool claimed = false;  // a wait() has taken this ticket (a second one is an error)
 };
 
 struct LevelPlan {
@@ -446,9 +447,13 @@ Result Server::wait(uint64_t ticket) {
   std::unique_lock<std::mutex> lock(impl_->mu);
   auto it = impl_->tickets.find(ticket);
   if (it == impl_->tickets.end()) throw std::invalid_argument("unknown ticket");
-  auto r = it->second;
+  // Keep the request alive by its shared pointer and erase by key afterwards:
+  // a submit() on another thread may rehash the map while this thread waits.
+  std::shared_ptr<Request> r = it->second;
+  if (r->claimed) throw std::invalid_argument("ticket already being waited on");
+  r->claimed = true;
   impl_->cv_done.wait(lock, [&] { return r->done; });
-  impl_->tickets.erase(it);
+  impl_->tickets.erase(ticket);
   if (!r->error.empty()) throw std::runtime_error(r->error);
   return std::move(r->result);
 }

@@ -15,8 +15,11 @@
  * q=1e9+7 KATs, bit-reversal KAT, x^(N-1)*x = -1, monomial/inverse-basis
  * closed forms, naive O(N^2) evaluation, the eight pinned OpenFHE (q, psi)
  * pairs as minimum roots, an arbitrary-precision CRT sum for basis
- * conversion, an independent Python big-integer restatement of ModDown and
- * the key-switch MAC). No function here is "parity unpinned".
+ * conversion, independent Python big-integer restatements of ModUp, ModDown
+ * and the key-switch MAC, the automorphism map and rotation against
+ * a(X) -> a(X^k) mod X^N + 1 applied to coefficients and evaluated by the
+ * naive transform, and a decryption check of a genuine key-switching key).
+ * No function here is "parity unpinned".
  */
 #include <omp.h>
 #include <stdint.h>

@@ -92,3 +92,19 @@ def test_null_arguments_rejected():
     assert lib.ntt_hip_keyswitch_run(None, None, 1, None, None, None, None) == hip.EINVAL
     assert lib.ntt_hip_memcpy(0, None, None, 8, hip.H2D, None) == hip.EINVAL
     assert lib.ntt_hip_memcpy(0, ctypes.c_void_p(8), ctypes.c_void_p(8), 8, 7, None) == hip.EINVAL
+
+
+def test_null_keys_rejected_before_any_use():
+    """A null key pointer in the key array is a contract error, not a crash."""
+    lib = hip.lib()
+    keys = (ctypes.c_void_p * 1)(None)
+    one = ctypes.c_void_p(8)
+    higher = (ctypes.c_void_p * 1)(ctypes.c_void_p(24))
+    assert lib.ntt_hip_relinearize(keys, 1, ctypes.c_void_p(8), ctypes.c_void_p(16), higher, 1, one,
+                                   None) == hip.EINVAL
+    assert "null" in lib.ntt_hip_last_error().decode()
+    autos = (ctypes.c_uint32 * 1)(3)
+    outs = (ctypes.c_void_p * 1)(ctypes.c_void_p(32))
+    assert lib.ntt_hip_keyswitch_rotate(keys, 1, autos, ctypes.c_void_p(8), ctypes.c_void_p(16), 1, outs, outs,
+                                        one, None) == hip.EINVAL
+    assert "null" in lib.ntt_hip_last_error().decode()

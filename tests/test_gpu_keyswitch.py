@@ -509,10 +509,34 @@ def test_rotation_rejections():
     assert hp3.q == hp.q and hp3.p == hp.p
     kb3, ka3 = _key(hp3, 2)
     other.upload_key(kb3, ka3, hp3.M, hp3.q)
+    words = 6 * 1024
+    c0, c1, o1, o2 = (hip.DeviceBuffer(words) for _ in range(4))
+    ws = hip.DeviceBuffer(a.rotate_workspace_words(1))
     for keys, autos in (([a], [4]), ([a], [2048]), ([a, other], [5, 5])):  # even, out of range, plan mismatch
         with pytest.raises(hip.ContractError):
-            hip.KeySwitch.rotate(keys, autos, buf.ptr, buf.ptr, 1, [buf.ptr] * len(keys), [buf.ptr] * len(keys),
-                                 buf.ptr)
+            hip.KeySwitch.rotate(keys, autos, c0.ptr, c1.ptr, 1, [o1.ptr] * len(keys), [o2.ptr] * len(keys),
+                                 ws.ptr)
+    # outputs over an input or over each other: every rotation re-reads c0 and c1
+    for ob, oa in ((c0, o2), (o1, c1), (o1, o1), (c1, c0)):
+        with pytest.raises(hip.ContractError, match="alias"):
+            hip.KeySwitch.rotate([a], [5], c0.ptr, c1.ptr, 1, [ob.ptr], [oa.ptr], ws.ptr)
+    with pytest.raises(hip.ContractError, match="alias"):
+        hip.KeySwitch.rotate([a], [5], c0.ptr, c0.ptr, 1, [o1.ptr], [o2.ptr], ws.ptr)
+    # a valid call after the rejections still runs
+    hip.KeySwitch.rotate([a], [5], c0.ptr, c1.ptr, 1, [o1.ptr], [o2.ptr], ws.ptr)
+    ctx.synchronize()
+
+
+def test_hybrid_rotate_rejects_p_towers_inside_q(small):
+    """ntt_hip_hybrid_rotate takes P tower indices of the context: an index past
+    the basis or inside the rotated Q range is a contract error."""
+    hp, ctx = small
+    n, L, K = hp.n, hp.M, hp.K
+    T = L + K
+    bufs = [hip.DeviceBuffer(T * n) for _ in range(5)]
+    for p in ([L + K], [0], [L - 1, L]):
+        with pytest.raises(hip.ContractError, match="P tower"):
+            ctx.hybrid_rotate(bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, bufs[3].ptr, bufs[4].ptr, 0, T, L, p, 5)
 
 
 def _add_towers(a, b, q, n):

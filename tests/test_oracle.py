@@ -310,6 +310,100 @@ def test_mod_down_bigint_restatement():
     assert [int(v) for v in o.mod_down(z, n, q, qr, p, pr)] == want
 
 
+@pytest.mark.parametrize("evaluation", [True, False])
+def test_mod_up_bigint_restatement(evaluation):
+    """ModUp (src/resident.cpp:431-447) restated with Python ints: the Q towers
+    pass through (evaluation input) or are transformed (coefficient input);
+    the P towers are NTT(unreduced CRT lift of the Q coefficients)."""
+    n, L, K = 16, 4, 3
+    q, qr, p, pr = _small_params(n, L, K)
+    rng = random.Random(0x5EED + evaluation)
+    x = [rng.randrange(m) for m in q for _ in range(n)]
+    if evaluation:
+        coeff = []
+        for l, (m, r) in enumerate(zip(q, qr)):
+            coeff += _naive_inverse(x[l * n:(l + 1) * n], m, r)
+        q_part = x
+    else:
+        coeff = x
+        q_part = []
+        for l, (m, r) in enumerate(zip(q, qr)):
+            q_part += _naive_forward(x[l * n:(l + 1) * n], m, r)
+    lift, lifted, product = _bigint_lift(coeff, n, q, p)
+    p_part = []
+    for j, (m, r) in enumerate(zip(p, pr)):
+        p_part += _naive_forward(lift[j * n:(j + 1) * n], m, r)
+    got = [int(v) for v in o.mod_up(x, n, q, qr, p, pr, evaluation=evaluation)]
+    assert got == q_part + p_part
+    # the lift is the unreduced sum: a canonical CRT (reduced mod Q first) differs
+    canonical = []
+    for j, (m, r) in enumerate(zip(p, pr)):
+        canonical += _naive_forward([(v % product) % m for v in lifted], m, r)
+    assert canonical != p_part
+
+
+def _sigma_coefficients(coeffs, k, q, signed=True):
+    """a(X) -> a(X^k) mod (X^n + 1): X^i goes to X^(ik mod 2n), negated when
+    ik mod 2n >= n (the definition the automorphism map must agree with)."""
+    n = len(coeffs)
+    out = [0] * n
+    for i, a in enumerate(coeffs):
+        e = i * k % (2 * n)
+        if e < n:
+            out[e] = (out[e] + a) % q
+        else:
+            out[e - n] = (out[e - n] + (-a if signed else a)) % q
+    return out
+
+
+@pytest.mark.parametrize("n", [8, 16, 64])
+def test_automorphism_matches_definition(n):
+    """or_automorphism_source / or_hybrid_rotate (include/ntt/resident_conversion.hpp:15-29,
+    src/resident.cpp:571-617) against sigma_k applied in the coefficient domain and
+    evaluated by the naive forward transform: for odd k (3, 5, 25 and 2n-1 among them),
+    NTT(a(X^k)) must equal the map applied to NTT(a). A transposed map (k^-1) or a map
+    without the negacyclic sign must fail this check."""
+    L, K = 2, 2
+    q, qr, p, pr = _small_params(n, L, K)
+    rng = random.Random(0xA0 + n)
+    ks = sorted({3, 5, 25 % (2 * n), 2 * n - 1, 2 * n - 3} | {rng.randrange(3, 2 * n, 2) for _ in range(3)})
+    mods, roots = q + p, qr + pr
+    c_b = [[rng.randrange(m) for _ in range(n)] for m in mods]
+    c_a = [[rng.randrange(m) for _ in range(n)] for m in mods]
+    c_0 = [[rng.randrange(m) for _ in range(n)] for m in q]
+    ext_b = [v for t, (m, r) in enumerate(zip(mods, roots)) for v in _naive_forward(c_b[t], m, r)]
+    ext_a = [v for t, (m, r) in enumerate(zip(mods, roots)) for v in _naive_forward(c_a[t], m, r)]
+    c0 = [v for t, (m, r) in enumerate(zip(q, qr)) for v in _naive_forward(c_0[t], m, r)]
+    P = math.prod(p)
+    discriminated = 0
+    for k in ks:
+        got_b, got_a = o.hybrid_rotate(ext_b, ext_a, c0, n, q, p, k)
+        want_b, want_a = [], []
+        for t, (m, r) in enumerate(zip(mods, roots)):
+            # B + P*c0 on the Q towers, B alone on the P towers; A unchanged
+            bc = [(b + (P * c if t < L else 0)) % m for b, c in zip(c_b[t], c_0[t] if t < L else [0] * n)]
+            want_b += _naive_forward(_sigma_coefficients(bc, k, m), m, r)
+            want_a += _naive_forward(_sigma_coefficients(c_a[t], k, m), m, r)
+        assert [int(v) for v in got_b] == want_b, f"k={k}"
+        assert [int(v) for v in got_a] == want_a, f"k={k}"
+        # the address map alone, tower 0 of A: out[j] = in[source(j)]
+        for j in range(n):
+            assert want_a[j] == ext_a[o.automorphism_source(j, n, k)]
+        # plausible mistakes give different outputs
+        kinv = pow(k, -1, 2 * n)
+        m0, r0 = mods[0], roots[0]
+        transposed = _naive_forward(_sigma_coefficients(c_a[0], kinv, m0), m0, r0)
+        unsigned = _naive_forward(_sigma_coefficients(c_a[0], k, m0, signed=False), m0, r0)
+        if kinv != k:
+            assert transposed != want_a[:n]
+            discriminated += 1
+        assert unsigned != want_a[:n]
+    assert discriminated >= 2
+    # k = 1 is the identity
+    ib, ia = o.hybrid_rotate(ext_b, ext_a, [0] * (L * n), n, q, p, 1)
+    assert [int(v) for v in ib] == ext_b and [int(v) for v in ia] == ext_a
+
+
 def test_accumulate_before_mod_down_witness():
     """Down(16)=0, Down(1)=0 but Down(17)=1 (docs/openfhe-hybrid-operation-graph.md:343-348):
     reducing each digit separately differs from reducing the sum."""

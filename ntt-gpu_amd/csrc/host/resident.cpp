@@ -1,15 +1,30 @@
-// Resident operation chain (reference src/resident.cpp:1-662 semantics,
-// include/ntt/resident.hpp interface). Every Session call validates on the
-// host, reserves budget, pins its inputs and then enqueues on the driver's
-// single ordered stream; outputs are fresh immutable allocations. The first
-// transport error poisons the session; pins are released only after the
-// driver proves quiescence, otherwise everything stays quarantined.
+// Resident Session runtime over the Driver seam (public interface
+// include/ntt/resident.hpp; the semantics a caller observes follow the
+// reference's src/resident.cpp:1-662 and docs/resident-operation-chain.md).
+//
+// Structure of this implementation:
+//   Ledger      device/host byte budgets; a Charge is an RAII debit.
+//   Allocation  one device output: its budget charge, record and buffer.
+//   HostStorage host staging (uploads, downloads, constant tables).
+//   Operation   one journal entry: its completion record, its event and a
+//               type-erased pin list (inputs, producers, staging, constants)
+//               that keeps everything the queued work may touch alive.
+//   State       the journal (append-only, bounded by Limits::operations), a
+//               settled prefix (operations proven complete), the first error
+//               and the driver. The single ordered stream makes completion
+//               monotone: when entry k is proven done, entries 1..k are too.
+// Every public operation validates on the host, charges its budgets and
+// records its journal entry before the first driver call, then issues the
+// driver calls in one fixed order (event, output buffers, constant buffer,
+// constant upload, the work itself, event record). The first exception from
+// the driver poisons the session: pending entries become Failed and their
+// pins are released only if the driver proves it is quiescent; otherwise
+// they stay pinned (quarantined) for the life of the process.
 #include <algorithm>
 #include <array>
 #include <atomic>
 #include <limits>
 #include <map>
-#include <unordered_map>
 
 #include "ntt/resident_conversion.hpp"
 #include "ntt/resident_driver.hpp"
@@ -19,255 +34,296 @@ using namespace proposal;
 
 namespace detail {
 
-struct Budget {
+// ---------------------------------------------------------------- budgets
+enum class Pool { Device, Host };
+
+struct Ledger {
   Limits limits;
-  size_t device = 0, host = 0;
+  size_t used[2] = {0, 0};
+  size_t cap(Pool p) const { return p == Pool::Device ? limits.device_bytes : limits.host_bytes; }
 };
 
-// RAII credit against the device or host byte budget.
-struct Reservation {
-  std::shared_ptr<Budget> budget;
-  size_t count;
-  bool host;
-  Reservation(std::shared_ptr<Budget> b, size_t n, bool on_host) : budget(std::move(b)), count(n), host(on_host) {
-    size_t &used = host ? budget->host : budget->device;
-    const size_t limit = host ? budget->limits.host_bytes : budget->limits.device_bytes;
-    require(used <= limit && n <= limit - used, "resident memory budget exhausted");
-    used += n;
+class Charge {
+  std::shared_ptr<Ledger> ledger_;
+  size_t bytes_;
+  Pool pool_;
+
+ public:
+  Charge(std::shared_ptr<Ledger> ledger, size_t bytes, Pool pool)
+      : ledger_(std::move(ledger)), bytes_(bytes), pool_(pool) {
+    size_t &used = ledger_->used[int(pool_)];
+    const size_t cap = ledger_->cap(pool_);
+    require(used <= cap && bytes_ <= cap - used,
+            pool_ == Pool::Device ? "device byte limit of the session exceeded"
+                                  : "host byte limit of the session exceeded");
+    used += bytes_;
   }
-  ~Reservation() { (host ? budget->host : budget->device) -= count; }
-  Reservation(const Reservation &) = delete;
-  Reservation &operator=(const Reservation &) = delete;
+  ~Charge() { ledger_->used[int(pool_)] -= bytes_; }
+  Charge(const Charge &) = delete;
+  Charge &operator=(const Charge &) = delete;
 };
 
+// ---------------------------------------------------------------- resources
 struct Allocation {
-  Reservation reservation;  // declared first: released after the buffer
+  Charge charge;  // first member: released after the buffer below
   AllocationRecord record;
   std::unique_ptr<Buffer> buffer;
-  Allocation(std::shared_ptr<Budget> b, AllocationRecord r)
-      : reservation(std::move(b), r.capacity_bytes, false), record(std::move(r)) {}
+  Allocation(std::shared_ptr<Ledger> ledger, AllocationRecord r)
+      : charge(std::move(ledger), r.capacity_bytes, Pool::Device), record(std::move(r)) {}
+  size_t words() const { return record.capacity_bytes / sizeof(uint64_t); }
 };
 
 struct HostStorage {
-  Reservation reservation;
+  Charge charge;
   std::vector<uint64_t> values;
-  HostStorage(std::shared_ptr<Budget> b, size_t count)
-      : reservation(std::move(b), checked_product(count, sizeof(uint64_t)), true), values(count) {}
+  HostStorage(std::shared_ptr<Ledger> ledger, std::span<const uint64_t> from)
+      : charge(std::move(ledger), checked_product(from.size(), sizeof(uint64_t)), Pool::Host),
+        values(from.begin(), from.end()) {}
+  HostStorage(std::shared_ptr<Ledger> ledger, size_t count)
+      : charge(std::move(ledger), checked_product(count, sizeof(uint64_t)), Pool::Host), values(count) {}
 };
 
-struct DeviceStorage {
-  Reservation reservation;
+// A per-operation device table (basis-conversion plan, correction or
+// rotation scales), uploaded from a HostStorage in the same operation.
+struct ConstantBlock {
+  Charge charge;
   std::unique_ptr<Buffer> buffer;
-  DeviceStorage(std::shared_ptr<Budget> b, size_t count)
-      : reservation(std::move(b), checked_product(count, sizeof(uint64_t)), false) {}
+  ConstantBlock(std::shared_ptr<Ledger> ledger, size_t words)
+      : charge(std::move(ledger), checked_product(words, sizeof(uint64_t)), Pool::Device) {}
 };
 
 struct Operation {
   CompletionRecord record;
   std::unique_ptr<Event> event;
-  std::vector<std::shared_ptr<Allocation>> allocations;
-  std::vector<std::shared_ptr<Operation>> dependencies;
-  std::shared_ptr<HostStorage> staging;
-  std::shared_ptr<DeviceStorage> constants;
+  std::vector<std::shared_ptr<const void>> pins;
   explicit Operation(CompletionRecord r) : record(r) {}
-  void unpin() noexcept {
+  void release() noexcept {
     event.reset();
-    allocations.clear();
-    dependencies.clear();
-    staging.reset();
-    constants.reset();
+    pins.clear();
   }
 };
 
-static uint64_t next_session_id() {
-  static std::atomic<uint64_t> serial{1};
-  uint64_t value = serial.load();
-  do {
-    require(value != std::numeric_limits<uint64_t>::max(), "session identity exhausted");
-  } while (!serial.compare_exchange_weak(value, value + 1));
-  return value;
+uint64_t fresh_session_id() {
+  static std::atomic<uint64_t> counter{0};
+  const uint64_t id = counter.fetch_add(1, std::memory_order_relaxed) + 1;
+  require(id != 0 && id != std::numeric_limits<uint64_t>::max(), "session identifiers exhausted");
+  return id;
 }
+
+// What one enqueue needs besides its body: inputs to pin and order after,
+// fresh outputs, optional host staging and an optional device table filled
+// from that staging.
+struct Request {
+  std::vector<const Polynomial *> inputs;
+  std::vector<std::shared_ptr<Allocation>> outputs;
+  std::shared_ptr<HostStorage> staging;
+  std::shared_ptr<ConstantBlock> table;
+  std::span<const Completion> waits;
+};
 
 struct State : std::enable_shared_from_this<State> {
   std::shared_ptr<Driver> driver;
   PolynomialLayout base;
-  Device device;
-  Stream stream;
-  std::shared_ptr<Budget> budget;
-  Reservation constants;
-  std::unordered_map<uint64_t, std::weak_ptr<Allocation>> registry;
-  std::vector<std::shared_ptr<Operation>> operations;  // bounded journal
-  std::shared_ptr<State> flight_lease;                 // self-pin while work may be queued
-  std::exception_ptr error;
-  std::exception_ptr drain_error = std::make_exception_ptr(std::runtime_error("resident quiescence unproven"));
-  uint64_t allocation_id = 0;
-  size_t uploads = 0, downloads = 0, copies = 0, parameter_uploads = 0;
-  bool closed = false, quiescent = true;
+  Device device{};
+  Stream stream{};
+  std::shared_ptr<Ledger> ledger;
+  std::unique_ptr<Charge> table_charge;  // the driver's own constant tables
+  std::vector<std::shared_ptr<Operation>> journal;
+  size_t settled = 0;             // journal[0, settled) is proven complete (or failed and released)
+  std::shared_ptr<State> anchor;  // self reference while work may be queued
+  std::exception_ptr first_error;
+  uint64_t last_token = 0;
+  size_t uploads = 0, downloads = 0, copies = 0, table_uploads = 0;
+  bool closed = false;
 
-  // Packed constant tables by moduli. Every switch_basis and correct still
-  // stages and uploads its table (the driver sees the same calls), but each
-  // distinct table is built once per session: building a 6 -> 24 basis plan
-  // re-validates the primality of all 30 moduli and costs ~0.2 ms of host
-  // time, which the device otherwise spends idle.
-  static constexpr size_t kTableCache = 256;
-  std::map<std::pair<std::vector<uint64_t>, std::vector<uint64_t>>, std::vector<uint64_t>> basis_tables;
-  std::map<std::vector<uint64_t>, std::vector<uint64_t>> correction_tables;
-  const std::vector<uint64_t> &basis_table(std::vector<uint64_t> from, std::vector<uint64_t> to) {
-    auto key = std::make_pair(std::move(from), std::move(to));
-    auto it = basis_tables.find(key);
-    if (it == basis_tables.end()) {
-      auto packed = pack_basis(ApproximateBasisPlan(key.first, key.second));
-      if (basis_tables.size() >= kTableCache) basis_tables.clear();
-      it = basis_tables.emplace(std::move(key), std::move(packed)).first;
-    }
-    return it->second;
-  }
-  // [q_t; P^-1 mod q_t] for the given Q moduli (reference resident.cpp:404-430).
-  const std::vector<uint64_t> &correction_table(const ntt::proposal::Parameters &params, std::vector<uint64_t> q) {
-    auto it = correction_tables.find(q);
-    if (it == correction_tables.end()) {
-      std::vector<uint64_t> packed = q;
-      for (const uint64_t m : q) packed.push_back(inv_mod(special_prime_product(params, m), m));
-      if (correction_tables.size() >= kTableCache) correction_tables.clear();
-      it = correction_tables.emplace(std::move(q), std::move(packed)).first;
-    }
-    return it->second;
-  }
+  // Each distinct conversion/correction table is packed once per session
+  // (building a 6 -> 24 plan re-validates 30 moduli, ~0.2 ms of host time);
+  // every operation still stages and uploads its own copy.
+  std::map<std::pair<std::vector<uint64_t>, std::vector<uint64_t>>, std::vector<uint64_t>> plan_memo;
+  std::map<std::vector<uint64_t>, std::vector<uint64_t>> correction_memo;
 
-  State(std::shared_ptr<Driver> d, Limits limits)
-      : driver(std::move(d)),
-        base(driver->layout()),
-        device{driver->ordinal(), next_session_id()},
-        stream{device, {1, 1}},
-        budget(std::make_shared<Budget>(Budget{limits})),
-        constants(budget, driver->constant_bytes(), false) {
+  State(std::shared_ptr<Driver> d, Limits limits) : driver(std::move(d)) {
+    base = driver->layout();
     validate(base);
+    require(base.domain == Domain::Coefficient && base.order == Order::Natural,
+            "a session basis is given in coefficient form, natural order");
+    require(base.parameters->n() <= 65536, "ring dimension above 65536 is not supported");
+    require(base.towers.size() <= 65535, "more than 65535 towers is not supported");
+    require(limits.operations > 0, "the operation limit must be positive");
+    device = {driver->ordinal(), fresh_session_id()};
     validate(device);
-    require(base.domain == Domain::Coefficient && base.order == Order::Natural, "session requires coefficient layout");
-    require(base.parameters->n() <= 65536 && base.towers.size() <= 65535, "prototype extent unsupported");
-    require(limits.operations > 0, "empty resident operation budget");
+    stream = {device, {1, 1}};
+    ledger = std::make_shared<Ledger>(Ledger{limits});
+    table_charge = std::make_unique<Charge>(ledger, driver->constant_bytes(), Pool::Device);
     driver->attach_once();
   }
 
-  void open() const { require(!closed && !error, "resident session closed or poisoned"); }
+  bool quiescent() const { return settled == journal.size(); }
+  void require_open() const {
+    require(!closed, "session is closed");
+    require(!first_error, "session is poisoned by an earlier failure");
+  }
 
-  // p is a contiguous run of the session basis with the same parameters.
-  void compatible(const PolynomialLayout &p) const {
+  // ---- layouts
+  // A contiguous run of session towers at no higher level than the session.
+  void within_basis(const PolynomialLayout &p) const {
     validate(p);
-    require(*p.parameters == *base.parameters && p.active_q <= base.active_q, "foreign resident conversion parameters");
-    const auto it = std::find(base.towers.begin(), base.towers.end(), p.towers.front());
-    require(it != base.towers.end() && p.towers.size() <= static_cast<size_t>(base.towers.end() - it) &&
-                std::equal(p.towers.begin(), p.towers.end(), it),
-            "layout is not a contiguous subset of the session basis");
+    require(*p.parameters == *base.parameters, "layout belongs to other parameters");
+    require(p.active_q <= base.active_q, "layout is above the session level");
+    const auto start = std::find(base.towers.begin(), base.towers.end(), p.towers.front());
+    const bool run = start != base.towers.end() &&
+                     p.towers.size() <= size_t(base.towers.end() - start) &&
+                     std::equal(p.towers.begin(), p.towers.end(), start);
+    require(run, "towers are not a contiguous run of the session basis");
+  }
+  // Conversion sessions hold active Q then P; inputs are the whole Q prefix
+  // (towards QP) or the whole QP basis (back to Q, evaluation form).
+  void conversion_layout(const PolynomialLayout &p, bool towards_qp) const {
+    within_basis(p);
+    const Parameters &par = *base.parameters;
+    require(base.towers == qp_basis(par, base.active_q), "session basis is not active Q followed by P");
+    require(p.active_q == base.active_q && p.active_q <= 40 && par.p().size() <= 40,
+            "conversion needs the session level and at most 40 Q and 40 P towers");
+    require(p.towers == (towards_qp ? q_prefix(par, p.active_q) : qp_basis(par, p.active_q)),
+            towards_qp ? "expected the complete active Q basis" : "expected the complete active QP basis");
+    require(towards_qp || p.domain == Domain::Evaluation, "reduction to Q takes evaluation input");
+  }
+  PolynomialLayout basis_slice(size_t first, size_t end) const {
+    PolynomialLayout out = base;
+    out.towers.assign(base.towers.begin() + first, base.towers.begin() + end);
+    return out;
   }
 
-  void conversion_input(const PolynomialLayout &p, bool up) const {
-    compatible(p);
-    require(base.towers == qp_basis(*base.parameters, base.active_q) && p.active_q == base.active_q &&
-                p.active_q <= 40 && base.parameters->p().size() <= 40,
-            "conversion requires a session with active Q then 1..40 P towers");
-    require(p.towers == (up ? q_prefix(*p.parameters, p.active_q) : qp_basis(*p.parameters, p.active_q)),
-            "conversion requires complete active Q or QP in original order");
-    require(up || p.domain == Domain::Evaluation, "ModDown requires evaluation input");
+  // ---- handles
+  void own(const Polynomial &p) const {
+    require(p.state_.get() == this, "polynomial belongs to another session");
+    require(p.allocation_ && p.producer_, "empty polynomial");
+    require_layout(validate_view(p.view_, p.allocation_->record, device), p.layout_);
+    require(p.producer_->record.outcome != Outcome::Failed, "polynomial was produced by a failed operation");
+  }
+  size_t word_offset(const Polynomial &p) const { return byte_offset(p.view_, p.allocation_->record) / 8; }
+  static size_t word_count(const Polynomial &p) { return bytes(p.layout_) / 8; }
+  const Buffer &buffer_of(const Polynomial &p) const { return *p.allocation_->buffer; }
+
+  std::shared_ptr<Allocation> new_output(const PolynomialLayout &layout) {
+    require(last_token != std::numeric_limits<uint64_t>::max(), "allocation identifiers exhausted");
+    return std::make_shared<Allocation>(ledger, AllocationRecord{{++last_token, 1}, device, bytes(layout), layout});
+  }
+  Polynomial handle(const std::shared_ptr<Allocation> &a, const std::shared_ptr<Operation> &op) {
+    const View v{a->record.token, 0, a->record.layout.towers.size(), a->record.layout.active_q};
+    return Polynomial(shared_from_this(), a, op, a->record.layout, v);
   }
 
-  void check(const Polynomial &p) const;
-
-  std::shared_ptr<Allocation> reserve(const PolynomialLayout &layout) {
-    for (auto it = registry.begin(); it != registry.end();) it = it->second.expired() ? registry.erase(it) : ++it;
-    require(allocation_id != std::numeric_limits<uint64_t>::max(), "allocation identity exhausted");
-    auto a = std::make_shared<Allocation>(budget, AllocationRecord{{++allocation_id, 1}, device, bytes(layout), layout});
-    registry.emplace(a->record.token.id, a);
-    return a;
+  // ---- the one submission routine
+  template <class Body>
+  std::shared_ptr<Operation> submit(Request &rq, Body &&body) {
+    require_open();
+    require(journal.size() < ledger->limits.operations, "operation limit of the session reached");
+    auto op = std::make_shared<Operation>(CompletionRecord{{journal.size() + 1, 1}, stream, Outcome::Pending});
+    Execution exec{stream, op->record.operation.id, {}};
+    std::vector<std::shared_ptr<Operation>> after;
+    for (const Completion &w : rq.waits) {
+      require(w.state_.get() == this && w.operation_, "wait on a completion of another session");
+      exec.waits.push_back(w.operation_->record);
+      after.push_back(w.operation_);
+    }
+    for (const Polynomial *in : rq.inputs) after.push_back(in->producer_);
+    for (const auto &dep : after) validate_dependency(dep->record, exec);
+    // every pin and the journal entry exist before the first driver call
+    op->pins.reserve(after.size() + rq.inputs.size() + rq.outputs.size() + 2);
+    for (auto &dep : after) op->pins.push_back(std::move(dep));
+    for (const Polynomial *in : rq.inputs) op->pins.push_back(in->allocation_);
+    for (const auto &out : rq.outputs) op->pins.push_back(out);
+    if (rq.staging) op->pins.push_back(rq.staging);
+    if (rq.table) op->pins.push_back(rq.table);
+    journal.push_back(op);
+    anchor = shared_from_this();
+    try {
+      op->event = driver->make_event();
+      require(bool(op->event), "driver returned no event");
+      for (const auto &out : rq.outputs) {
+        out->buffer = driver->allocate(out->words());
+        require(bool(out->buffer), "driver returned no buffer");
+      }
+      if (rq.table) {
+        rq.table->buffer = driver->allocate(rq.staging->values.size());
+        require(bool(rq.table->buffer), "driver returned no buffer");
+        driver->upload(*rq.table->buffer, rq.staging->values);
+        ++table_uploads;
+      }
+      body();
+      driver->record(*op->event);
+    } catch (...) {
+      poison(std::current_exception());
+      throw;
+    }
+    return op;
   }
 
-  // Journal the operation with every pin in place before any native work.
-  std::shared_ptr<Operation> begin(std::vector<std::shared_ptr<Allocation>> pins,
-                                   std::vector<std::shared_ptr<Operation>> sources, std::span<const Completion> waits,
-                                   std::shared_ptr<HostStorage> staging = {});
-
-  void finish(const std::shared_ptr<Operation> &op) { driver->record(*op->event); }
-
+  // ---- completion
+  // Release every entry up to and including index `last` (0-based).
+  void settle_through(size_t last) noexcept {
+    for (; settled <= last && settled < journal.size(); ++settled) {
+      Operation &e = *journal[settled];
+      if (e.record.outcome == Outcome::Pending) e.record.outcome = Outcome::Succeeded;
+      e.release();
+    }
+    if (quiescent()) anchor.reset();
+  }
+  // Drain after a failure or at close: only a quiescent driver lets pins go.
   bool drain() noexcept {
     if (!driver->quiesce()) return false;
-    quiescent = true;
-    for (auto &op : operations) {
-      if (op->record.outcome == Outcome::Pending) op->record.outcome = error ? Outcome::Failed : Outcome::Succeeded;
-      op->unpin();
+    for (; settled < journal.size(); ++settled) {
+      Operation &e = *journal[settled];
+      if (e.record.outcome == Outcome::Pending) e.record.outcome = first_error ? Outcome::Failed : Outcome::Succeeded;
+      e.release();
     }
-    flight_lease.reset();
+    anchor.reset();
     return true;
   }
-
-  void fail(std::exception_ptr reason) noexcept {
-    if (!error) error = reason;
-    for (auto &op : operations)
-      if (op->record.outcome == Outcome::Pending) op->record.outcome = Outcome::Failed;
-    (void)drain();  // false keeps the journal, staging, buffers and self lease
+  void poison(std::exception_ptr why) noexcept {
+    if (!first_error) first_error = why;
+    for (size_t i = settled; i < journal.size(); ++i)
+      if (journal[i]->record.outcome == Outcome::Pending) journal[i]->record.outcome = Outcome::Failed;
+    (void)drain();  // false: everything stays pinned (quarantine)
   }
-
-  // Completion of op retires it and everything submitted before it.
-  void poll(const std::shared_ptr<Operation> &op, bool wait) {
-    if (op->record.outcome == Outcome::Failed) std::rethrow_exception(error);
-    if (op->record.outcome == Outcome::Succeeded) return;
+  void await(const std::shared_ptr<Operation> &op, bool block) {
+    if (op->record.outcome == Outcome::Failed) std::rethrow_exception(first_error);
+    const size_t index = op->record.operation.id - 1;
+    if (op->record.outcome == Outcome::Succeeded || index < settled) return;
     try {
-      if (wait) driver->wait(*op->event);
+      if (block) driver->wait(*op->event);
       else if (!driver->ready(*op->event)) return;
-      for (auto &prior : operations) {
-        if (prior->record.operation.id > op->record.operation.id) break;
-        if (prior->record.outcome == Outcome::Pending) prior->record.outcome = Outcome::Succeeded;
-        prior->unpin();
-      }
-      if (op == operations.back()) {
-        quiescent = true;
-        flight_lease.reset();
-      }
+      settle_through(index);
     } catch (...) {
-      fail(std::current_exception());
+      poison(std::current_exception());
       throw;
     }
   }
 
-  size_t offset(const Polynomial &p) const;
-  Polynomial output(const std::shared_ptr<Allocation> &a, const std::shared_ptr<Operation> &op);
-
-  // Event, output buffer, enqueue and record under one failure guard.
-  template <class Enqueue>
-  Polynomial submit_output(const std::shared_ptr<Allocation> &a, const std::shared_ptr<Operation> &op,
-                           Enqueue enqueue) {
-    try {
-      op->event = driver->make_event();
-      a->buffer = driver->allocate(a->record.capacity_bytes / sizeof(uint64_t));
-      require(bool(a->buffer) && bool(op->event), "driver returned null resource");
-      enqueue(*a->buffer);
-      finish(op);
-      return output(a, op);
-    } catch (...) {
-      fail(std::current_exception());
-      throw;
-    }
+  // ---- constant tables
+  const std::vector<uint64_t> &plan_table(std::vector<uint64_t> from, std::vector<uint64_t> to) {
+    auto key = std::make_pair(std::move(from), std::move(to));
+    auto hit = plan_memo.find(key);
+    if (hit != plan_memo.end()) return hit->second;
+    if (plan_memo.size() >= 256) plan_memo.clear();
+    std::vector<uint64_t> packed = pack_basis(ApproximateBasisPlan(key.first, key.second));
+    return plan_memo.emplace(std::move(key), std::move(packed)).first->second;
   }
-
-  // Shared body of operations that upload a small constant table first
-  // (switch_basis, correct): event, outputs, table buffer, table upload.
-  template <class Enqueue>
-  void submit_with_constants(const std::shared_ptr<Operation> &op, const std::shared_ptr<DeviceStorage> &table,
-                             std::initializer_list<std::shared_ptr<Allocation>> outputs, Enqueue enqueue) {
-    try {
-      op->event = driver->make_event();
-      for (const auto &a : outputs) a->buffer = driver->allocate(a->record.capacity_bytes / sizeof(uint64_t));
-      table->buffer = driver->allocate(op->staging->values.size());
-      bool ok = bool(table->buffer) && bool(op->event);
-      for (const auto &a : outputs) ok = ok && bool(a->buffer);
-      require(ok, "driver returned null resource");
-      driver->upload(*table->buffer, op->staging->values);
-      ++parameter_uploads;
-      enqueue(*table->buffer);
-      finish(op);
-    } catch (...) {
-      fail(std::current_exception());
-      throw;
-    }
+  // [q_t ...; P^-1 mod q_t ...]
+  const std::vector<uint64_t> &correction_table(std::vector<uint64_t> moduli) {
+    auto hit = correction_memo.find(moduli);
+    if (hit != correction_memo.end()) return hit->second;
+    if (correction_memo.size() >= 256) correction_memo.clear();
+    std::vector<uint64_t> packed = moduli;
+    for (const uint64_t m : moduli) packed.push_back(inv_mod(special_prime_product(*base.parameters, m), m));
+    return correction_memo.emplace(std::move(moduli), std::move(packed)).first->second;
+  }
+  std::vector<uint64_t> moduli_of(const PolynomialLayout &p) const {
+    std::vector<uint64_t> out;
+    out.reserve(p.towers.size());
+    for (const TowerRef t : p.towers) out.push_back(p.parameters->tower(t).q);
+    return out;
   }
 };
 
@@ -280,8 +336,9 @@ Completion::Completion(std::shared_ptr<detail::State> s, std::shared_ptr<detail:
 CompletionRecord Completion::query() const {
   require(state_ && operation_, "empty completion");
   try {
-    state_->poll(operation_, false);
+    state_->await(operation_, false);
   } catch (...) {
+    // a failed operation reports its record; any other error propagates
     if (operation_->record.outcome != Outcome::Failed) throw;
   }
   return operation_->record;
@@ -289,7 +346,7 @@ CompletionRecord Completion::query() const {
 
 void Completion::wait() const {
   require(state_ && operation_, "empty completion");
-  state_->poll(operation_, true);
+  state_->await(operation_, true);
 }
 
 Polynomial::Polynomial(std::shared_ptr<detail::State> s, std::shared_ptr<detail::Allocation> a,
@@ -318,454 +375,400 @@ std::span<const uint64_t> Download::wait_values() const {
   if (!validated_) {
     try {
       validate_host_values(layout_, storage_->values);
-      validated_ = true;
     } catch (...) {
+      // a non-canonical result is a transport failure of this operation
       operation_->record.outcome = Outcome::Failed;
-      state_->fail(std::current_exception());
+      state_->poison(std::current_exception());
       throw;
     }
+    validated_ = true;
   }
   return storage_->values;
 }
 
-// ---------------------------------------------------------------- State helpers needing complete types
-namespace detail {
-
-void State::check(const Polynomial &p) const {
-  require(p.state_.get() == this && p.allocation_ && p.producer_, "foreign or empty polynomial");
-  const auto it = registry.find(p.view_.allocation.id);
-  require(it != registry.end() && it->second.lock() == p.allocation_, "unregistered allocation");
-  require_layout(validate_view(p.view_, p.allocation_->record, device), p.layout_);
-  require(p.producer_->record.outcome != Outcome::Failed, "failed producer");
-}
-
-std::shared_ptr<Operation> State::begin(std::vector<std::shared_ptr<Allocation>> pins,
-                                        std::vector<std::shared_ptr<Operation>> sources,
-                                        std::span<const Completion> waits, std::shared_ptr<HostStorage> staging) {
-  open();
-  require(operations.size() < budget->limits.operations, "resident operation budget exhausted");
-  auto op = std::make_shared<Operation>(
-      CompletionRecord{{operations.size() + 1, 1}, stream, Outcome::Pending});
-  Execution execution{stream, op->record.operation.id, {}};
-  for (const Completion &w : waits) {
-    require(w.state_.get() == this && w.operation_, "foreign or empty dependency");
-    execution.waits.push_back(w.operation_->record);
-    sources.push_back(w.operation_);
-  }
-  for (const auto &source : sources) validate_dependency(source->record, execution);
-  op->allocations = std::move(pins);
-  op->dependencies = std::move(sources);
-  op->staging = std::move(staging);
-  operations.push_back(op);
-  flight_lease = shared_from_this();
-  quiescent = false;
-  return op;
-}
-
-size_t State::offset(const Polynomial &p) const {
-  return byte_offset(p.view_, p.allocation_->record) / sizeof(uint64_t);
-}
-
-Polynomial State::output(const std::shared_ptr<Allocation> &a, const std::shared_ptr<Operation> &op) {
-  const auto &rec = a->record;
-  return Polynomial(shared_from_this(), a, op, rec.layout, {rec.token, 0, rec.layout.towers.size(), rec.layout.active_q});
-}
-
-}  // namespace detail
-
-// ---------------------------------------------------------------- Session
+// ---------------------------------------------------------------- session
 Session::Session(std::shared_ptr<detail::Driver> driver, Limits limits) {
-  require(bool(driver), "missing resident driver");
+  require(bool(driver), "a session needs a driver");
   state_ = std::make_shared<detail::State>(std::move(driver), limits);
 }
 
 Session::~Session() noexcept {
-  if (state_) try {
-      close();
-    } catch (...) {
-    }
+  if (!state_) return;
+  try {
+    close();
+  } catch (...) {
+  }
 }
 
 detail::State &Session::open_state() {
-  require(bool(state_), "moved session");
-  state_->open();
+  require(bool(state_), "session was moved from");
+  state_->require_open();
   return *state_;
 }
 
 Device Session::device() const {
-  require(bool(state_), "moved session");
+  require(bool(state_), "session was moved from");
   return state_->device;
 }
 
 Stats Session::stats() const {
-  require(bool(state_), "moved session");
-  const auto &s = *state_;
+  require(bool(state_), "session was moved from");
+  const detail::State &s = *state_;
   size_t pending = 0;
-  for (const auto &op : s.operations) pending += op->record.outcome == Outcome::Pending;
-  return {s.operations.size(), pending,     s.budget->device, s.budget->host, s.uploads, s.downloads,
-          s.copies,            s.parameter_uploads, s.closed, bool(s.error), s.quiescent};
+  for (size_t i = s.settled; i < s.journal.size(); ++i) pending += s.journal[i]->record.outcome == Outcome::Pending;
+  return {s.journal.size(), pending, s.ledger->used[0], s.ledger->used[1], s.uploads, s.downloads, s.copies,
+          s.table_uploads, s.closed, bool(s.first_error), s.quiescent()};
 }
 
 std::vector<CompletionRecord> Session::history() const {
-  require(bool(state_), "moved session");
+  require(bool(state_), "session was moved from");
   std::vector<CompletionRecord> out;
-  for (const auto &op : state_->operations) out.push_back(op->record);
+  out.reserve(state_->journal.size());
+  for (const auto &e : state_->journal) out.push_back(e->record);
   return out;
 }
 
 Polynomial Session::upload(std::span<const uint64_t> values, std::span<const Completion> waits) {
-  auto &s = open_state();
+  detail::State &s = open_state();
   validate_host_values(s.base, values);
   return upload_checked(s.base, values, waits);
 }
 
-Polynomial Session::upload_evaluation(std::span<const uint64_t> values, std::span<const Completion> waits) {
-  auto &s = open_state();
-  const auto layout = transform_layout(s.base, false);
+Polynomial Session::upload(const PolynomialLayout &layout, std::span<const uint64_t> values,
+                           std::span<const Completion> waits) {
+  detail::State &s = open_state();
+  s.within_basis(layout);
   validate_host_values(layout, values);
   return upload_checked(layout, values, waits);
 }
 
-Polynomial Session::upload(const PolynomialLayout &p, std::span<const uint64_t> values,
-                           std::span<const Completion> waits) {
-  auto &s = open_state();
-  s.compatible(p);
-  validate_host_values(p, values);
-  return upload_checked(p, values, waits);
+Polynomial Session::upload_evaluation(std::span<const uint64_t> values, std::span<const Completion> waits) {
+  detail::State &s = open_state();
+  const PolynomialLayout layout = transform_layout(s.base, false);
+  validate_host_values(layout, values);
+  return upload_checked(layout, values, waits);
 }
 
 Polynomial Session::upload_checked(const PolynomialLayout &layout, std::span<const uint64_t> values,
                                    std::span<const Completion> waits) {
-  auto &s = *state_;
-  // The session owns a staging copy: the caller's memory is never borrowed.
-  auto staging = std::make_shared<detail::HostStorage>(s.budget, values.size());
-  std::copy(values.begin(), values.end(), staging->values.begin());
-  auto a = s.reserve(layout);
-  auto op = s.begin({a}, {}, waits, staging);
-  return s.submit_output(a, op, [&](detail::Buffer &out) {
-    s.driver->upload(out, staging->values);
+  detail::State &s = *state_;
+  detail::Request rq;
+  rq.staging = std::make_shared<detail::HostStorage>(s.ledger, values);
+  rq.outputs = {s.new_output(layout)};
+  rq.waits = waits;
+  auto op = s.submit(rq, [&] {
+    s.driver->upload(*rq.outputs[0]->buffer, rq.staging->values);
     ++s.uploads;
   });
+  return s.handle(rq.outputs[0], op);
 }
 
 Polynomial Session::transform(const Polynomial &input, bool inverse, std::span<const Completion> waits) {
-  auto &s = open_state();
-  s.check(input);
-  auto a = s.reserve(transform_layout(input.layout_, inverse));
-  auto op = s.begin({input.allocation_, a}, {input.producer_}, waits);
-  return s.submit_output(a, op, [&](detail::Buffer &out) {
-    s.driver->copy(out, *input.allocation_->buffer, s.offset(input), bytes(input.layout_) / sizeof(uint64_t));
+  detail::State &s = open_state();
+  s.own(input);
+  const PolynomialLayout layout = transform_layout(input.layout_, inverse);
+  detail::Request rq;
+  rq.inputs = {&input};
+  rq.outputs = {s.new_output(layout)};
+  rq.waits = waits;
+  auto op = s.submit(rq, [&] {
+    // outputs are fresh: copy the input, then transform the copy in place
+    detail::Buffer &out = *rq.outputs[0]->buffer;
+    s.driver->copy(out, s.buffer_of(input), s.word_offset(input), s.word_count(input));
     ++s.copies;
-    s.driver->transform(out, a->record.layout, inverse);
+    s.driver->transform(out, layout, inverse);
   });
+  return s.handle(rq.outputs[0], op);
 }
 
 Polynomial Session::forward(const Polynomial &p, std::span<const Completion> waits) { return transform(p, false, waits); }
 Polynomial Session::inverse(const Polynomial &p, std::span<const Completion> waits) { return transform(p, true, waits); }
 
 Polynomial Session::pointwise(const Polynomial &left, const Polynomial &right, std::span<const Completion> waits) {
-  auto &s = open_state();
-  s.check(left);
-  s.check(right);
-  auto a = s.reserve(pointwise_layout(left.layout_, right.layout_));
-  auto op = s.begin({left.allocation_, right.allocation_, a}, {left.producer_, right.producer_}, waits);
-  return s.submit_output(a, op, [&](detail::Buffer &out) {
-    s.driver->copy(out, *left.allocation_->buffer, s.offset(left), bytes(left.layout_) / sizeof(uint64_t));
+  detail::State &s = open_state();
+  s.own(left);
+  s.own(right);
+  const PolynomialLayout layout = pointwise_layout(left.layout_, right.layout_);
+  detail::Request rq;
+  rq.inputs = {&left, &right};
+  rq.outputs = {s.new_output(layout)};
+  rq.waits = waits;
+  auto op = s.submit(rq, [&] {
+    detail::Buffer &out = *rq.outputs[0]->buffer;
+    s.driver->copy(out, s.buffer_of(left), s.word_offset(left), s.word_count(left));
     ++s.copies;
-    s.driver->pointwise(out, *right.allocation_->buffer, s.offset(right), a->record.layout);
+    s.driver->pointwise(out, s.buffer_of(right), s.word_offset(right), layout);
   });
+  return s.handle(rq.outputs[0], op);
 }
 
 Polynomial Session::switch_basis(const Polynomial &input, const PolynomialLayout &target,
                                  std::span<const Completion> waits) {
-  auto &s = open_state();
-  s.check(input);
-  s.compatible(input.layout_);
-  s.compatible(target);
-  require(input.layout_.domain == Domain::Coefficient && target.domain == Domain::Coefficient &&
-              input.layout_.active_q == target.active_q,
-          "basis switching requires coefficient layouts at the same active level");
-  std::vector<uint64_t> from, to;
-  for (const TowerRef t : input.layout_.towers) from.push_back(target.parameters->tower(t).q);
-  for (const TowerRef t : target.towers) to.push_back(target.parameters->tower(t).q);
-  const auto &packed = s.basis_table(std::move(from), std::move(to));
-  auto staging = std::make_shared<detail::HostStorage>(s.budget, packed.size());
-  std::copy(packed.begin(), packed.end(), staging->values.begin());
-  auto table = std::make_shared<detail::DeviceStorage>(s.budget, packed.size());
-  auto a = s.reserve(target);
-  auto op = s.begin({input.allocation_, a}, {input.producer_}, waits, staging);
-  op->constants = table;  // pinned before the first possible enqueue
-  s.submit_with_constants(op, table, {a}, [&](detail::Buffer &constants) {
-    s.driver->switch_basis(*a->buffer, *input.allocation_->buffer, s.offset(input), constants, input.layout_, target);
+  detail::State &s = open_state();
+  s.own(input);
+  s.within_basis(input.layout_);
+  s.within_basis(target);
+  require(input.layout_.domain == Domain::Coefficient && target.domain == Domain::Coefficient,
+          "basis switching works on coefficient layouts");
+  require(input.layout_.active_q == target.active_q, "basis switching keeps the level");
+  const std::vector<uint64_t> &table = s.plan_table(s.moduli_of(input.layout_), s.moduli_of(target));
+  detail::Request rq;
+  rq.inputs = {&input};
+  rq.staging = std::make_shared<detail::HostStorage>(s.ledger, table);
+  rq.table = std::make_shared<detail::ConstantBlock>(s.ledger, table.size());
+  rq.outputs = {s.new_output(target)};
+  rq.waits = waits;
+  auto op = s.submit(rq, [&] {
+    s.driver->switch_basis(*rq.outputs[0]->buffer, s.buffer_of(input), s.word_offset(input), *rq.table->buffer,
+                           input.layout_, target);
   });
-  return s.output(a, op);
+  return s.handle(rq.outputs[0], op);
 }
 
+// Concatenation of whole-tower parts into one fresh allocation (the digits of
+// Hybrid key switching and the QP output of ModUp).
 Polynomial Session::join(std::span<const Polynomial> parts) {
-  auto &s = open_state();
-  require(!parts.empty(), "empty join");
-  PolynomialLayout layout = parts.front().layout_;
+  detail::State &s = open_state();
+  require(!parts.empty(), "nothing to join");
+  PolynomialLayout layout = parts[0].layout_;
   layout.towers.clear();
-  std::vector<std::shared_ptr<detail::Allocation>> pins;
-  std::vector<std::shared_ptr<detail::Operation>> producers;
+  detail::Request rq;
   for (const Polynomial &part : parts) {
-    s.check(part);
-    require(*layout.parameters == *part.layout_.parameters && layout.active_q == part.layout_.active_q &&
-                layout.domain == part.layout_.domain && layout.order == part.layout_.order,
-            "incompatible join layouts");
-    layout.towers.insert(layout.towers.end(), part.layout_.towers.begin(), part.layout_.towers.end());
-    pins.push_back(part.allocation_);
-    producers.push_back(part.producer_);
+    s.own(part);
+    const PolynomialLayout &l = part.layout_;
+    require(*l.parameters == *layout.parameters && l.active_q == layout.active_q && l.domain == layout.domain &&
+                l.order == layout.order,
+            "joined parts differ in parameters, level or form");
+    layout.towers.insert(layout.towers.end(), l.towers.begin(), l.towers.end());
+    rq.inputs.push_back(&part);
   }
-  s.compatible(layout);
-  auto a = s.reserve(layout);
-  pins.push_back(a);
-  auto op = s.begin(std::move(pins), std::move(producers), {});
-  return s.submit_output(a, op, [&](detail::Buffer &out) {
+  s.within_basis(layout);
+  rq.outputs = {s.new_output(layout)};
+  auto op = s.submit(rq, [&] {
     size_t at = 0;
     for (const Polynomial &part : parts) {
-      const size_t count = bytes(part.layout_) / sizeof(uint64_t);
-      s.driver->copy_range(out, at, *part.allocation_->buffer, s.offset(part), count);
-      at += count;
+      const size_t words = s.word_count(part);
+      s.driver->copy_range(*rq.outputs[0]->buffer, at, s.buffer_of(part), s.word_offset(part), words);
+      at += words;
       ++s.copies;
     }
   });
+  return s.handle(rq.outputs[0], op);
 }
 
+// (left - right) * P^-1 per Q tower (ModDown's last step).
 Polynomial Session::correct(const Polynomial &left, const Polynomial &right) {
-  auto &s = open_state();
-  s.check(left);
-  s.check(right);
-  const auto layout = pointwise_layout(left.layout_, right.layout_);
-  std::vector<uint64_t> moduli;
-  for (const TowerRef t : layout.towers) moduli.push_back(layout.parameters->tower(t).q);
-  const auto &packed = s.correction_table(*layout.parameters, std::move(moduli));
-  auto staging = std::make_shared<detail::HostStorage>(s.budget, packed.size());
-  std::copy(packed.begin(), packed.end(), staging->values.begin());
-  auto table = std::make_shared<detail::DeviceStorage>(s.budget, packed.size());
-  auto a = s.reserve(layout);
-  auto op = s.begin({left.allocation_, right.allocation_, a}, {left.producer_, right.producer_}, {}, staging);
-  op->constants = table;
-  s.submit_with_constants(op, table, {a}, [&](detail::Buffer &constants) {
-    s.driver->correct(*a->buffer, *left.allocation_->buffer, s.offset(left), *right.allocation_->buffer,
-                      s.offset(right), constants, layout);
+  detail::State &s = open_state();
+  s.own(left);
+  s.own(right);
+  const PolynomialLayout layout = pointwise_layout(left.layout_, right.layout_);
+  const std::vector<uint64_t> &table = s.correction_table(s.moduli_of(layout));
+  detail::Request rq;
+  rq.inputs = {&left, &right};
+  rq.staging = std::make_shared<detail::HostStorage>(s.ledger, table);
+  rq.table = std::make_shared<detail::ConstantBlock>(s.ledger, table.size());
+  rq.outputs = {s.new_output(layout)};
+  auto op = s.submit(rq, [&] {
+    s.driver->correct(*rq.outputs[0]->buffer, s.buffer_of(left), s.word_offset(left), s.buffer_of(right),
+                      s.word_offset(right), *rq.table->buffer, layout);
   });
-  return s.output(a, op);
+  return s.handle(rq.outputs[0], op);
 }
 
 Polynomial Session::mod_up(const Polynomial &input, std::span<const Completion> waits) {
-  auto &s = open_state();
-  s.check(input);
-  s.conversion_input(input.layout_, true);
-  PolynomialLayout p_layout = s.base;
-  p_layout.towers.erase(p_layout.towers.begin(), p_layout.towers.begin() + s.base.active_q);
-  Polynomial coefficients, evaluations;
-  if (input.layout_.domain == Domain::Coefficient) {
-    coefficients = input;
-    evaluations = forward(input, waits);
-  } else {
-    evaluations = input;
-    coefficients = inverse(input, waits);
-  }
-  const Polynomial p_coefficients = switch_basis(coefficients, p_layout);
-  // Built from named handles: g++ < 13 leaks already-constructed elements of
-  // an aggregate whose initializer throws (GCC bug 66139).
-  const Polynomial p_evaluations = forward(p_coefficients);
-  const std::array<Polynomial, 2> parts{evaluations, p_evaluations};
-  return join(parts);
+  detail::State &s = open_state();
+  s.own(input);
+  s.conversion_layout(input.layout_, true);
+  const size_t L = s.base.active_q;
+  const bool coefficient_input = input.layout_.domain == Domain::Coefficient;
+  // the Q part of the result in evaluation form, the conversion source in coefficients
+  const Polynomial q_eval = coefficient_input ? forward(input, waits) : input;
+  const Polynomial q_coeff = coefficient_input ? input : inverse(input, waits);
+  const Polynomial p_coeff = switch_basis(q_coeff, s.basis_slice(L, s.base.towers.size()));
+  const Polynomial p_eval = forward(p_coeff);
+  const std::array<Polynomial, 2> halves{q_eval, p_eval};
+  return join(halves);
 }
 
 Polynomial Session::mod_down(const Polynomial &input, std::span<const Completion> waits) {
-  auto &s = open_state();
-  s.check(input);
-  s.conversion_input(input.layout_, false);
-  const size_t L = input.layout_.active_q;
-  const Polynomial q = subview(input, 0, L, L);
-  const Polynomial p = subview(input, L, input.layout_.parameters->p().size(), L);
-  const auto q_coefficient = transform_layout(q.layout_, true);
-  const Polynomial correction = forward(switch_basis(inverse(p, waits), q_coefficient));
-  return correct(q, correction);
+  detail::State &s = open_state();
+  s.own(input);
+  s.conversion_layout(input.layout_, false);
+  const size_t L = input.layout_.active_q, K = input.layout_.parameters->p().size();
+  const Polynomial q_part = subview(input, 0, L, L);
+  const Polynomial p_part = subview(input, L, K, L);
+  const Polynomial p_coeff = inverse(p_part, waits);
+  const Polynomial lifted = switch_basis(p_coeff, transform_layout(q_part.layout_, true));
+  const Polynomial lifted_eval = forward(lifted);
+  return correct(q_part, lifted_eval);
 }
 
 std::vector<Polynomial> Session::hybrid_digits(const Polynomial &input, std::span<const Completion> waits) {
-  auto &s = open_state();
-  s.check(input);
-  s.conversion_input(input.layout_, true);
-  require(input.layout_.domain == Domain::Evaluation, "Hybrid digits require evaluation Q input");
-  const size_t L = input.layout_.active_q, alpha = s.base.parameters->alpha();
+  detail::State &s = open_state();
+  s.own(input);
+  s.conversion_layout(input.layout_, true);
+  require(input.layout_.domain == Domain::Evaluation, "Hybrid digits take evaluation Q input");
+  const size_t L = input.layout_.active_q, alpha = s.base.parameters->alpha(), T = s.base.towers.size();
   std::vector<Polynomial> digits;
-  digits.reserve(1 + (L - 1) / alpha);
   for (size_t first = 0; first < L; first += alpha) {
-    const size_t count = std::min(alpha, L - first);
-    const Polynomial evaluations = subview(input, first, count, L);
-    const Polynomial coefficients = inverse(evaluations, waits);
+    const size_t end = std::min(L, first + alpha);
+    const Polynomial own_eval = subview(input, first, end - first, L);
+    const Polynomial own_coeff = inverse(own_eval, waits);
     std::vector<Polynomial> parts;
-    // Complement targets in basis order, at most 40 per conversion plan.
-    auto extend = [&](size_t begin, size_t end) {
-      while (begin < end) {
-        const size_t next = begin + std::min<size_t>(40, end - begin);
-        PolynomialLayout target = s.base;
-        target.towers.assign(s.base.towers.begin() + begin, s.base.towers.begin() + next);
-        parts.push_back(forward(switch_basis(coefficients, target)));
-        begin = next;
+    // towers outside the digit, converted in runs of at most 40 targets
+    auto extend = [&](size_t from, size_t to) {
+      for (size_t lo = from; lo < to; lo += 40) {
+        const size_t hi = std::min(to, lo + 40);
+        const Polynomial converted = switch_basis(own_coeff, s.basis_slice(lo, hi));
+        parts.push_back(forward(converted));
       }
     };
     extend(0, first);
-    parts.push_back(evaluations);
-    extend(first + count, s.base.towers.size());
+    parts.push_back(own_eval);
+    extend(end, T);
     digits.push_back(join(parts));
   }
   return digits;
 }
 
 HybridKey Session::upload_hybrid_key(const PolynomialLayout &full_key_layout, std::span<const HybridKeyDigit> digits) {
-  auto &s = open_state();
-  const auto active = transform_layout(s.base, false);
-  s.conversion_input(active, false);
+  detail::State &s = open_state();
+  const PolynomialLayout active = transform_layout(s.base, false);
+  s.conversion_layout(active, false);
   validate(full_key_layout);
-  const Parameters &params = *s.base.parameters;
-  const size_t M = params.q().size(), alpha = params.alpha();
-  require(*full_key_layout.parameters == params && full_key_layout.active_q == M &&
-              full_key_layout.towers == qp_basis(params, M) && full_key_layout.domain == Domain::Evaluation,
-          "Hybrid keys require full-context evaluation QP");
-  require(digits.size() == 1 + (M - 1) / alpha, "wrong full-context Hybrid key digit count");
-  for (const HybridKeyDigit &d : digits) {  // every row is checked before any submission
+  const Parameters &par = *s.base.parameters;
+  const size_t M = par.q().size(), alpha = par.alpha();
+  require(*full_key_layout.parameters == par && full_key_layout.active_q == M &&
+              full_key_layout.towers == qp_basis(par, M) && full_key_layout.domain == Domain::Evaluation,
+          "a Hybrid key is given in evaluation form over the full QP basis");
+  require(digits.size() == (M + alpha - 1) / alpha, "a Hybrid key has one digit per alpha full-context Q towers");
+  for (const HybridKeyDigit &d : digits) {
     validate_host_values(full_key_layout, d.b);
     validate_host_values(full_key_layout, d.a);
   }
-  const auto rows = key_indices(params, active.active_q);
-  const size_t active_digits = 1 + (active.active_q - 1) / alpha, n = params.n();
-  std::vector<uint64_t> packed(rows.size() * n);
-  auto upload_rows = [&](std::span<const uint64_t> values) {
-    for (size_t t = 0; t < rows.size(); ++t) std::copy_n(values.begin() + rows[t] * n, n, packed.begin() + t * n);
-    return upload_checked(active, packed, {});
+  // rows of the active towers: i for Q, i + (M - L) for P
+  const std::vector<size_t> rows = key_indices(par, active.active_q);
+  const size_t n = par.n(), used = (active.active_q + alpha - 1) / alpha;
+  std::vector<uint64_t> gathered(rows.size() * n);
+  auto channel = [&](std::span<const uint64_t> full) {
+    for (size_t t = 0; t < rows.size(); ++t)
+      std::copy_n(full.begin() + rows[t] * n, n, gathered.begin() + t * n);
+    return upload_checked(active, gathered, {});
   };
   HybridKey key;
-  for (const HybridKeyDigit &d : digits.first(active_digits)) {
-    key.b_.push_back(upload_rows(d.b));
-    key.a_.push_back(upload_rows(d.a));
+  for (size_t d = 0; d < used; ++d) {
+    key.b_.push_back(channel(digits[d].b));
+    key.a_.push_back(channel(digits[d].a));
   }
   return key;
 }
 
 HybridContributions Session::hybrid_accumulate(std::span<const Polynomial> digits, const HybridKey &key,
                                                std::span<const Completion> waits) {
-  auto &s = open_state();
-  const auto layout = transform_layout(s.base, false);
-  s.conversion_input(layout, false);
-  const size_t active_digits = 1 + (layout.active_q - 1) / layout.parameters->alpha();
-  require(digits.size() == active_digits && key.b_.size() == active_digits && key.a_.size() == active_digits,
-          "wrong active Hybrid digit count");
-  std::vector<std::shared_ptr<detail::Allocation>> pins;
-  std::vector<std::shared_ptr<detail::Operation>> producers;
-  for (size_t d = 0; d < active_digits; ++d)
+  detail::State &s = open_state();
+  const PolynomialLayout layout = transform_layout(s.base, false);
+  s.conversion_layout(layout, false);
+  const size_t count = (layout.active_q + layout.parameters->alpha() - 1) / layout.parameters->alpha();
+  require(digits.size() == count && key.b_.size() == count && key.a_.size() == count,
+          "digit and key counts must match the session level");
+  detail::Request rq;
+  for (size_t d = 0; d < count; ++d)
     for (const Polynomial *part : {&digits[d], &key.b_[d], &key.a_[d]}) {
-      s.check(*part);
+      s.own(*part);
       require_layout(part->layout_, layout);
-      require(s.offset(*part) == 0, "Hybrid MAC requires full active QP buffers");
-      pins.push_back(part->allocation_);
-      producers.push_back(part->producer_);
+      require(s.word_offset(*part) == 0, "the key inner product takes whole QP allocations");
+      rq.inputs.push_back(part);
     }
-  auto b = s.reserve(layout), a = s.reserve(layout);
-  pins.insert(pins.end(), {b, a});
-  auto op = s.begin(std::move(pins), std::move(producers), waits);
-  try {
-    op->event = s.driver->make_event();
-    const size_t elements = bytes(layout) / sizeof(uint64_t);
-    b->buffer = s.driver->allocate(elements);
-    a->buffer = s.driver->allocate(elements);
-    require(bool(b->buffer) && bool(a->buffer) && bool(op->event), "driver returned null resource");
-    for (size_t d = 0; d < active_digits; ++d)
-      s.driver->hybrid_mac(*b->buffer, *a->buffer, *digits[d].allocation_->buffer, *key.b_[d].allocation_->buffer,
-                           *key.a_[d].allocation_->buffer, layout, d == 0);
-    s.finish(op);
-    return {s.output(b, op), s.output(a, op)};
-  } catch (...) {
-    s.fail(std::current_exception());
-    throw;
-  }
+  rq.outputs = {s.new_output(layout), s.new_output(layout)};
+  rq.waits = waits;
+  auto op = s.submit(rq, [&] {
+    for (size_t d = 0; d < count; ++d)
+      s.driver->hybrid_mac(*rq.outputs[0]->buffer, *rq.outputs[1]->buffer, s.buffer_of(digits[d]),
+                           s.buffer_of(key.b_[d]), s.buffer_of(key.a_[d]), layout, d == 0);
+  });
+  return {s.handle(rq.outputs[0], op), s.handle(rq.outputs[1], op)};
 }
 
 HybridContributions Session::hybrid_rotate_extended(const HybridContributions &extended, const Polynomial &c0,
                                                     uint32_t automorphism) {
-  auto &s = open_state();
-  s.check(extended.b);
-  s.check(extended.a);
-  s.check(c0);
-  const auto &layout = extended.b.layout_;
-  s.conversion_input(layout, false);
+  detail::State &s = open_state();
+  s.own(extended.b);
+  s.own(extended.a);
+  s.own(c0);
+  const PolynomialLayout &layout = extended.b.layout_;
+  s.conversion_layout(layout, false);
   require_layout(extended.a.layout_, layout);
-  s.conversion_input(c0.layout_, true);
-  require(c0.layout_.domain == Domain::Evaluation && s.offset(extended.b) == 0 && s.offset(extended.a) == 0,
-          "Hybrid rotation requires evaluation Q and full evaluation QP buffers");
+  s.conversion_layout(c0.layout_, true);
+  require(c0.layout_.domain == Domain::Evaluation, "c0 is given in evaluation form");
+  require(s.word_offset(extended.b) == 0 && s.word_offset(extended.a) == 0,
+          "the rotation takes whole QP allocations");
   require((automorphism & 1) && automorphism < 2 * layout.parameters->n(), "invalid negacyclic automorphism index");
-  auto staging = std::make_shared<detail::HostStorage>(s.budget, layout.active_q);
+  // P mod q_t for every active Q tower (the scale of c0)
+  std::vector<uint64_t> scales(layout.active_q);
   for (size_t t = 0; t < layout.active_q; ++t)
-    staging->values[t] = detail::special_prime_product(*layout.parameters, layout.parameters->q()[t].q);
-  auto table = std::make_shared<detail::DeviceStorage>(s.budget, layout.active_q);
-  auto b = s.reserve(layout), a = s.reserve(layout);
-  auto op = s.begin({extended.b.allocation_, extended.a.allocation_, c0.allocation_, b, a},
-                    {extended.b.producer_, extended.a.producer_, c0.producer_}, {}, staging);
-  op->constants = table;
-  s.submit_with_constants(op, table, {b, a}, [&](detail::Buffer &scales) {
-    s.driver->hybrid_rotate(*b->buffer, *a->buffer, *extended.b.allocation_->buffer, *extended.a.allocation_->buffer,
-                            *c0.allocation_->buffer, s.offset(c0), scales, layout, automorphism);
+    scales[t] = detail::special_prime_product(*layout.parameters, layout.parameters->q()[t].q);
+  detail::Request rq;
+  rq.inputs = {&extended.b, &extended.a, &c0};
+  rq.staging = std::make_shared<detail::HostStorage>(s.ledger, scales);
+  rq.table = std::make_shared<detail::ConstantBlock>(s.ledger, scales.size());
+  rq.outputs = {s.new_output(layout), s.new_output(layout)};
+  auto op = s.submit(rq, [&] {
+    s.driver->hybrid_rotate(*rq.outputs[0]->buffer, *rq.outputs[1]->buffer, s.buffer_of(extended.b),
+                            s.buffer_of(extended.a), s.buffer_of(c0), s.word_offset(c0), *rq.table->buffer, layout,
+                            automorphism);
   });
-  return {s.output(b, op), s.output(a, op)};
+  return {s.handle(rq.outputs[0], op), s.handle(rq.outputs[1], op)};
 }
 
-Polynomial Session::subview(const Polynomial &p, size_t first, size_t count, size_t active_q) {
-  auto &s = open_state();
-  s.check(p);
-  require(first <= p.view_.tower_count && count <= p.view_.tower_count - first, "subview exceeds parent extent");
-  require(active_q <= p.view_.active_q, "subview cannot add Q towers");
-  const View v{p.view_.allocation, p.view_.first_tower + first, count, active_q};
-  auto layout = validate_view(v, p.allocation_->record, s.device);
+Polynomial Session::subview(const Polynomial &p, size_t first_tower, size_t count, size_t active_q) {
+  detail::State &s = open_state();
+  s.own(p);
+  require(first_tower <= p.view_.tower_count && count <= p.view_.tower_count - first_tower,
+          "subview reaches past its parent");
+  require(active_q <= p.view_.active_q, "a subview cannot raise the level");
+  const View v{p.view_.allocation, p.view_.first_tower + first_tower, count, active_q};
+  PolynomialLayout layout = validate_view(v, p.allocation_->record, s.device);
   return Polynomial(state_, p.allocation_, p.producer_, std::move(layout), v);
 }
 
 Download Session::download(const Polynomial &p, std::span<const Completion> waits) {
-  auto &s = open_state();
-  s.check(p);
-  auto staging = std::make_shared<detail::HostStorage>(s.budget, bytes(p.layout_) / sizeof(uint64_t));
-  auto op = s.begin({p.allocation_}, {p.producer_}, waits, staging);
-  try {
-    op->event = s.driver->make_event();
-    require(bool(op->event), "driver returned null event");
-    s.driver->download(staging->values, *p.allocation_->buffer, s.offset(p));
+  detail::State &s = open_state();
+  s.own(p);
+  detail::Request rq;
+  rq.inputs = {&p};
+  rq.staging = std::make_shared<detail::HostStorage>(s.ledger, s.word_count(p));
+  rq.waits = waits;
+  auto op = s.submit(rq, [&] {
+    s.driver->download(rq.staging->values, s.buffer_of(p), s.word_offset(p));
     ++s.downloads;
-    s.finish(op);
-    return Download(state_, op, staging, p.layout_);
-  } catch (...) {
-    s.fail(std::current_exception());
-    throw;
-  }
+  });
+  return Download(state_, op, rq.staging, p.layout_);
 }
 
 Download Session::multiply(std::span<const uint64_t> a, std::span<const uint64_t> b) {
-  auto &s = open_state();
-  validate_host_values(s.base, a);  // both inputs checked before the first enqueue
+  detail::State &s = open_state();
+  // both operands are checked before anything is enqueued
+  validate_host_values(s.base, a);
   validate_host_values(s.base, b);
-  const Polynomial da = upload_checked(s.base, a, {}), db = upload_checked(s.base, b, {});
-  const Polynomial product = pointwise(forward(da), forward(db));
+  const Polynomial da = upload_checked(s.base, a, {});
+  const Polynomial db = upload_checked(s.base, b, {});
+  const Polynomial ea = forward(da), eb = forward(db);
+  const Polynomial product = pointwise(ea, eb);
   return download(inverse(product));
 }
 
 void Session::close() {
-  require(bool(state_), "moved session");
-  auto &s = *state_;
+  require(bool(state_), "session was moved from");
+  detail::State &s = *state_;
   s.closed = true;
-  if (!s.quiescent && !s.drain()) s.fail(s.drain_error);
-  for (auto &[id, weak] : s.registry) {
-    (void)id;
-    if (auto a = weak.lock()) a->record.live = false;
-  }
+  if (!s.quiescent() && !s.drain())
+    s.poison(std::make_exception_ptr(std::runtime_error("queued work could not be proven complete")));
   try {
     s.driver->check_health();
   } catch (...) {
-    s.fail(std::current_exception());
+    s.poison(std::current_exception());
   }
-  if (s.error) std::rethrow_exception(s.error);
+  if (s.first_error) std::rethrow_exception(s.first_error);
 }
 
 }  // namespace ntt::resident

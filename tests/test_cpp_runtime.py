@@ -1,5 +1,5 @@
 """The C++ include/ntt runtime, as the reference gates it (ctest binaries):
-resident_tests runs the Session on the deferred CPU transport; the GPU
+session_tests runs the Session on the deferred CPU transport; the GPU
 binary drives the same API over the gfx950 transport; arith_tests checks the
 NTT kernels' wide-lazy modular arithmetic on the host against 128-bit math."""
 import os
@@ -26,7 +26,7 @@ def test_wide_lazy_arithmetic_ranges():
 
 
 def test_resident_runtime_on_host_transport():
-    assert "RESIDENT HOST RUNTIME CHECKS PASSED" in _run("resident_tests", 600)
+    assert "SESSION RUNTIME CHECKS PASSED" in _run("session_tests", 600)
 
 
 @pytest.mark.gpu

@@ -395,6 +395,12 @@ def test_keyswitch_full_batch_64_graph():
         for c, (eb, ea) in alone.items():
             sl = slice(c * L * n, (c + 1) * L * n)
             assert np.array_equal(gb[sl], eb) and np.array_equal(ga[sl], ea), f"replay {replay} component {c}"
+    # components across both lanes and the batch, straight against the oracle
+    for c in (1, 13, 31, 32, 47, 63):
+        sl = slice(c * L * n, (c + 1) * L * n)
+        log_case(f"batch-64 graph component {c} vs oracle")
+        wb, wa = o.key_switch(x[sl], n, hp.q, hp.q_psi, hp.M, hp.p, hp.p_psi, hp.alpha, kb, ka)
+        assert np.array_equal(gb[sl], wb) and np.array_equal(ga[sl], wa), f"component {c} differs from the oracle"
     graph.close()
     for buf in (d_in, d_b, d_a, ws, ws1):
         buf.free()

@@ -120,3 +120,51 @@ def test_device_resident_requests_equal_host_requests():
     assert np.array_equal(dev[i0][1].download(), wb) and np.array_equal(dev[i0][2].download(), wa)
     srv.close()
     host.close()
+
+
+def test_served_stream_at_the_stated_ranges():
+    """The served mixed stream at its full ranges: N in {2^13, 2^14, 2^15, 2^16},
+    L uniform in [3, 30], dnum in {2, 3, 4}, K = ceil(L / dnum), ops 40% KeySwitch,
+    30% NTT/INTT, 30% ModUp/ModDown, submitted at their Poisson arrival times.
+    Every response equals the same request served alone (tolerance 0), and at
+    least 8 responses of every op are checked against the oracle."""
+    import time
+
+    reqs, pools = workload.stream(96, rate_per_s=3000.0, seed=0xC5C5)
+    sizes = {r.n for r in reqs}
+    levels = [r.level for r in reqs]
+    assert sizes == {8192, 16384, 32768, 65536} and min(levels) <= 6 and max(levels) >= 27
+    srv = serve.Server(streams=4, max_batch=16, hbm_budget_bytes=48 << 30)
+    ms = workload.MixedServer(srv, pools)
+    for r in reqs:
+        ms.params_for(r)
+    log_case(f"served stream, stated ranges: {len(reqs)} requests")
+    t0 = time.perf_counter()
+    tickets = []
+    for r in reqs:  # Poisson arrivals: submit each request at its arrival time
+        delay = t0 + r.arrival_s - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        tickets.append(ms.submit(r))
+    got = [ms.wait(t, r) for t, r in zip(tickets, reqs)]
+    assert srv.stats()["completed"] == len(reqs)
+    alone_srv = serve.Server(streams=1, max_batch=1, hbm_budget_bytes=48 << 30)
+    alone = workload.MixedServer(alone_srv, pools)
+    checked = {op: 0 for op in serve.OP_NAMES}
+    for i, r in enumerate(reqs):
+        (a1, a2), _, _ = alone.wait(alone.submit(r), r)
+        (m1, m2), _, _ = got[i]
+        assert np.array_equal(a1, m1), f"request {i}: in-mix result differs from alone"
+        if r.op == serve.KEYSWITCH:
+            assert np.array_equal(a2, m2), f"request {i}: in-mix A differs from alone"
+        if checked[r.op] < 8:
+            log_case(f"stream oracle {i} op={serve.OP_NAMES[r.op]} n={r.n} L={r.level} dnum={r.dnum}")
+            hp, kb, ka = ms.sets[(r.n, r.level, r.dnum)]
+            w1, w2 = _oracle(r, hp, kb, ka)
+            assert np.array_equal(m1, w1), f"request {i} ({serve.OP_NAMES[r.op]}) differs from the oracle"
+            if w2 is not None:
+                assert np.array_equal(m2, w2)
+            checked[r.op] += 1
+    assert all(v >= 8 for v in checked.values()), checked
+    srv.close()
+    alone_srv.close()

@@ -158,6 +158,18 @@ int ntt_hip_keyswitch_workspace_bytes(ntt_hip_keyswitch *ks, size_t batch, size_
  * All pointers are device pointers; workspace of the size above. */
 int ntt_hip_keyswitch_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batch, uint64_t *out_b, uint64_t *out_a,
                           void *workspace, void *stream);
+/* Allocation-free form of ntt_hip_keyswitch_run (no counterpart in the
+ * reference). For callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc. The
+ * workspace is a static device array inside the library, 136 MiB per GPU
+ * (one component at N = 2^16, L = 24, K = 6, dnum = 4), held from the first
+ * load until the process ends; a batch whose workspace
+ * (ntt_hip_keyswitch_workspace_bytes) exceeds it is rejected with
+ * NTT_HIP_EINVAL. Calls no hipMalloc/hipFree and creates no streams or events
+ * (one lane on `stream`). Every call on a GPU shares that GPU's array, so calls
+ * that overlap on one GPU (other streams, other threads) race. */
+int ntt_hip_keyswitch_run_static(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batch, uint64_t *out_b,
+                                 uint64_t *out_a, void *stream);
 /* The same batch captured once as a HIP graph on `stream` (fixed buffer
  * addresses; no memset nodes) and replayed by ntt_hip_graph_launch. No
  * reference counterpart: the reference launches every stage per call. */

@@ -1176,6 +1176,60 @@ int ntt_hip_keyswitch_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batc
   });
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------- allocation-free key switch
+// The workspace of ntt_hip_keyswitch_run_static lives in this file-scope
+// device array: every GPU the library runs on holds its own copy until the
+// process ends. Host code never uses the array's name as an address (on the
+// host it is only a stand-in for the symbol): the device address comes from
+// hipGetSymbolAddress, once per device, with that device current.
+namespace {
+constexpr size_t kStaticScratchBytes = size_t{136} << 20;  // one component at N = 2^16, L = 24, K = 6, dnum = 4
+constexpr int kMaxStaticDevices = 64;
+__device__ __align__(256) uint64_t scratch[kStaticScratchBytes / sizeof(uint64_t)];
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxStaticDevices] = {};
+
+// Device address of `scratch` on `device` (current): looked up on the first
+// call per device and kept.
+void *static_scratch(int device) {
+  require(device >= 0 && device < kMaxStaticDevices, "device ordinal beyond the static scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *ptr = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    require(ptr != nullptr, "static scratch has no device address");
+    g_scratch_ptr[device] = ptr;
+  }
+  return g_scratch_ptr[device];
+}
+}  // namespace
+
+extern "C" {
+
+int ntt_hip_keyswitch_run_static(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batch, uint64_t *out_b,
+                                 uint64_t *out_a, void *stream) {
+  return guarded([&] {
+    require(ks && in && out_b && out_a, "null argument");
+    require(ks->key_loaded, "evaluation key not uploaded");
+    require(batch >= 1 && batch <= ks->max_batch, "batch exceeds the plan capacity");
+    const size_t need = batch * ks->per_component() * sizeof(uint64_t);
+    if (need > kStaticScratchBytes)
+      throw ContractError("the batch needs " + std::to_string(need) + " bytes of key-switch workspace; the static "
+                          "scratch holds " + std::to_string(kStaticScratchBytes) +
+                          " (use ntt_hip_keyswitch_run with a caller workspace)");
+    const hipStream_t s = ks->ctx->pick(stream);
+    DeviceGuard g(ks->ctx->device);
+    void *workspace = static_scratch(ks->ctx->device);
+    // one lane on the caller's stream: no side streams or events are created
+    const KsBuffers w = ks_buffers(*ks, workspace, batch);
+    ks_modup_mac(*ks, *ks, in, batch, w, w.z, s);
+    ks_moddown(*ks, w.z, batch, out_b, out_a, w, s);
+    return NTT_HIP_OK;
+  });
+}
+
 int ntt_hip_keyswitch_in_place(ntt_hip_keyswitch *ks, uint64_t *c0, uint64_t *c1, const uint64_t *c2, size_t batch,
                                void *workspace, void *stream) {
   return guarded([&] {

@@ -68,6 +68,7 @@ SIGNATURES = {
     "ntt_hip_keyswitch_upload_key": [VP, U64P, U64P, SZ, U64P],
     "ntt_hip_keyswitch_workspace_bytes": [VP, SZ, SZP],
     "ntt_hip_keyswitch_run": [VP, VP, SZ, VP, VP, VP, VP],
+    "ntt_hip_keyswitch_run_static": [VP, VP, SZ, VP, VP, VP],
     "ntt_hip_keyswitch_capture": [VP, VP, SZ, VP, VP, VP, VP, ctypes.POINTER(VP)],
     "ntt_hip_graph_launch": [VP, VP],
     "ntt_hip_graph_destroy": [VP],
@@ -349,6 +350,12 @@ class KeySwitch:
     def run(self, in_ptr, batch, out_b_ptr, out_a_ptr, workspace_ptr, stream=None):
         check(lib().ntt_hip_keyswitch_run(self.handle, in_ptr, batch, out_b_ptr, out_a_ptr, workspace_ptr, stream),
               "keyswitch_run")
+
+    def run_static(self, in_ptr, batch, out_b_ptr, out_a_ptr, stream=None):
+        """The key switch over the library's static device workspace (no
+        allocation; calls overlapping on one GPU race)."""
+        check(lib().ntt_hip_keyswitch_run_static(self.handle, in_ptr, batch, out_b_ptr, out_a_ptr, stream),
+              "keyswitch_run_static")
 
     def modup_workspace_words(self, batch):
         out = SZ(0)

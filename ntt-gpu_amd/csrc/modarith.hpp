@@ -179,9 +179,56 @@ __host__ __device__ __forceinline__ void gs8(uint64_t &x, uint64_t &y, uint64_t 
   y = shoup4(d, w, wp, m);
 }
 
+// Any x < 2^64 -> [0, 4q), congruent (q < 2^61): x - e q with e the quotient
+// estimate of x v / 2^64, v = floor(2^64 / q). floor(x v / 2^64) is at most
+// one short of floor(x / q) for x < 2^64, mulhi_est at most two more, so the
+// remainder is below 4q. Cheaper than a chain of conditional subtracts from
+// bounds of 32q and more.
+__host__ __device__ __forceinline__ uint64_t reduce4(uint64_t x, uint64_t v, const Lazy8 &m) {
+  return mad_lo(x, 0, 0, mulhi_est(x, v), m.nq);
+}
+
 // [0, 8q) -> [0, q).
 __host__ __device__ __forceinline__ uint64_t canon8(uint64_t x, const Lazy8 &m) {
   return sub_if_ge_neg(sub_if_ge_neg(sub_if_ge_neg(x, m.nq4), m.nq2), m.nq);
+}
+
+// ---------------------------------------------------------------- forward reduction schedule (N = 2^16)
+// A forward (Cooley-Tukey) butterfly only needs its x below 2^64 - 4q: it
+// computes x + T and x - T + 4q with T = y w mod q in [0, 4q) for any y, so
+// both outputs grow by 4q over the bound of x. With every q < 2^(61 - H) a
+// residue may reach 2^(3 + H) q, so a tower can run several stages before x
+// must be brought down. make_fwd_plan(H) is the greedy schedule over the 16 stages
+// of a 2^16-point transform (pass A: stages 0-7, pass B: 8-15), starting from
+// canonical input: stage g reduces x below c = 2^red[g] q (red[g] = -1: no
+// reduction) only when skipping it would pass the bound; `bound_log` bounds
+// the output by 2^bound_log q. H = 0 keeps one reduction per stage (below
+// 4q, outputs below 8q): the schedule of every other launch shape.
+// Reductions per transform: 16 for H = 0, 7 for q < 2^60, 3 for q < 2^59,
+// 1 for q < 2^58 and none from q < 2^57 down.
+struct FwdPlanData {
+  int red[16];
+  int bound_log;
+};
+__host__ __device__ constexpr FwdPlanData make_fwd_plan(int H) {
+  FwdPlanData p{};
+  const int cap = 8 << H;  // residues below cap * q <= 2^64
+  int bound = 1;           // canonical input
+  for (int g = 0; g < 16; ++g) {
+    if (H > 0 && bound + 4 <= cap) {
+      p.red[g] = -1;
+      bound += 4;
+    } else {
+      int k = 2;
+      while ((2 << k) < bound) ++k;  // x < 2c is what one conditional subtract needs
+      p.red[g] = k;
+      bound = (1 << k) + 4;
+    }
+  }
+  int l = 0;
+  while ((1 << l) < bound) ++l;
+  p.bound_log = l;
+  return p;
 }
 
 // ---------------------------------------------------------------- Barrett

@@ -59,10 +59,22 @@ struct TowerMap {
   } epi;
 };
 
-// Arithmetic of a launch, chosen per context from its largest modulus.
+// Arithmetic of a launch, chosen per context from its largest modulus. For
+// kLazy the bits above kArithMask carry the headroom class H of the forward
+// transform at N = 2^16: every modulus is below 2^(61 - H), so residues may
+// grow to 2^(3 + H) q before they must be reduced (0 <= H <= kMaxHeadroom).
 enum Arith : int { kExact = 0, kLazy = 1, kNarrow = 2 };
+constexpr int kArithMask = 15, kArithShift = 4, kMaxHeadroom = 5;
+inline int headroom_for(uint64_t max_q) {
+  int bits = 0;
+  while (bits < 64 && (max_q >> bits) != 0) ++bits;
+  const int h = 61 - bits;
+  return h < 0 ? 0 : h > kMaxHeadroom ? kMaxHeadroom : h;
+}
 inline int arith_for(uint64_t max_q) {
-  return max_q < (uint64_t{1} << 30) ? kNarrow : max_q < (uint64_t{1} << 61) ? kLazy : kExact;
+  if (max_q < (uint64_t{1} << 30)) return kNarrow;
+  if (max_q < (uint64_t{1} << 61)) return kLazy | (headroom_for(max_q) << kArithShift);
+  return kExact;
 }
 
 hipError_t launch_ntt(const TowerMap &map, const PrimeConst *primes, uint32_t log_n, uint32_t towers, bool inverse,

@@ -187,6 +187,77 @@ __device__ __forceinline__ uint64_t epi_diff(uint64_t sub, Val<A> x, const Lazy8
   else return sub_mod(sub, x, q);
 }
 
+// The forward reduction schedule of headroom class H (make_fwd_plan, modarith.hpp).
+template <int H>
+struct FwdPlan {
+  static constexpr FwdPlanData d = make_fwd_plan(H);
+};
+
+// 2^64 - 2^k q for k = 0..6 (k <= 2 precomputed per prime; larger ones as
+// shifts of 2^64 - 4q, opaque to the optimizer, which otherwise folds the add
+// back into a subtract with a carry chain).
+struct NegMultiples {
+  uint64_t c[7];
+};
+__device__ __forceinline__ NegMultiples neg_multiples(const Lazy8 &m) {
+  NegMultiples r;
+  r.c[0] = m.nq;
+  r.c[1] = m.nq2;
+  r.c[2] = m.nq4;
+#pragma unroll
+  for (int k = 3; k < 7; ++k) r.c[k] = m.nq4 << (k - 2);
+  return r;
+}
+
+// Forward butterfly of stage g of plan H.
+template <int A, int H, int G>
+__device__ __forceinline__ void ct_planned(Val<A> &x, Val<A> &y, tw2 w, const Lazy8 &m, const NegMultiples &neg) {
+  if constexpr (A == kLazy && H > 0) {
+    constexpr int k = FwdPlan<H>::d.red[G];
+    uint64_t xr = x;
+    if constexpr (k >= 0) xr = sub_if_ge_neg(x, neg.c[k]);
+    const uint64_t xn = mad_lo(xr, y, w.x, mulhi_est(y, w.y), m.nq);
+    y = 2 * xr + m.q4 - xn;  // xr - T + 4q; the wrap of 2xr + 4q cancels mod 2^64
+    x = xn;
+  } else {
+    ct_bfly<A>(x, y, w, m);
+  }
+}
+
+// Forward output of plan H -> canonical: conditional subtracts from the
+// plan's bound, or (bounds of 32q and more) the quotient-estimate reduction
+// to [0, 4q) first (v = floor(2^64 / q)).
+template <int A, int H>
+__device__ __forceinline__ Val<A> canon_planned(Val<A> x, const Lazy8 &m, const NegMultiples &neg, uint64_t v) {
+  if constexpr (A == kLazy && H > 0) {
+    constexpr int bl = FwdPlan<H>::d.bound_log;
+    if constexpr (bl >= 5) {
+      return sub_if_ge_neg(sub_if_ge_neg(reduce4(x, v, m), neg.c[1]), neg.c[0]);
+    } else {
+#pragma unroll
+      for (int k = bl - 1; k >= 0; --k) x = sub_if_ge_neg(x, neg.c[k]);
+      return x;
+    }
+  } else {
+    return canon<A>(x, m);
+  }
+}
+
+// Forward output of plan H -> below 8q (the input range of epi_diff).
+template <int A, int H>
+__device__ __forceinline__ Val<A> below8_planned(Val<A> x, const Lazy8 &m, const NegMultiples &neg, uint64_t v) {
+  if constexpr (A == kLazy && H > 0) {
+    constexpr int bl = FwdPlan<H>::d.bound_log;
+    if constexpr (bl >= 6) {
+      return reduce4(x, v, m);
+    } else {
+#pragma unroll
+      for (int k = bl - 1; k >= 3; --k) x = sub_if_ge_neg(x, neg.c[k]);
+    }
+  }
+  return x;
+}
+
 // Four register stages on v[0..15]. H runs 8,4,2,1 (forward) or 1,2,4,8
 // (inverse). Twiddle index = ml * row_factor + group, ml = (8/h) * ml_scale,
 // group = g_base * (8/h) + k / (2h).
@@ -206,6 +277,25 @@ __device__ __forceinline__ void reg_stages(Val<A> (&v)[16], TwPtr tw, uint32_t m
       if constexpr (Inverse) gs_bfly<A>(v[k], v[k + h], w, m);
       else ct_bfly<A>(v[k], v[k + h], w, m);
     }
+  }
+}
+
+// Four forward register stages G0..G0+3 of plan H (the forward form of
+// reg_stages; distances 8, 4, 2, 1).
+template <int A, int H, int G0, int S = 0>
+__device__ __forceinline__ void reg_stages_fwd(Val<A> (&v)[16], TwPtr tw, uint32_t ml_scale, uint32_t row_factor,
+                                               uint32_t g_base, const Lazy8 &m, const NegMultiples &neg) {
+  if constexpr (S < 4) {
+    constexpr int h = 8 >> S;
+    const uint32_t ml = (8u / h) * ml_scale;
+    const uint32_t gb = g_base * (8u / h);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (k & h) continue;
+      const tw2 w = tw[ml * row_factor + gb + (uint32_t)(k / (2 * h))];
+      ct_planned<A, H, G0 + S>(v[k], v[k + h], w, m, neg);
+    }
+    reg_stages_fwd<A, H, G0, S + 1>(v, tw, ml_scale, row_factor, g_base, m, neg);
   }
 }
 
@@ -242,7 +332,7 @@ __device__ __forceinline__ uint64_t *epi_dest(const TowerMap &m, uint32_t k, uin
 }
 
 // ---------------------------------------------------------------- pass A, R = 256
-template <bool Inverse, int A>
+template <bool Inverse, int A, int H = 0>
 __global__ __launch_bounds__(256) void ntt_pass_a256(TowerMap map, const PrimeConst *__restrict__ primes) {
   __shared__ Val<A> lds[16 * kHalfRow];
   uint64_t *data;
@@ -258,9 +348,10 @@ __global__ __launch_bounds__(256) void ntt_pass_a256(TowerMap map, const PrimeCo
   if constexpr (!Inverse) {
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = src[(uint64_t)(u + 16 * k) * kRow + c];
-    reg_stages<false, A>(v, tw, 1, 1, 0, m);
+    const NegMultiples neg = neg_multiples(m);
+    reg_stages_fwd<A, H, 0>(v, tw, 1, 1, 0, m, neg);
     xchg_cols_p1_p2(v, lds, cl, u);
-    reg_stages<false, A>(v, tw, 16, 1, u, m);
+    reg_stages_fwd<A, H, 4>(v, tw, 16, 1, u, m, neg);
 #pragma unroll
     for (int k = 0; k < 16; ++k) data[(uint64_t)(16 * u + k) * kRow + c] = v[k];
   } else {
@@ -369,7 +460,26 @@ __device__ __forceinline__ void wave_sync8() {
 }
 __device__ __forceinline__ int pad16(int j) { return j + (j >> 4); }
 
-template <int A, bool Epi>
+// Row-pass stages S..SEnd-1 (row stage s is transform stage 8 + s) in the
+// three layouts of ntt_pass_b8_fwd: s = 0-2 in P1, 3-5 in P2, 6-7 in P3.
+template <int A, int H, int S, int SEnd>
+__device__ __forceinline__ void b8_fwd_stages(Val<A> (&v)[8], TwPtr tw, uint32_t rf, int u, const Lazy8 &m,
+                                              const NegMultiples &neg) {
+  if constexpr (S < SEnd) {
+    constexpr int h = S < 3 ? 4 >> S : S < 6 ? 4 >> (S - 3) : 2 >> (S - 6);
+    const uint32_t gb = S < 3 ? 0u : S < 6 ? (uint32_t)(u >> 2) << (S - 3) : (uint32_t)u << (S - 5);
+    constexpr int shift = S < 3 ? 3 - S : S < 6 ? 6 - S : 8 - S;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (k & h) continue;
+      const tw2 w = tw[(1u << S) * rf + gb + (uint32_t)(k >> shift)];
+      ct_planned<A, H, 8 + S>(v[k], v[k + h], w, m, neg);
+    }
+    b8_fwd_stages<A, H, S + 1, SEnd>(v, tw, rf, u, m, neg);
+  }
+}
+
+template <int A, bool Epi, int H = 0>
 __global__ __launch_bounds__(256) void ntt_pass_b8_fwd(TowerMap map, const PrimeConst *__restrict__ primes, uint32_t R) {
   __shared__ Val<A> lds[8 * kLdsRow];
   uint64_t *data;
@@ -385,18 +495,10 @@ __global__ __launch_bounds__(256) void ntt_pass_b8_fwd(TowerMap map, const Prime
   const uint32_t rf = R + r;
   Val<A> *buf = lds + rl * kLdsRow;
   Val<A> v[8];
+  const NegMultiples neg = neg_multiples(m);
 #pragma unroll
   for (int k = 0; k < 8; ++k) v[k] = (Val<A>)row[u + 32 * k];
-#pragma unroll
-  for (int s = 0; s < 3; ++s) {
-    const int h = 4 >> s;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      if (k & h) continue;
-      const tw2 w = tw[(1u << s) * rf + (uint32_t)(k >> (3 - s))];
-      ct_bfly<A>(v[k], v[k + h], w, m);
-    }
-  }
+  b8_fwd_stages<A, H, 0, 3>(v, tw, rf, u, m, neg);
   const int p2 = ((u >> 2) << 5) | (u & 3);
 #pragma unroll
   for (int k = 0; k < 8; ++k) buf[pad16(u + 32 * k)] = v[k];
@@ -404,34 +506,14 @@ __global__ __launch_bounds__(256) void ntt_pass_b8_fwd(TowerMap map, const Prime
 #pragma unroll
   for (int k = 0; k < 8; ++k) v[k] = buf[pad16(p2 | (k << 2))];
   wave_sync8();
-#pragma unroll
-  for (int s = 3; s < 6; ++s) {
-    const int h = 4 >> (s - 3);
-    const uint32_t gb = (uint32_t)(u >> 2) << (s - 3);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      if (k & h) continue;
-      const tw2 w = tw[(1u << s) * rf + gb + (uint32_t)(k >> (6 - s))];
-      ct_bfly<A>(v[k], v[k + h], w, m);
-    }
-  }
+  b8_fwd_stages<A, H, 3, 6>(v, tw, rf, u, m, neg);
 #pragma unroll
   for (int k = 0; k < 8; ++k) buf[pad16(p2 | (k << 2))] = v[k];
   wave_sync8();
 #pragma unroll
   for (int k = 0; k < 8; ++k) v[k] = buf[pad16((u << 3) | k)];
   wave_sync8();
-#pragma unroll
-  for (int s = 6; s < 8; ++s) {
-    const int h = 2 >> (s - 6);
-    const uint32_t gb = (uint32_t)u << (s - 5);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      if (k & h) continue;
-      const tw2 w = tw[(1u << s) * rf + gb + (uint32_t)(k >> (8 - s))];
-      ct_bfly<A>(v[k], v[k + h], w, m);
-    }
-  }
+  b8_fwd_stages<A, H, 6, 8>(v, tw, rf, u, m, neg);
   if constexpr (Epi) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) buf[pad16((u << 3) | k)] = v[k];
@@ -446,19 +528,19 @@ __global__ __launch_bounds__(256) void ntt_pass_b8_fwd(TowerMap map, const Prime
     if (acc) {
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
-        const uint64_t d = epi_diff<A>(sub[u + 32 * k], v[k], m, pc.q);
+        const uint64_t d = epi_diff<A>(sub[u + 32 * k], below8_planned<A, H>(v[k], m, neg, pc.mu_hi), m, pc.q);
         out[u + 32 * k] = add_mod(out[u + 32 * k], shoup(d, c.x, c.y, pc.q), pc.q);
       }
       return;
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const uint64_t d = epi_diff<A>(sub[u + 32 * k], v[k], m, pc.q);
+      const uint64_t d = epi_diff<A>(sub[u + 32 * k], below8_planned<A, H>(v[k], m, neg, pc.mu_hi), m, pc.q);
       out[u + 32 * k] = shoup(d, c.x, c.y, pc.q);
     }
   } else {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) buf[pad16((u << 3) | k)] = canon<A>(v[k], m);
+    for (int k = 0; k < 8; ++k) buf[pad16((u << 3) | k)] = canon_planned<A, H>(v[k], m, neg, pc.mu_hi);
     wave_sync8();
 #pragma unroll
     for (int k = 0; k < 8; ++k) row[u + 32 * k] = buf[pad16(u + 32 * k)];
@@ -615,7 +697,7 @@ __global__ __launch_bounds__(256) void ntt_lds_generic(TowerMap map, const Prime
 }
 
 // ---------------------------------------------------------------- launcher
-template <bool Inverse, int A>
+template <bool Inverse, int A, int H = 0>
 static hipError_t launch_t(const TowerMap &map, const PrimeConst *primes, uint32_t log_n, uint32_t towers,
                            hipStream_t stream) {
   const uint32_t n = 1u << log_n;
@@ -627,7 +709,8 @@ static hipError_t launch_t(const TowerMap &map, const PrimeConst *primes, uint32
   const uint32_t R = n / kRow, log_r = log_n - 8;
   auto pass_a = [&]() {
     if (R == 256) {
-      hipLaunchKernelGGL((ntt_pass_a256<Inverse, A>), dim3(kRow / 16, towers), dim3(256), 0, stream, map, primes);
+      hipLaunchKernelGGL((ntt_pass_a256<Inverse, A, Inverse ? 0 : H>), dim3(kRow / 16, towers), dim3(256), 0, stream,
+                         map, primes);
     } else {
       const uint32_t cols = 4096 / R;
       hipLaunchKernelGGL((ntt_lds_generic<Inverse, A>), dim3(kRow / cols, towers), dim3(256), 0, stream, map,
@@ -644,9 +727,11 @@ static hipError_t launch_t(const TowerMap &map, const PrimeConst *primes, uint32
     if constexpr (!Inverse && (A == kLazy || A == kNarrow)) {
       if (R == 256) {
         if (map.epi.sub)
-          hipLaunchKernelGGL((ntt_pass_b8_fwd<A, true>), dim3(R / 8, towers), dim3(256), 0, stream, map, primes, R);
+          hipLaunchKernelGGL((ntt_pass_b8_fwd<A, true, H>), dim3(R / 8, towers), dim3(256), 0, stream, map, primes,
+                             R);
         else
-          hipLaunchKernelGGL((ntt_pass_b8_fwd<A, false>), dim3(R / 8, towers), dim3(256), 0, stream, map, primes, R);
+          hipLaunchKernelGGL((ntt_pass_b8_fwd<A, false, H>), dim3(R / 8, towers), dim3(256), 0, stream, map, primes,
+                             R);
         return;
       }
     }
@@ -666,6 +751,33 @@ static hipError_t launch_t(const TowerMap &map, const PrimeConst *primes, uint32
   return hipGetLastError();
 }
 
+// One launch of `count` launch indices in the arithmetic `arith` (mode and,
+// for kLazy, the forward headroom class).
+static hipError_t dispatch(const TowerMap &m, const PrimeConst *primes, uint32_t log_n, uint32_t count, bool inverse,
+                           int arith, hipStream_t stream) {
+  const int mode = arith & kArithMask, h = arith >> kArithShift;
+  if (mode == kLazy) {
+    if (inverse) return launch_t<true, kLazy>(m, primes, log_n, count, stream);
+    switch (h) {
+      case 0: return launch_t<false, kLazy, 0>(m, primes, log_n, count, stream);
+      case 1: return launch_t<false, kLazy, 1>(m, primes, log_n, count, stream);
+      case 2: return launch_t<false, kLazy, 2>(m, primes, log_n, count, stream);
+      case 3: return launch_t<false, kLazy, 3>(m, primes, log_n, count, stream);
+      case 4: return launch_t<false, kLazy, 4>(m, primes, log_n, count, stream);
+      case 5: return launch_t<false, kLazy, 5>(m, primes, log_n, count, stream);
+      default: return hipErrorInvalidValue;
+    }
+  }
+  if (h != 0) return hipErrorInvalidValue;
+  switch (mode * 2 + (inverse ? 1 : 0)) {
+    case kNarrow * 2: return launch_t<false, kNarrow>(m, primes, log_n, count, stream);
+    case kNarrow * 2 + 1: return launch_t<true, kNarrow>(m, primes, log_n, count, stream);
+    case kExact * 2: return launch_t<false, kExact>(m, primes, log_n, count, stream);
+    case kExact * 2 + 1: return launch_t<true, kExact>(m, primes, log_n, count, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 hipError_t launch_ntt_span(const TowerMap &map, const PrimeConst *primes, uint32_t log_n, uint32_t polys, uint32_t y0,
                            uint32_t count, bool inverse, int arith, hipStream_t stream) {
   if (count == 0) return hipSuccess;
@@ -673,15 +785,7 @@ hipError_t launch_ntt_span(const TowerMap &map, const PrimeConst *primes, uint32
   TowerMap m = map;
   m.polys = polys;
   m.y_offset = map.y_offset + y0;
-  switch (arith * 2 + (inverse ? 1 : 0)) {
-    case kLazy * 2: return launch_t<false, kLazy>(m, primes, log_n, count, stream);
-    case kLazy * 2 + 1: return launch_t<true, kLazy>(m, primes, log_n, count, stream);
-    case kNarrow * 2: return launch_t<false, kNarrow>(m, primes, log_n, count, stream);
-    case kNarrow * 2 + 1: return launch_t<true, kNarrow>(m, primes, log_n, count, stream);
-    case kExact * 2: return launch_t<false, kExact>(m, primes, log_n, count, stream);
-    case kExact * 2 + 1: return launch_t<true, kExact>(m, primes, log_n, count, stream);
-    default: return hipErrorInvalidValue;
-  }
+  return dispatch(m, primes, log_n, count, inverse, arith, stream);
 }
 
 // `chunk` > 0 splits the towers into groups run pass after pass (A then B for
@@ -697,16 +801,7 @@ hipError_t launch_ntt(const TowerMap &map, const PrimeConst *primes, uint32_t lo
     m.polys = towers / map.towers_per_poly;
     m.y_offset = map.y_offset + first;
     const uint32_t count = towers - first < step ? towers - first : step;
-    hipError_t e;
-    switch (arith * 2 + (inverse ? 1 : 0)) {
-      case kLazy * 2: e = launch_t<false, kLazy>(m, primes, log_n, count, stream); break;
-      case kLazy * 2 + 1: e = launch_t<true, kLazy>(m, primes, log_n, count, stream); break;
-      case kNarrow * 2: e = launch_t<false, kNarrow>(m, primes, log_n, count, stream); break;
-      case kNarrow * 2 + 1: e = launch_t<true, kNarrow>(m, primes, log_n, count, stream); break;
-      case kExact * 2: e = launch_t<false, kExact>(m, primes, log_n, count, stream); break;
-      case kExact * 2 + 1: e = launch_t<true, kExact>(m, primes, log_n, count, stream); break;
-      default: return hipErrorInvalidValue;
-    }
+    const hipError_t e = dispatch(m, primes, log_n, count, inverse, arith, stream);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;

@@ -8,6 +8,7 @@
 #include <initializer_list>
 #include <random>
 #include <string>
+#include <vector>
 
 #include "../../ntt-gpu_amd/csrc/modarith.hpp"
 
@@ -54,8 +55,108 @@ void one(uint64_t q, uint64_t w, uint64_t x, uint64_t y) {
 }
 }  // namespace
 
+uint64_t powmod(uint64_t b, uint64_t e, uint64_t q) {
+  uint64_t r = 1 % q;
+  for (b %= q; e; e >>= 1, b = mulmod(b, b, q))
+    if (e & 1) r = mulmod(r, b, q);
+  return r;
+}
+bool is_prime(uint64_t n) {
+  if (n < 2) return false;
+  for (uint64_t p : {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37})
+    if (n % p == 0) return n == p;
+  uint64_t d = n - 1;
+  int s = 0;
+  while (!(d & 1)) d >>= 1, ++s;
+  for (uint64_t a : {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37}) {
+    uint64_t x = powmod(a, d, n);
+    if (x == 1 || x == n - 1) continue;
+    bool composite = true;
+    for (int r = 1; r < s && composite; ++r) composite = (x = mulmod(x, x, n)) != n - 1;
+    if (composite) return false;
+  }
+  return true;
+}
+uint32_t brev(uint32_t x, int bits) {
+  uint32_t r = 0;
+  for (int i = 0; i < bits; ++i) r = (r << 1) | ((x >> i) & 1);
+  return r;
+}
+
+// The forward reduction schedule of every headroom class H, run over a whole
+// 2^16-point transform with the largest NTT prime of that class: every
+// butterfly's exact value stays below 2^64, every residue below the plan's
+// bound, and the canonicalised output equals an exact 128-bit transform.
+void planned_forward_transforms(std::mt19937_64 &rng) {
+  const int n = 1 << 16, lg = 16;
+  const int reductions[6] = {16, 7, 3, 1, 0, 0};
+  for (int H = 1; H <= 5; ++H) {
+    const FwdPlanData plan = make_fwd_plan(H);
+    int count = 0;
+    for (int g = 0; g < 16; ++g) count += plan.red[g] >= 0;
+    check(count == reductions[H], "reductions of plan " + std::to_string(H));
+    uint64_t q = (uint64_t{1} << (61 - H)) - (uint64_t{1} << 17) + 1;
+    while (!is_prime(q)) q -= uint64_t{1} << 17;
+    uint64_t psi = 0;
+    for (uint64_t g = 2; !psi; ++g) {
+      const uint64_t z = powmod(g, (q - 1) / (2 * n), q);
+      if (powmod(z, n, q) == q - 1) psi = z;
+    }
+    std::vector<uint64_t> tw(n), twp(n);
+    for (uint32_t e = 0; e < (uint32_t)n; ++e) {
+      tw[brev(e, lg)] = powmod(psi, e, q);
+    }
+    for (int k = 0; k < n; ++k) twp[k] = shoup_companion(tw[k], q);
+    const Lazy8 m = lazy8_of(q);
+    for (int pattern = 0; pattern < 2; ++pattern) {
+      std::vector<uint64_t> a(n), want(n);
+      for (int i = 0; i < n; ++i) a[i] = pattern == 0 ? q - 1 : rng() % q;
+      want = a;
+      uint64_t bound = 1;  // in units of q, tracked from the plan
+      for (int g = 0; g < lg; ++g) {
+        const int mm = 1 << g, d = n >> (g + 1);
+        const int k = plan.red[g];
+        bound = k >= 0 ? (uint64_t{1} << k) + 4 : bound + 4;
+        for (int grp = 0; grp < mm; ++grp) {
+          const uint64_t w = tw[mm + grp], wp = twp[mm + grp];
+          for (int j = 0; j < d; ++j) {
+            const int i = 2 * grp * d + j;
+            // exact reference on canonical values
+            const uint64_t t = mulmod(want[i + d], w, q);
+            const uint64_t rx = (want[i] + t) % q, ry = (want[i] + q - t) % q;
+            want[i] = rx;
+            want[i + d] = ry;
+            // the planned butterfly
+            const uint64_t x = a[i], y = a[i + d];
+            const uint64_t xr = k >= 0 ? sub_if_ge_neg(x, 0 - (q << k)) : x;
+            const uint64_t xn = mad_lo(xr, y, w, mulhi_est(y, wp), m.nq);
+            const u128 exact = (u128)xr + ((u128)xn - xr);  // xr + T as computed
+            check(xn >= xr && xn - xr < 4 * q && (xn - xr) % q == mulmod(y % q, w, q) &&
+                      exact < ((u128)1 << 64),
+                  "planned butterfly: T in [0, 4q), no wrap");
+            a[i] = xn;
+            a[i + d] = 2 * xr + m.q4 - xn;
+            check((u128)a[i] < (u128)bound * q && (u128)a[i + d] < (u128)bound * q, "residues within the plan bound");
+          }
+        }
+      }
+      check(bound <= (uint64_t{1} << plan.bound_log), "plan bound_log covers the output");
+      const uint64_t v = (uint64_t)((~(u128)0 / q) >> 64);
+      for (int i = 0; i < n; ++i) {
+        uint64_t x = a[i], y = a[i];
+        for (int k = plan.bound_log - 1; k >= 0; --k) x = sub_if_ge_neg(x, 0 - (q << k));
+        y = sub_if_ge_neg(sub_if_ge_neg(reduce4(y, v, m), 0 - 2 * q), 0 - q);
+        if (x != want[i] || y != want[i])
+          check(false, "planned transform equals the exact transform, H = " + std::to_string(H));
+      }
+      ++checks;
+    }
+  }
+}
+
 int main() {
   std::mt19937_64 rng(0x7251d9);
+  planned_forward_transforms(rng);
   // OpenFHE-shaped 60- and 50-bit primes, a prime just below 2^61 and tiny ones
   const uint64_t primes[] = {1152921504606584833ULL, 1125899906826241ULL, 2305843009213693951ULL,
                              1152921504598720513ULL, 562949953216513ULL,  65537ULL, 17ULL};
@@ -70,6 +171,15 @@ int main() {
       const uint64_t x = rng(), y = (i & 1) ? rng() : ~0ULL - (rng() & 0xffff);
       one(q, w, x, y);
     }
+  }
+  // quotient-estimate reduction of any 64-bit value to [0, 4q)
+  for (int i = 0; i < 2000000; ++i) {
+    uint64_t q = (rng() >> (3 + i % 50)) | 1;
+    if (i % 7 == 0) q = (uint64_t{1} << 61) - 1 - 2 * (rng() & 0xffff);
+    if (q < 3) q = 3;
+    const uint64_t x = (i % 5 == 0) ? ~0ULL - (rng() & 0xff) : rng();
+    const uint64_t r = reduce4(x, (uint64_t)((~(u128)0 / q) >> 64), lazy8_of(q));
+    check(r < 4 * q && r % q == x % q, "reduce4 in [0, 4q) and congruent");
   }
   // mulhi_est on arbitrary 64-bit operands (not only Shoup companions)
   for (int i = 0; i < 1000000; ++i) {

@@ -647,3 +647,17 @@ def test_relinearize_rejections():
     assert hip.KeySwitch(ctx, 6, 2, 2, 65535 // 80).max_batch == 65535 // 80
     with pytest.raises(hip.ContractError, match="rows of one launch"):
         hip.KeySwitch(ctx, 6, 2, 2, 65535 // 80 + 1)
+
+
+@pytest.mark.parametrize("bits", [59, 58, 57, 50])
+def test_keyswitch_headroom_classes(bits):
+    """Key switch at N=2^16 on bases whose moduli all sit at the top of one
+    forward headroom class (the complement transforms and the ModDown
+    epilogue, which brings the fewer-reduced output below 8q first), vs the oracle."""
+    n, L, K = 65536, 4, 2
+    qs = [params.last_prime(bits, 2 * n)]
+    while len(qs) < L + K:
+        qs.append(params.previous_prime(qs[-1], 2 * n))
+    hp = params.HybridParams(n=n, q=qs[:L], p=qs[L:], alpha=2)
+    log_case(f"keyswitch headroom {bits}-bit n=65536 L=4 K=2 alpha=2")
+    _check_keyswitch(hp, L, 1, 15000 + bits)

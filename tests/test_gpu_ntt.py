@@ -211,3 +211,31 @@ def test_out_of_place_and_subrange():
     ctx.transform(src.ptr, 2, 3, True, src_ptr=dst.ptr)
     ctx.synchronize()
     assert np.array_equal(src.download(), x)
+
+
+def _top_primes(bits, count, n=N16):
+    """The `count` largest NTT primes below 2^bits (q = 1 mod 2n): the top of a
+    forward headroom class, where the skipped reductions leave the least room."""
+    qs = [params.last_prime(bits, 2 * n)]
+    while len(qs) < count:
+        qs.append(params.previous_prime(qs[-1], 2 * n))
+    return qs
+
+
+@pytest.mark.parametrize("bits", [61, 60, 59, 58, 57, 56, 50])
+def test_forward_headroom_classes(bits):
+    """The forward transform at N=2^16 skips x reductions while residues stay
+    below 2^64 (every q < 2^(61 - H): 7, 3, 1 or 0 reductions over 16 stages).
+    Top-of-class primes, patterns that drive every residue to its largest value
+    (q - 1 everywhere) and seeded ones: bit-exact vs the oracle, and the round
+    trip."""
+    qs = _top_primes(bits, 3)
+    rs = [params.min_root(q, N16) for q in qs]
+    ctx = hip.Context(N16, qs, rs)
+    for name, x in [("q-1", np.concatenate([np.full(N16, q - 1, np.uint64) for q in qs])),
+                    ("uniform", uniform(N16, qs, bits)),
+                    ("x^(N-1)", np.concatenate([(np.arange(N16) == N16 - 1).astype(np.uint64)] * 3))]:
+        log_case(f"forward headroom {bits}-bit {name}")
+        got = _gpu_round(ctx, x, 0, 3, False)
+        assert np.array_equal(got, o.ntt_towers(x, N16, qs, rs)), f"{bits}-bit forward ({name})"
+        assert np.array_equal(_gpu_round(ctx, got, 0, 3, True), x), f"{bits}-bit round trip ({name})"

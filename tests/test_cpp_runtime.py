@@ -32,3 +32,10 @@ def test_resident_runtime_on_host_transport():
 @pytest.mark.gpu
 def test_resident_runtime_on_hip_transport():
     assert "HIP RESIDENT CHECKS PASSED" in _run("hip_resident_tests", 600)
+
+
+def test_openfhe_adapter_on_host_transport():
+    """include/ntt/openfhe_hybrid_adapter.hpp with an OpenFHE test double and the
+    replay transport: the five entry points against the oracle, commit order
+    and the unchanged ciphertext on failure."""
+    assert "ADAPTER CHECKS PASSED" in _run("adapter_tests", 600)

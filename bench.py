@@ -434,25 +434,66 @@ def reference_workloads():
             "rows": rows}
 
 
+def host_description():
+    """nproc, the CPU cores this process may use, the CPU model and the compiler
+    of the oracle (the reference's CPU-baseline protocol records them)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cc = subprocess.run(["gcc", "--version"], capture_output=True, text=True, timeout=10).stdout.splitlines()[0]
+    except (OSError, subprocess.TimeoutExpired, IndexError):
+        cc = None
+    return {"nproc": os.cpu_count(), "usable_cores": len(os.sched_getaffinity(0)), "cpu_model": model,
+            "compiler": cc, "flags": "-O2 -fopenmp (oracle/Makefile)"}
+
+
 def cpu_baseline(hp, x0, kb, ka, gpu_b, gpu_a, samples):
-    """The oracle's key switch (reference algorithm, 1 thread, towers in sequence)
-    on component 0 of the same batch; also checks the GPU output bit-exactly."""
+    """The oracle's key switch (the reference algorithm in C) on component 0 of
+    the same batch, timed on this host's cores by the reference's protocol
+    (host wall clock, one warm-up, then samples; median, p5, p95): one thread
+    with towers in sequence (the reference's OpenFHE CPU protocol), and towers
+    in parallel on every core this process may use (OpenMP; the box's share,
+    OMP_NUM_THREADS). Also checks the GPU output bit-exactly."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as o
-    o.set_threads(1)
-    times, exact = [], True
-    for _ in range(samples):
-        t0 = time.perf_counter()
-        wb, wa = o.key_switch(x0, N, hp.q, hp.q_psi, hp.M, hp.p, hp.p_psi, hp.alpha, kb, ka)
-        times.append(time.perf_counter() - t0)
-        exact = exact and np.array_equal(wb, gpu_b) and np.array_equal(wa, gpu_a)
+    exact = True
+
+    def leg(threads):
+        nonlocal exact
+        o.set_threads(threads)
+        o.key_switch(x0, N, hp.q, hp.q_psi, hp.M, hp.p, hp.p_psi, hp.alpha, kb, ka)  # warm-up
+        times = []
+        for _ in range(samples):
+            t0 = time.perf_counter()
+            wb, wa = o.key_switch(x0, N, hp.q, hp.q_psi, hp.M, hp.p, hp.p_psi, hp.alpha, kb, ka)
+            times.append(time.perf_counter() - t0)
+            exact = exact and np.array_equal(wb, gpu_b) and np.array_equal(wa, gpu_a)
+        med = statistics.median(times)
+        return {"value": round(1.0 / med, 4), "unit": "KeySwitch/s", "cores": threads,
+                "median_s": round(med, 4), "p5_s": round(pct(times, 0.05), 4), "p95_s": round(pct(times, 0.95), 4)}
+
+    host = host_description()
+    all_cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or host["usable_cores"]
+    one = leg(1)
+    many = leg(all_cores)
+    progress(f"cpu baseline: 1 core {one['value']} KS/s, {all_cores} cores {many['value']} KS/s")
     return {
-        "value": round(1.0 / statistics.median(times), 4),
+        "value": one["value"],
         "unit": "KeySwitch/s",
         "cores": 1,
         "kind": "port",
         "sample": f"{samples} key switch(es) of one component (N=2^16, L=24, K=6, dnum=4), C oracle, "
-                  f"1 thread, towers in sequence; median {statistics.median(times):.2f} s",
+                  f"1 thread, towers in sequence; median {one['median_s']:.2f} s (OpenFHE CPU baseline "
+                  f"unavailable on this box)",
+        "all_cores": dict(many, sample="the same key switch, towers and conversion targets in parallel (OpenMP)"),
+        "host": host,
     }, exact
 
 

@@ -53,12 +53,21 @@ def seeded(n, moduli, seed):
     return np.concatenate([rng.integers(0, q, n, dtype=np.uint64) for q in moduli])
 
 
-def ntt_roofline(towers, reps, stream, bits=59):
+def tuned(ctx, towers):
+    """Run the device autotuner on a context (before any timed work) and keep
+    the short form of its report: the choice and the NTT default/best times."""
+    rep = hip.autotune(ctx, towers)
+    return {"towers": rep["towers"], "chosen": rep["chosen"],
+            "ntt_default_ms": round(rep["ntt"]["default_ms"], 4), "ntt_tuned_ms": round(rep["ntt"]["best_ms"], 4)}
+
+
+def ntt_roofline(towers, reps, stream, bits=59, autotune=True):
     """Forward NTT over `towers` distinct 2^16-point towers (one prime just below
     2^bits): the key switch's dominant kernel pair, measured with HIP events on
     its stream. bits <= 30 runs the 32-bit butterflies of narrow towers."""
     q = params.last_prime(bits, 2 * N)
     ctx = hip.Context(N, [q], [params.min_root(q, N)])
+    tuning = tuned(ctx, min(towers, 1024)) if autotune else None
     data = hip.DeviceBuffer(towers * N)
     chunk = 256
     for t0 in range(0, towers, chunk):
@@ -93,6 +102,7 @@ def ntt_roofline(towers, reps, stream, bits=59):
         "launch_ms": round(ms, 4),
         "inverse_launch_ms": round(statistics.mean(inv), 4),
         "alg_bytes_per_launch": alg_bytes,
+        "autotune": tuning,
     }
 
 
@@ -288,10 +298,15 @@ def served_stream(requests, rates, device_rates, seed=0xC5):
         if mode == "host":
             exact = all(np.array_equal(w[0][0], ref[0]) and (ref[1] is None or np.array_equal(w[0][1], ref[1]))
                         for w, ref in zip(waited, reference))
+        by_op = {}
+        for w, r in zip(waited, reqs):
+            by_op.setdefault(serve.OP_NAMES[r.op], []).append(w[2])
         results.append({"mode": mode, "offered_rate_per_s": rate, "requests": len(reqs),
                         "throughput_per_s": round(len(reqs) / wall, 1),
                         "p50_ms": round(pct(lat, 0.5), 3), "p99_ms": round(pct(lat, 0.99), 3),
-                        "mean_batch": round(statistics.mean(w[2] for w in waited), 2), "all_equal_alone": exact})
+                        "mean_batch": round(statistics.mean(w[2] for w in waited), 2),
+                        "mean_batch_by_op": {k: round(statistics.mean(v), 2) for k, v in sorted(by_op.items())},
+                        "all_equal_alone": exact})
 
     def report():
         r = results[-1]
@@ -328,7 +343,31 @@ def served_stream(requests, rates, device_rates, seed=0xC5):
     stats = srv.stats()
     srv.close()
     alone_srv.close()
-    return {"rates": results, "peak_device_bytes": stats["peak_device_bytes"], "max_in_flight": stats["max_in_flight"],
+    # the alone results against the oracle: two requests of every op
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as o
+    o.set_threads(int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0)))
+    checked, equal, per_op = 0, True, {}
+    for r, ref in zip(reqs, reference):
+        if per_op.get(r.op, 0) >= 2:
+            continue
+        per_op[r.op] = per_op.get(r.op, 0) + 1
+        hp, kb, ka = ms.sets[(r.n, r.level, r.dnum)]
+        if r.op == serve.KEYSWITCH:
+            want = o.key_switch(r.data, r.n, hp.q, hp.q_psi, hp.M, hp.p, hp.p_psi, hp.alpha, kb, ka)
+        elif r.op in (serve.FORWARD, serve.INVERSE):
+            want = (o.ntt_towers(r.data, r.n, hp.q, hp.q_psi, inverse=r.op == serve.INVERSE), None)
+        elif r.op == serve.MODUP:
+            want = (o.mod_up(r.data, r.n, hp.q, hp.q_psi, hp.p, hp.p_psi, evaluation=True), None)
+        else:
+            want = (o.mod_down(r.data, r.n, hp.q, hp.q_psi, hp.p, hp.p_psi), None)
+        equal = equal and np.array_equal(ref[0], want[0]) and (want[1] is None or np.array_equal(ref[1], want[1]))
+        checked += 1
+    progress(f"serve: {checked} alone results vs the oracle: equal={equal}")
+    return {"rates": results, "peak_device_bytes": stats["peak_device_bytes"],
+            "resident_bytes": stats["resident_bytes"], "plans_resident": stats["plans_resident"],
+            "plans_evicted": stats["plans_evicted"], "max_in_flight": stats["max_in_flight"],
+            "budget_bytes": 48 << 30, "oracle_sample": {"checked": checked, "equal": equal},
             "mix": "40% KeySwitch, 30% NTT/INTT, 30% ModUp/ModDown; N in 2^13..2^16; L in [3,30]; dnum in {2,3,4}"}
 
 
@@ -518,6 +557,7 @@ def main():
     ap.add_argument("--rotations", type=int, default=8)
     ap.add_argument("--session-requests", type=int, default=16)
     ap.add_argument("--rotation-batch", type=int, default=16)
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -530,7 +570,7 @@ def main():
         # For counter runs: the NTT roofline launches plus a plain copy kernel
         # of known size (1 GiB read + 1 GiB written) to calibrate the counters.
         s = hip.Stream(0)
-        roof = ntt_roofline(args.ntt_towers, 12, s.handle, bits=args.roofline_bits)
+        roof = ntt_roofline(args.ntt_towers, 12, s.handle, bits=args.roofline_bits, autotune=not args.no_autotune)
         words = 1 << 27
         a, b = hip.DeviceBuffer(words), hip.DeviceBuffer(words)
         e0, e1 = hip.Event(), hip.Event()
@@ -546,6 +586,9 @@ def main():
     hp = params.openfhe_like(N, L, K, ALPHA)
     B = args.batch
     ctx = hip.Context(N, hp.q + hp.p, hp.q_psi + hp.p_psi)
+    ks_tuning = tuned(ctx, 512) if not args.no_autotune else None
+    if ks_tuning:
+        progress(f"autotune: {ks_tuning['chosen']}")
     ks = hip.KeySwitch(ctx, L, K, ALPHA, max_batch=B)
     full = hp.q + hp.p
     beta = hp.beta()
@@ -607,9 +650,9 @@ def main():
     wrappers = in_place_and_relinearize(ks, x, s, B)
     progress(f"in-place key switch: {wrappers['in_place']['keyswitch_per_s']} KS/s, "
              f"relinearize: {wrappers['relinearize']['keyswitch_per_s']} KS/s")
-    roof = ntt_roofline(args.ntt_towers, 12, s)
+    roof = ntt_roofline(args.ntt_towers, 12, s, autotune=not args.no_autotune)
     progress(f"ntt roofline done: {roof['achieved']} GB/s")
-    narrow = ntt_roofline(args.ntt_towers, 12, s, bits=30)
+    narrow = ntt_roofline(args.ntt_towers, 12, s, bits=30, autotune=not args.no_autotune)
     progress(f"narrow (30-bit) ntt roofline: {narrow['achieved']} GB/s")
     valu = valu_roofline(roof, narrow)
     if valu:
@@ -644,6 +687,7 @@ def main():
         "config": {"workload": "batched Hybrid key switch, N=65536, L=24 Q (60+50-bit), K=6 P (60-bit), "
                                "dnum=4 (alpha=6), OpenFHE-shaped primes and minimum roots",
                    "batch": B, "graph": True},
+        "autotune": ks_tuning,
         "batched_p50_ms": round(pct(step_ms, 0.5), 3),
         "batched_p99_ms": round(pct(step_ms, 0.99), 3),
         "plain_keyswitch_per_s": round(plain_kss, 2),

@@ -27,7 +27,8 @@ struct ParamSet {
 
 struct Options {
   int device = 0;
-  size_t hbm_budget_bytes = size_t{32} << 30;  // request buffers + workspaces in flight
+  size_t hbm_budget_bytes = size_t{32} << 30;  // everything the server holds on the device: resident plans and keys,
+                                               // request buffers and workspaces in flight
   size_t streams = 4;                          // concurrent batches (hardware queues)
   size_t max_batch = 32;                       // requests per launch
 };
@@ -40,7 +41,10 @@ struct Result {
 
 struct Stats {
   size_t submitted = 0, completed = 0, batches = 0, max_in_flight = 0;
+  // device bytes held (slot arenas + resident plans) now and at the peak
   size_t device_bytes_in_flight = 0, peak_device_bytes = 0;
+  size_t resident_bytes = 0;  // contexts, keys and per-ring pools kept across requests
+  size_t plans_resident = 0, plans_evicted = 0;
 };
 
 class Server {

@@ -86,13 +86,27 @@ int ntt_hip_context_device(ntt_hip_context *ctx);
  * resident_driver.hpp:28 / resident_adapter.inc:171-174) */
 size_t ntt_hip_context_table_bytes(ntt_hip_context *ctx);
 
+/* Device bytes a context holds: its twiddle tables and prime records plus the
+ * basis-conversion plans and scale tables it has cached (admission budgets). */
+int ntt_hip_context_device_bytes(ntt_hip_context *ctx, size_t *bytes);
+
 /* ---- autotuner (no reference counterpart): times candidate launch
- * geometries on `towers` scratch towers and keeps the fastest in the
- * context — the NTT tower chunk (towers per pass-A/pass-B pair; 0 = all) and
- * basis-conversion targets per workgroup. `report` receives a JSON line. ---- */
+ * shapes on `towers` scratch towers (forward + inverse transforms of the
+ * context's first prime; a first-half -> rest basis switch) and keeps the
+ * fastest in the context, one knob at a time, a candidate replacing the
+ * current choice only when clearly faster: for N = 2^16 the forward headroom
+ * (reductions skipped while residues stay below 2^64, or one per stage), the
+ * row-pass residues per thread (8 or 16) and the workgroups per CU (occupancy
+ * cap); for every N the tower chunk (towers per pass-A/pass-B pair; 0 = all)
+ * and the basis-conversion targets per workgroup. `report` receives a JSON
+ * line with every timing and the choice. Every shape is bit-exact. ---- */
 int ntt_hip_autotune(ntt_hip_context *ctx, size_t towers, char *report, size_t capacity);
 int ntt_hip_set_tuning(ntt_hip_context *ctx, uint32_t ntt_chunk, uint32_t conv_targets);
 int ntt_hip_get_tuning(ntt_hip_context *ctx, uint32_t *ntt_chunk, uint32_t *conv_targets);
+/* fwd_headroom: -1 = from the moduli (default) or at most 0..5; row_residues:
+ * 8 (default) or 16; wg_per_cu: 0 (no cap, default) or 1..15. */
+int ntt_hip_set_ntt_shape(ntt_hip_context *ctx, int fwd_headroom, uint32_t row_residues, uint32_t wg_per_cu);
+int ntt_hip_get_ntt_shape(ntt_hip_context *ctx, int *fwd_headroom, uint32_t *row_residues, uint32_t *wg_per_cu);
 
 /* ---- Driver seam operations (include/ntt/resident_driver.hpp:13-57) on
  * contiguous towers [first, first + towers) of the context basis ---- */
@@ -106,6 +120,20 @@ int ntt_hip_transform(ntt_hip_context *ctx, uint64_t *data, const uint64_t *src,
 int ntt_hip_transform_batch(ntt_hip_context *ctx, uint64_t *data, const uint64_t *src, size_t polys,
                             size_t poly_stride, size_t src_poly_stride, size_t first, size_t towers, int inverse,
                             void *stream);
+/* Transforms of arbitrary towers in one launch (no reference counterpart:
+ * the reference launches one basis per context): job j transforms the tower
+ * at data + jobs[j].offset (reading src + jobs[j].src_offset when src != NULL)
+ * with the prime of context tower jobs[j].tower. Polynomials of different
+ * tower counts, levels or parameter sets that share the context's primes run
+ * together. The host array is checked (tower indices) and staged into
+ * `jobs_workspace` (device, count * sizeof(ntt_hip_tower_job) bytes) on
+ * `stream`, which must stay untouched until the launch has run. */
+typedef struct ntt_hip_tower_job {
+  uint64_t offset, src_offset; /* elements */
+  uint32_t tower, reserved;
+} ntt_hip_tower_job;
+int ntt_hip_transform_jobs(ntt_hip_context *ctx, uint64_t *data, const uint64_t *src, const ntt_hip_tower_job *jobs,
+                           size_t count, void *jobs_workspace, int inverse, void *stream);
 /* Driver::pointwise (resident_adapter.inc:204-213): out[i] = out[i] * right[i] mod q */
 int ntt_hip_pointwise(ntt_hip_context *ctx, uint64_t *out, const uint64_t *right, size_t first, size_t towers,
                       void *stream);
@@ -144,6 +172,8 @@ int ntt_hip_hybrid_rotate(ntt_hip_context *ctx, uint64_t *out_b, uint64_t *out_a
 int ntt_hip_keyswitch_create(ntt_hip_keyswitch **ks, ntt_hip_context *ctx, size_t L, size_t K, size_t alpha,
                              size_t max_batch);
 int ntt_hip_keyswitch_destroy(ntt_hip_keyswitch *ks);
+/* Device bytes the plan owns (resident key, job tables), beside its context's. */
+int ntt_hip_keyswitch_device_bytes(ntt_hip_keyswitch *ks, size_t *bytes);
 /* Resident evaluation key (Session::upload_hybrid_key, src/resident.cpp:492-529):
  * host key_b/key_a hold ceil(M/alpha) digits of the FULL context (M Q towers,
  * then the K P towers), digit-major, each (M + K) * n; q_full/psi_full are the

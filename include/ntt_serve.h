@@ -25,7 +25,8 @@ extern "C" {
 typedef struct ntt_serve ntt_serve;
 
 const char *ntt_serve_last_error(void);
-/* hbm_budget_bytes bounds request buffers and workspaces in flight; streams =
+/* hbm_budget_bytes bounds everything the server holds on the device (resident
+ * plans, keys and per-ring pools, request buffers and workspaces); streams =
  * concurrent batches; max_batch = requests per launch. */
 int ntt_serve_create(ntt_serve **server, int device, size_t hbm_budget_bytes, size_t streams, size_t max_batch);
 int ntt_serve_destroy(ntt_serve *server); /* drains every admitted request */
@@ -51,6 +52,9 @@ int ntt_serve_wait(ntt_serve *server, uint64_t ticket, uint64_t *out, size_t out
                    size_t out2_words, double *latency_ms, size_t *batch_size);
 /* submitted, completed, batches, max concurrent batches, device bytes held, peak */
 int ntt_serve_stats(ntt_serve *server, size_t *stats6);
+/* the first `count` (<= 9) of: the six above, resident bytes (contexts, keys,
+ * per-ring pools), resident plans, plans evicted to make room */
+int ntt_serve_stats_ex(ntt_serve *server, size_t *stats, size_t count);
 
 #pragma GCC visibility pop
 #ifdef __cplusplus

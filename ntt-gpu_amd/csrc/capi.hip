@@ -168,8 +168,21 @@ struct ntt_hip_context {
   int device = 0;
   uint32_t n = 0, log_n = 0;
   std::vector<uint64_t> q, psi;
-  int arith = kLazy;  // NTT arithmetic mode for every tower of the basis
+  int arith = kLazy;  // NTT arithmetic mode for every tower of the basis (with the largest forward headroom)
   uint32_t ntt_chunk = 0;      // towers per NTT pass pair (0 = all / default), set by the autotuner
+  // NTT shape chosen by the autotuner (defaults: the largest forward headroom
+  // the moduli allow, 8 residues per thread in the row pass, no occupancy cap)
+  int fwd_headroom = -1;       // -1: from the moduli; 0..kMaxHeadroom: at most that
+  bool row16 = false;          // 16 residues per thread in the N = 2^16 row pass
+  uint32_t wg_cap = 0;         // workgroups per CU for the N = 2^16 passes (0 = no cap)
+  int ntt_mode() const {
+    int a = arith;
+    if ((a & kArithMask) == kLazy && fwd_headroom >= 0) {
+      const int h = std::min(fwd_headroom, (a >> kArithShift) & 7);
+      a = kLazy | (h << kArithShift);
+    }
+    return nttamd::ntt_mode(a, row16, wg_cap);
+  }
   // Standalone transforms (ntt_hip_transform*) with no tuned chunk run the
   // towers in groups whose pass-A output (the hand-off pass B reads back)
   // fits the 256 MB Infinity Cache: measured at N = 2^16 on 2048 towers,
@@ -695,6 +708,47 @@ size_t ntt_hip_context_table_bytes(ntt_hip_context *ctx) {
   return ctx ? ctx->tables->bytes + ctx->primes->bytes : 0;
 }
 
+int ntt_hip_context_device_bytes(ntt_hip_context *ctx, size_t *bytes) {
+  return guarded([&] {
+    require(ctx && bytes, "null argument");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    size_t total = ctx->tables->bytes + ctx->primes->bytes;
+    for (const auto &kv : ctx->basis_cache)
+      if (kv.second && kv.second->blob) total += kv.second->blob->bytes;
+    for (const auto &kv : ctx->scale_cache)
+      if (kv.second) total += kv.second->bytes;
+    *bytes = total;
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_transform_jobs(ntt_hip_context *ctx, uint64_t *data, const uint64_t *src, const ntt_hip_tower_job *jobs,
+                           size_t count, void *jobs_workspace, int inverse, void *stream) {
+  return guarded([&] {
+    require(ctx && data && jobs && jobs_workspace, "null argument");
+    require(count >= 1 && count <= 65535, "1..65535 tower jobs per launch");
+    static_assert(sizeof(ntt_hip_tower_job) == sizeof(TowerJob), "job record layout");
+    for (size_t j = 0; j < count; ++j)
+      require(jobs[j].tower < ctx->q.size(), "job tower outside the context basis");
+    const hipStream_t s = ctx->pick(stream);
+    DeviceGuard g(ctx->device);
+    // the table is staged from the caller's host array on the launch stream
+    HIP_CHECK(hipMemcpyAsync(jobs_workspace, jobs, count * sizeof(TowerJob), hipMemcpyHostToDevice, s));
+    TowerMap m{};
+    m.base = data;
+    m.src = src;
+    m.jobs = (const TowerJob *)jobs_workspace;
+    m.poly_stride = 0;
+    m.src_poly_stride = 0;
+    m.tower_stride = ctx->n;
+    m.towers_per_poly = (uint32_t)count;
+    hip_ok(launch_ntt(m, ctx->dprimes(), ctx->log_n, (uint32_t)count, inverse != 0, ctx->ntt_mode(), s,
+                      ctx->transform_chunk()),
+           "ntt job launch");
+    return NTT_HIP_OK;
+  });
+}
+
 int ntt_hip_transform(ntt_hip_context *ctx, uint64_t *data, const uint64_t *src, size_t first, size_t towers,
                       int inverse, void *stream) {
   return ntt_hip_transform_batch(ctx, data, src, 1, 0, 0, first, towers, inverse, stream);
@@ -711,7 +765,7 @@ int ntt_hip_transform_batch(ntt_hip_context *ctx, uint64_t *data, const uint64_t
             "polynomial stride shorter than its towers");
     DeviceGuard g(ctx->device);
     const TowerMap m = affine(data, src, ctx->n, towers, first, poly_stride, src_poly_stride);
-    hip_ok(launch_ntt(m, ctx->dprimes(), ctx->log_n, (uint32_t)(polys * towers), inverse != 0, ctx->arith,
+    hip_ok(launch_ntt(m, ctx->dprimes(), ctx->log_n, (uint32_t)(polys * towers), inverse != 0, ctx->ntt_mode(),
                       ctx->pick(stream), ctx->transform_chunk()),
            "ntt launch");
     return NTT_HIP_OK;
@@ -922,6 +976,17 @@ int ntt_hip_keyswitch_create(ntt_hip_keyswitch **out, ntt_hip_context *ctx, size
   });
 }
 
+int ntt_hip_keyswitch_device_bytes(ntt_hip_keyswitch *ks, size_t *bytes) {
+  return guarded([&] {
+    require(ks && bytes, "null argument");
+    size_t total = 0;
+    for (const DeviceBlob *b : {ks->key.get(), ks->jobs.get(), ks->tower_prime.get()})
+      if (b) total += b->bytes;
+    *bytes = total;
+    return NTT_HIP_OK;
+  });
+}
+
 int ntt_hip_keyswitch_destroy(ntt_hip_keyswitch *ks) {
   return guarded([&] {
     delete ks;
@@ -1009,7 +1074,7 @@ void ks_modup_convert(const ntt_hip_keyswitch &ks, const uint64_t *in, size_t B,
 void ks_modup(const ntt_hip_keyswitch &ks, const uint64_t *in, size_t B, const KsBuffers &w, hipStream_t s) {
   const ntt_hip_context &c = *ks.ctx;
   ks_modup_convert(ks, in, B, w, s);
-  hip_ok(launch_ntt(complement_jobs(ks, w), c.dprimes(), c.log_n, (uint32_t)(B * ks.jobs_per_poly), false, c.arith,
+  hip_ok(launch_ntt(complement_jobs(ks, w), c.dprimes(), c.log_n, (uint32_t)(B * ks.jobs_per_poly), false, c.ntt_mode(),
                     s, c.ntt_chunk),
          "modup ntt");
 }
@@ -1019,7 +1084,7 @@ void ks_modup_convert(const ntt_hip_keyswitch &ks, const uint64_t *in, size_t B,
   const size_t L = ks.L, T = ks.L + ks.K, n = c.n;
   const PrimeConst *pr = c.dprimes();
   hip_ok(launch_ntt(affine(w.coeff, in, n, L, 0, ks.coeff_words(), L * n), pr, c.log_n, (uint32_t)(B * L), true,
-                    c.arith, s, c.ntt_chunk),
+                    c.ntt_mode(), s, c.ntt_chunk),
          "intt");
   for (uint32_t d = 0; d < ks.beta; ++d) {
     const auto &plan = *ks.up[d];
@@ -1085,7 +1150,7 @@ void ks_modup_mac(const ntt_hip_keyswitch &key, const ntt_hip_keyswitch &plan, c
     const uint32_t t1 = std::min(T, t0 + G);
     const uint32_t k0 = plan.tower_job_begin[t0], k1 = plan.tower_job_begin[t1];
     hip_ok(launch_ntt_span(m, c.dprimes(), c.log_n, (uint32_t)B, k0 * (uint32_t)B, (k1 - k0) * (uint32_t)B, false,
-                           c.arith, s),
+                           c.ntt_mode(), s),
            "modup ntt group");
     ks_mac(key, plan, in, B, w, z, s, t0, t1 - t0);
   }
@@ -1102,7 +1167,7 @@ void ks_moddown(const ntt_hip_keyswitch &ks, const uint64_t *z, size_t B, uint64
   const size_t L = ks.L, K = ks.K, n = c.n;
   const PrimeConst *pr = c.dprimes();
   hip_ok(launch_ntt(affine(w.pc, z + L * n, n, K, L, ks.pc_words(), ks.z_words()), pr, c.log_n,
-                    (uint32_t)(2 * B * K), true, c.arith, s, c.ntt_chunk),
+                    (uint32_t)(2 * B * K), true, c.ntt_mode(), s, c.ntt_chunk),
          "moddown intt");
   BasisLaunch a{};
   a.plan = ks.down->dev;
@@ -1120,7 +1185,7 @@ void ks_moddown(const ntt_hip_keyswitch &ks, const uint64_t *z, size_t B, uint64
   TowerMap fin = with_correction(affine(w.corr, nullptr, n, L, 0, ks.corr_words(), 0), z, ks.z_words(), out_b, out_a,
                                  (uint32_t)B, L * n, ks.pinv);
   fin.epi.accumulate = accumulate;
-  hip_ok(launch_ntt(fin, pr, c.log_n, (uint32_t)(2 * B * L), false, c.arith, s, c.ntt_chunk),
+  hip_ok(launch_ntt(fin, pr, c.log_n, (uint32_t)(2 * B * L), false, c.ntt_mode(), s, c.ntt_chunk),
          "moddown ntt + correction");
 }
 }  // namespace
@@ -1444,18 +1509,18 @@ int ntt_hip_modup_run(ntt_hip_keyswitch *ks, const uint64_t *in, int in_eval, si
     DeviceGuard g(c.device);
     if (in_eval) {
       // Q part of the output is the input itself; coefficients for the lift
-      hip_ok(launch_ntt(affine(coeff, in, n, L, 0, L * n, L * n), pr, c.log_n, (uint32_t)(batch * L), true, c.arith, s, c.ntt_chunk),
+      hip_ok(launch_ntt(affine(coeff, in, n, L, 0, L * n, L * n), pr, c.log_n, (uint32_t)(batch * L), true, c.ntt_mode(), s, c.ntt_chunk),
              "modup intt");
       for (size_t b = 0; b < batch; ++b)
         HIP_CHECK(hipMemcpyAsync(out + b * T * n, in + b * L * n, L * n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
       launch_basis(*ks->full_up, c.conv_targets, coeff, out + L * n, L * n, T * n, (uint32_t)n, (uint32_t)batch, s);
     } else {
-      hip_ok(launch_ntt(affine(out, in, n, L, 0, T * n, L * n), pr, c.log_n, (uint32_t)(batch * L), false, c.arith, s, c.ntt_chunk),
+      hip_ok(launch_ntt(affine(out, in, n, L, 0, T * n, L * n), pr, c.log_n, (uint32_t)(batch * L), false, c.ntt_mode(), s, c.ntt_chunk),
              "modup ntt q");
       launch_basis(*ks->full_up, c.conv_targets, in, out + L * n, L * n, T * n, (uint32_t)n, (uint32_t)batch, s);
     }
     hip_ok(launch_ntt(affine(out + L * n, nullptr, n, K, L, T * n, 0), pr, c.log_n, (uint32_t)(batch * K), false,
-                      c.arith, s, c.ntt_chunk),
+                      c.ntt_mode(), s, c.ntt_chunk),
            "modup ntt p");
     return NTT_HIP_OK;
   });
@@ -1481,13 +1546,13 @@ int ntt_hip_moddown_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batch,
     uint64_t *pc = (uint64_t *)workspace;  // batch * K * n
     uint64_t *corr = pc + batch * K * n;   // batch * L * n
     DeviceGuard g(c.device);
-    hip_ok(launch_ntt(affine(pc, in + L * n, n, K, L, K * n, T * n), pr, c.log_n, (uint32_t)(batch * K), true, c.arith,
+    hip_ok(launch_ntt(affine(pc, in + L * n, n, K, L, K * n, T * n), pr, c.log_n, (uint32_t)(batch * K), true, c.ntt_mode(),
                       s, c.ntt_chunk),
            "moddown intt");
     launch_basis(*ks->down, c.conv_targets, pc, corr, K * n, L * n, (uint32_t)n, (uint32_t)batch, s);
     hip_ok(launch_ntt(with_correction(affine(corr, nullptr, n, L, 0, L * n, 0), in, T * n, out, out, (uint32_t)batch,
                                       L * n, ks->pinv),
-                      pr, c.log_n, (uint32_t)(batch * L), false, c.arith, s, c.ntt_chunk),
+                      pr, c.log_n, (uint32_t)(batch * L), false, c.ntt_mode(), s, c.ntt_chunk),
            "moddown ntt + correction");
     return NTT_HIP_OK;
   });
@@ -1548,6 +1613,34 @@ int ntt_hip_get_tuning(ntt_hip_context *ctx, uint32_t *ntt_chunk, uint32_t *conv
   });
 }
 
+int ntt_hip_set_ntt_shape(ntt_hip_context *ctx, int fwd_headroom, uint32_t row_residues, uint32_t wg_per_cu) {
+  return guarded([&] {
+    require(ctx != nullptr, "null context");
+    require(fwd_headroom >= -1 && fwd_headroom <= kMaxHeadroom, "forward headroom must be -1 (from the moduli) or 0..5");
+    require(row_residues == 8 || row_residues == 16, "row-pass residues per thread must be 8 or 16");
+    require(wg_per_cu <= 15, "workgroups per CU must be 0 (no cap) .. 15");
+    ctx->fwd_headroom = fwd_headroom;
+    ctx->row16 = row_residues == 16;
+    ctx->wg_cap = wg_per_cu;
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_get_ntt_shape(ntt_hip_context *ctx, int *fwd_headroom, uint32_t *row_residues, uint32_t *wg_per_cu) {
+  return guarded([&] {
+    require(ctx && fwd_headroom && row_residues && wg_per_cu, "null argument");
+    *fwd_headroom = ctx->fwd_headroom;
+    *row_residues = ctx->row16 ? 16 : 8;
+    *wg_per_cu = ctx->wg_cap;
+    return NTT_HIP_OK;
+  });
+}
+
+// Coordinate search: each knob in turn over its candidates (the others held
+// at their best so far), a candidate replacing the incumbent only when it is
+// clearly faster. NTT knobs are timed on `towers` forward + inverse transforms
+// of context tower 0's prime, the conversion width on a first-half -> rest
+// basis switch of the context.
 int ntt_hip_autotune(ntt_hip_context *ctx, size_t towers, char *report, size_t capacity) {
   return guarded([&] {
     require(ctx && towers >= 1 && towers <= 65535, "autotune needs a context and 1..65535 towers");
@@ -1562,64 +1655,102 @@ int ntt_hip_autotune(ntt_hip_context *ctx, size_t towers, char *report, size_t c
     DeviceBlob scratch(c.device, (a_words + b_words) * sizeof(uint64_t));
     uint64_t *a = (uint64_t *)scratch.ptr, *b = a + a_words;
     HIP_CHECK(hipMemsetAsync(a, 0, (a_words + b_words) * sizeof(uint64_t), c.stream));
-    // NTT chunk: every tower uses context tower 0 (one prime, as a batch does)
-    std::string rep = "{\"ntt\":{";
-    uint32_t best_chunk = 0;
-    float best = 1e30f;
-    for (uint32_t chunk : {0u, 1024u, 512u, 256u, 128u, 64u, 32u}) {
-      if (chunk && chunk >= towers) continue;
-      TowerMap m = affine(a, nullptr, n, 1, 0, n, 0);
-      const float ms = time_ms(c, [&] {
-        hip_ok(launch_ntt(m, c.dprimes(), c.log_n, (uint32_t)towers, false, c.arith, c.stream, chunk), "tune ntt");
-        hip_ok(launch_ntt(m, c.dprimes(), c.log_n, (uint32_t)towers, true, c.arith, c.stream, chunk), "tune ntt");
+    const TowerMap m = affine(a, nullptr, n, 1, 0, n, 0);
+    // one forward and one inverse transform as a standalone transform runs
+    // them (the context's chunking)
+    auto ntt_ms = [&] {
+      return time_ms(c, [&] {
+        hip_ok(launch_ntt(m, c.dprimes(), c.log_n, (uint32_t)towers, false, c.ntt_mode(), c.stream,
+                          c.transform_chunk()),
+               "tune ntt");
+        hip_ok(launch_ntt(m, c.dprimes(), c.log_n, (uint32_t)towers, true, c.ntt_mode(), c.stream,
+                          c.transform_chunk()),
+               "tune ntt");
       }, 3);
-      rep += "\"" + std::to_string(chunk) + "\":" + std::to_string(ms) + ",";
-      if (ms < best * 0.98f) {  // prefer fewer launches unless clearly faster
-        best = ms;
-        best_chunk = chunk;
+    };
+    for (int w = 0; w < 4; ++w) (void)ntt_ms();  // steady clocks before the first comparison
+    std::string rep = "{\"ntt\":{";
+    const float first = ntt_ms();
+    rep += "\"default_ms\":" + std::to_string(first);
+    // a knob: each candidate against the incumbent in alternating pairs (the
+    // clock drifts less within a pair than across a sweep); the incumbent is
+    // kept unless the candidate is at least 2% faster in the sum of 3 pairs
+    auto sweep = [&](const char *name, size_t count, const std::function<void(size_t)> &set, size_t incumbent) {
+      rep += ",\"" + std::string(name) + "\":{";
+      size_t pick = incumbent;
+      for (size_t i = 0; i < count; ++i) {
+        if (i == pick) continue;
+        float t_inc = 0, t_cand = 0;
+        for (int r = 0; r < 3; ++r) {
+          set(pick);
+          t_inc += ntt_ms();
+          set(i);
+          t_cand += ntt_ms();
+        }
+        rep += (rep.back() == '{' ? "" : ",") + std::string("\"") + std::to_string(i) + "\":[" +
+               std::to_string(t_cand / 3) + "," + std::to_string(t_inc / 3) + "]";
+        if (t_cand < 0.98f * t_inc) pick = i;
       }
+      set(pick);
+      rep += "}";
+    };
+    const bool big = c.log_n == 16 && (c.arith & kArithMask) != kExact;  // the 2^16 two-pass kernels
+    const int hmax = (c.arith >> kArithShift) & 7;
+    if (big) {
+      // forward headroom: from the moduli (index 0) or none (index 1)
+      const int heads[2] = {-1, 0};
+      sweep("fwd_headroom", hmax > 0 ? 2 : 1, [&](size_t i) { c.fwd_headroom = heads[i]; }, 0);
+      const uint32_t rows[2] = {8, 16};
+      sweep("row_residues", 2, [&](size_t i) { c.row16 = rows[i] == 16; }, 0);
+      const uint32_t caps[4] = {0, 6, 5, 4};
+      sweep("wg_per_cu", 4, [&](size_t i) { c.wg_cap = caps[i]; }, 0);
     }
-    rep.back() = '}';
-    // conversion geometry: first half of the basis -> second half (<= 6 -> <= 24)
-    uint32_t best_targets = c.conv_targets;
-    if (c.q.size() >= 2) {
-      const size_t s = cs, t = ct;
+    // tower chunk: 0 (the context's default grouping) or groups smaller than the run
+    std::vector<uint32_t> chunks = {0};
+    for (uint32_t ch : {1024u, 512u, 448u, 256u, 128u, 64u})
+      if (ch < towers) chunks.push_back(ch);
+    sweep("ntt_chunk", chunks.size(), [&](size_t i) { c.ntt_chunk = chunks[i]; }, 0);
+    rep += ",\"best_ms\":" + std::to_string(ntt_ms()) + "}";
+    // conversion geometry: first half of the basis -> the rest (<= 6 -> <= 24)
+    if (cs >= 1 && ct >= 1) {
       std::vector<size_t> src, dst;
-      for (size_t i = 0; i < s; ++i) src.push_back(i);
-      for (size_t i = 0; i < t; ++i) dst.push_back(s + i);
+      for (size_t i = 0; i < cs; ++i) src.push_back(i);
+      for (size_t i = 0; i < ct; ++i) dst.push_back(cs + i);
       auto plan = basis_plan(c, src, dst);
-      const uint32_t polys = (uint32_t)cpolys;  // polys * s <= a_words / n, polys * t <= b_words / n
+      const uint32_t polys = (uint32_t)cpolys;  // polys * cs <= a_words / n, polys * ct <= b_words / n
       rep += ",\"conversion\":{";
-      best = 1e30f;
+      float cbest = 1e30f;
+      uint32_t best_targets = 0;
       for (uint32_t tpb : {0u, 1u, 2u, 4u, 6u, 8u, 12u, 16u, 24u}) {
-        if (tpb > t) continue;
+        if (tpb > ct) continue;
         BasisLaunch bl{};
         bl.plan = plan->dev;
         bl.in = a;
         bl.out = b;
-        bl.in_poly_stride = s * n;
-        bl.out_poly_stride = t * n;
+        bl.in_poly_stride = cs * n;
+        bl.out_poly_stride = ct * n;
         bl.n = (uint32_t)n;
         bl.targets_per_block = tpb;
         bl.plan_host_s = plan->s;
         bl.plan_host_t = plan->t;
         bl.skip_at = 0xffffffffu;
-        const float ms = time_ms(c, [&] { hip_ok(launch_basis_switch(bl, plan->per_term, polys, c.stream), "tune conv"); }, 3);
-        rep += "\"" + std::to_string(tpb) + "\":" + std::to_string(ms) + ",";
-        // 0 (geometry from the grid size, first candidate) adapts to every
-        // batch size; a fixed width must be clearly faster to replace it
-        if (tpb == 0 ? ms < best : ms < best * 0.95f) {
-          best = ms;
+        const float ms = time_ms(c, [&] { hip_ok(launch_basis_switch(bl, plan->per_term, polys, c.stream), "tune conv"); }, 4);
+        rep += (rep.back() == '{' ? "" : ",") + std::string("\"") + std::to_string(tpb) + "\":" + std::to_string(ms);
+        // 0 (geometry from the grid size) adapts to every batch size; a fixed
+        // width must be clearly faster to replace it
+        if (tpb == 0 ? ms < cbest : ms < cbest * 0.95f) {
+          cbest = ms;
           best_targets = tpb;
         }
       }
-      rep.back() = '}';
+      c.conv_targets = best_targets;
+      rep += "}";
     }
     HIP_CHECK(hipStreamSynchronize(c.stream));
-    c.ntt_chunk = best_chunk;
-    c.conv_targets = best_targets;
-    rep += ",\"chosen\":{\"ntt_chunk\":" + std::to_string(best_chunk) + ",\"conv_targets\":" +
-           std::to_string(best_targets) + "},\"towers\":" + std::to_string(towers) + ",\"n\":" + std::to_string(n) + "}";
+    rep += ",\"chosen\":{\"fwd_headroom\":" + std::to_string(c.fwd_headroom) +
+           ",\"row_residues\":" + std::to_string(c.row16 ? 16 : 8) + ",\"wg_per_cu\":" + std::to_string(c.wg_cap) +
+           ",\"ntt_chunk\":" + std::to_string(c.ntt_chunk) + ",\"conv_targets\":" + std::to_string(c.conv_targets) +
+           "},\"towers\":" + std::to_string(towers) + ",\"n\":" + std::to_string(n) + "}";
     if (report && capacity) std::snprintf(report, capacity, "%s", rep.c_str());
     return NTT_HIP_OK;
   });

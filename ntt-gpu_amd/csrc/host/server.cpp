@@ -4,20 +4,28 @@
 //   1. retires finished batches (event query; results copied out of the
 //      slot's pinned staging, tickets released),
 //   2. fills every idle stream slot with a new batch: the oldest pending
-//      request chooses the (params, level, op) group and all pending requests
-//      of that group join it, up to max_batch and as far as the device budget
-//      allows (admission by HBM footprint),
+//      request chooses the group and compatible pending requests join it, up
+//      to max_batch and as far as the device budget allows,
 //   3. sleeps briefly only when it neither retired nor launched anything.
-// So requests that arrive while others run are admitted as soon as a slot
-// and memory free up, and up to `streams` batches overlap on the device.
-// Contexts, conversion plans and evaluation keys are created on first use of
-// a (params, level) and stay resident.
+// Groups: NTT/INTT requests of one ring dimension share a launch whatever
+// their level or parameter set (a per-N context over every registered Q
+// prime and a tower job table); KeySwitch, ModUp and ModDown requests share
+// one when they have the same parameter set and level.
+// Admission counts the whole device footprint against the budget: the slot
+// arenas of the batches in flight plus every resident plan (contexts with
+// their twiddle tables and cached conversion plans, key-switch plans with
+// their evaluation keys, per-N pools). A batch that does not fit first frees
+// idle slot arenas, then evicts idle plans least recently used first (an
+// evicted plan is rebuilt and its key uploaded again on next use), and
+// otherwise waits for batches in flight to finish.
 #include <algorithm>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <map>
 #include <mutex>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -43,6 +51,7 @@ struct Request {
   uint32_t params = 0;
   Op op = Op::Forward;
   size_t level = 0;
+  uint64_t n = 0;
   std::vector<uint64_t> in;
   bool device = false;  // device-resident: the caller's buffers below
   const uint64_t *d_in = nullptr;
@@ -54,11 +63,30 @@ struct Request {
   bool claimed = false;  // a wait() has taken this ticket (a second one is an error)
 };
 
-struct LevelPlan {
+// Something resident on the device on behalf of many requests.
+struct Resident {
+  size_t device_bytes = 0;
+  uint64_t last_use = 0;
+  size_t inflight = 0;  // batches in flight using it (never evicted while > 0)
+  virtual ~Resident() = default;
+};
+
+// (parameter set, level): context over active Q then P, key-switch plan, key.
+struct LevelPlan : Resident {
   ntt_hip_context *ctx = nullptr;
   ntt_hip_keyswitch *ks = nullptr;
-  ~LevelPlan() {
+  ~LevelPlan() override {
     if (ks) ntt_hip_keyswitch_destroy(ks);
+    if (ctx) ntt_hip_context_destroy(ctx);
+  }
+};
+
+// Ring dimension n: one context over every Q prime registered for n (the
+// union over parameter sets), so NTT/INTT requests of any level share it.
+struct Pool : Resident {
+  ntt_hip_context *ctx = nullptr;
+  std::map<uint64_t, uint32_t> tower_of;  // prime -> context tower
+  ~Pool() override {
     if (ctx) ntt_hip_context_destroy(ctx);
   }
 };
@@ -70,7 +98,8 @@ struct Slot {
   void *pinned = nullptr;  // grow-only pinned host staging
   size_t pinned_bytes = 0;
   std::vector<std::shared_ptr<Request>> batch;
-  size_t out_words = 0, out2_words = 0, in_words = 0;
+  std::vector<size_t> out_at, out_words, out2_at;  // per request, words into the staged outputs
+  std::shared_ptr<Resident> holder;                // the plan or pool the batch runs on
   bool busy = false;
 };
 
@@ -79,16 +108,19 @@ struct Slot {
 struct Server::Impl {
   Options opt;
   std::vector<std::shared_ptr<const ParamSet>> params;  // shared: the worker keeps its copy
-  std::map<std::pair<uint32_t, size_t>, std::unique_ptr<LevelPlan>> plans;
+  std::map<std::pair<uint32_t, size_t>, std::shared_ptr<LevelPlan>> plans;
+  std::map<uint64_t, std::shared_ptr<Pool>> pools;  // by ring dimension
+  std::map<uint64_t, size_t> pool_sets;             // parameter sets a pool of n covers
   std::vector<Slot> slots;
   mutable std::mutex mu;
   std::condition_variable cv_submit, cv_done;
   std::deque<std::shared_ptr<Request>> pending;
   std::unordered_map<uint64_t, std::shared_ptr<Request>> tickets;
-  uint64_t next_ticket = 1;
+  uint64_t next_ticket = 1, tick = 0;
   bool stop = false;
   Stats stats;
-  size_t arena_bytes = 0;  // sum of device arenas (the admission budget)
+  size_t arena_bytes = 0;     // sum of slot arenas
+  size_t resident_bytes = 0;  // sum of resident plans and pools
   std::thread worker;
 
   size_t in_words(const ParamSet &p, Op op, size_t L) const {
@@ -97,25 +129,137 @@ struct Server::Impl {
   size_t out_words(const ParamSet &p, Op op, size_t L) const {
     return (op == Op::ModUp ? L + p.p.size() : L) * p.n;
   }
-
   std::shared_ptr<const ParamSet> param_set(uint32_t id) const {
     std::lock_guard<std::mutex> lock(mu);
     return params.at(id);
   }
+  size_t held() const { return arena_bytes + resident_bytes; }
+  void publish_memory() {
+    std::lock_guard<std::mutex> lock(mu);
+    stats.device_bytes_in_flight = held();
+    stats.resident_bytes = resident_bytes;
+    stats.peak_device_bytes = std::max(stats.peak_device_bytes, held());
+    stats.plans_resident = plans.size() + pools.size();
+  }
 
-  LevelPlan &plan(uint32_t id, size_t L) {
+  // ---- making room for `need` more bytes, the arena of keep_slot (which the
+  // need replaces) not counted: idle arenas first, then idle residents, least
+  // recently used first. keep (the plan the need is for) is never evicted.
+  bool fits(size_t need, const Slot *keep_slot) const {
+    return held() - (keep_slot ? keep_slot->device_bytes : 0) + need <= opt.hbm_budget_bytes;
+  }
+  bool make_room(size_t need, const Slot *keep_slot, const Resident *keep) {
+    if (fits(need, keep_slot)) return true;
+    for (Slot &o : slots)
+      if (&o != keep_slot && !o.busy && o.device) {
+        ok(ntt_hip_free(opt.device, o.device), "free");
+        arena_bytes -= o.device_bytes;
+        o.device = nullptr;
+        o.device_bytes = 0;
+        if (fits(need, keep_slot)) return true;
+      }
+    for (;;) {
+      Resident *victim = nullptr;
+      std::function<void()> erase;
+      for (auto it = plans.begin(); it != plans.end(); ++it)
+        if (it->second.get() != keep && it->second->inflight == 0 &&
+            (!victim || it->second->last_use < victim->last_use)) {
+          victim = it->second.get();
+          erase = [this, key = it->first] { plans.erase(key); };
+        }
+      for (auto it = pools.begin(); it != pools.end(); ++it)
+        if (it->second.get() != keep && it->second->inflight == 0 &&
+            (!victim || it->second->last_use < victim->last_use)) {
+          victim = it->second.get();
+          erase = [this, key = it->first] {
+            pools.erase(key);
+            pool_sets.erase(key);
+          };
+        }
+      if (!victim) return false;
+      resident_bytes -= victim->device_bytes;
+      {
+        std::lock_guard<std::mutex> lock(mu);
+        ++stats.plans_evicted;
+      }
+      erase();  // destroys the contexts and frees their device memory
+      if (fits(need, keep_slot)) return true;
+    }
+  }
+
+  // Device bytes of a new plan, before it exists: twiddle tables (two
+  // directions of 16-byte pairs per tower) and the active key (both channels).
+  static size_t plan_estimate(const ParamSet &p, size_t L) {
+    const size_t T = L + p.p.size(), beta = (L + p.alpha - 1) / p.alpha;
+    return T * p.n * 32 + 2 * beta * T * p.n * 8;
+  }
+
+  std::shared_ptr<LevelPlan> plan(uint32_t id, size_t L, const Slot &slot) {
     auto key = std::make_pair(id, L);
     auto it = plans.find(key);
-    if (it != plans.end()) return *it->second;
+    if (it != plans.end()) {
+      it->second->last_use = ++tick;
+      return it->second;
+    }
     const ParamSet &p = *param_set(id);
-    auto lp = std::make_unique<LevelPlan>();
+    (void)slot;
+    if (!make_room(plan_estimate(p, L), nullptr, nullptr)) return nullptr;
+    auto lp = std::make_shared<LevelPlan>();
     std::vector<uint64_t> q(p.q.begin(), p.q.begin() + L), r(p.q_psi.begin(), p.q_psi.begin() + L);
     q.insert(q.end(), p.p.begin(), p.p.end());
     r.insert(r.end(), p.p_psi.begin(), p.p_psi.end());
     ok(ntt_hip_context_create(&lp->ctx, opt.device, p.n, q.data(), r.data(), q.size()), "context");
     ok(ntt_hip_keyswitch_create(&lp->ks, lp->ctx, L, p.p.size(), p.alpha, opt.max_batch), "keyswitch plan");
     ok(ntt_hip_keyswitch_upload_key(lp->ks, p.key_b.data(), p.key_a.data(), p.q.size(), p.q.data()), "key");
-    return *plans.emplace(key, std::move(lp)).first->second;
+    size_t cb = 0, kb = 0;
+    ok(ntt_hip_context_device_bytes(lp->ctx, &cb), "context bytes");
+    ok(ntt_hip_keyswitch_device_bytes(lp->ks, &kb), "plan bytes");
+    lp->device_bytes = cb + kb;
+    lp->last_use = ++tick;
+    resident_bytes += lp->device_bytes;
+    plans.emplace(key, lp);
+    publish_memory();
+    return lp;
+  }
+
+  // The pool of ring dimension n over every Q prime registered for n (rebuilt
+  // when parameter sets were added since, and only while idle).
+  std::shared_ptr<Pool> pool(uint64_t n, const Slot &slot) {
+    std::vector<uint64_t> q, r;
+    size_t sets = 0;
+    {
+      std::lock_guard<std::mutex> lock(mu);
+      for (const auto &ps : params) {
+        if (ps->n != n) continue;
+        ++sets;
+        for (size_t t = 0; t < ps->q.size(); ++t)
+          if (std::find(q.begin(), q.end(), ps->q[t]) == q.end()) {
+            q.push_back(ps->q[t]);
+            r.push_back(ps->q_psi[t]);
+          }
+      }
+    }
+    auto it = pools.find(n);
+    if (it != pools.end() && (pool_sets[n] == sets || it->second->inflight > 0)) {
+      it->second->last_use = ++tick;
+      return it->second;
+    }
+    if (it != pools.end()) {  // idle and stale: rebuild over the larger union
+      resident_bytes -= it->second->device_bytes;
+      pools.erase(it);
+    }
+    (void)slot;
+    if (q.empty() || !make_room(q.size() * n * 32, nullptr, nullptr)) return nullptr;
+    auto pl = std::make_shared<Pool>();
+    ok(ntt_hip_context_create(&pl->ctx, opt.device, n, q.data(), r.data(), q.size()), "pool context");
+    for (size_t t = 0; t < q.size(); ++t) pl->tower_of[q[t]] = (uint32_t)t;
+    ok(ntt_hip_context_device_bytes(pl->ctx, &pl->device_bytes), "pool bytes");
+    pl->last_use = ++tick;
+    resident_bytes += pl->device_bytes;
+    pools[n] = pl;
+    pool_sets[n] = sets;
+    publish_memory();
+    return pl;
   }
 
   size_t workspace_bytes(LevelPlan &lp, Op op, size_t b) {
@@ -124,11 +268,6 @@ struct Server::Impl {
     else if (op == Op::ModUp) ok(ntt_hip_modup_workspace_bytes(lp.ks, b, &bytes), "workspace");
     else if (op == Op::ModDown) ok(ntt_hip_moddown_workspace_bytes(lp.ks, b, &bytes), "workspace");
     return bytes;
-  }
-
-  size_t footprint(const ParamSet &p, LevelPlan &lp, Op op, size_t L, size_t b) {
-    const size_t words = in_words(p, op, L) + out_words(p, op, L) * (op == Op::KeySwitch ? 2 : 1);
-    return b * words * 8 + workspace_bytes(lp, op, b);
   }
 
   void ensure_arena(Slot &s, size_t device_bytes, size_t pinned_bytes) {
@@ -150,117 +289,174 @@ struct Server::Impl {
     }
   }
 
-  // Launch one batch on an idle slot. Returns false if nothing could be admitted.
-  bool launch(Slot &slot) {
+  static bool is_transform(Op op) { return op == Op::Forward || op == Op::Inverse; }
+
+  // Take the head request's group off the queue (at most max_batch).
+  std::vector<std::shared_ptr<Request>> take_group() {
+    std::lock_guard<std::mutex> lock(mu);
     std::vector<std::shared_ptr<Request>> batch;
-    {
-      std::lock_guard<std::mutex> lock(mu);
-      if (pending.empty()) return false;
-      const auto head = pending.front();
-      for (auto it = pending.begin(); it != pending.end() && batch.size() < opt.max_batch;) {
-        const auto &r = *it;
-        if (r->params == head->params && r->op == head->op && r->level == head->level && r->device == head->device) {
-          batch.push_back(r);
-          it = pending.erase(it);
-        } else {
-          ++it;
-        }
+    if (pending.empty()) return batch;
+    const auto head = pending.front();
+    for (auto it = pending.begin(); it != pending.end() && batch.size() < opt.max_batch;) {
+      const Request &r = **it;
+      const bool same = r.op == head->op && r.device == head->device &&
+                        (is_transform(head->op) ? r.n == head->n : r.params == head->params && r.level == head->level);
+      if (same) {
+        batch.push_back(*it);
+        it = pending.erase(it);
+      } else {
+        ++it;
       }
     }
-    const auto &h = *batch.front();
-    const auto pset = param_set(h.params);
-    const ParamSet &p = *pset;
-    LevelPlan *lp = nullptr;
+    return batch;
+  }
+
+  // Launch one batch on an idle slot. Returns false if nothing could be admitted.
+  bool launch(Slot &slot) {
+    std::vector<std::shared_ptr<Request>> batch = take_group();
+    if (batch.empty()) return false;
+    const Request &h = *batch.front();
+    const auto hset = param_set(h.params);
+    const bool pooled = is_transform(h.op);
+    std::shared_ptr<LevelPlan> lp;
+    std::shared_ptr<Pool> pl;
     try {
-      lp = &plan(h.params, h.level);
+      if (pooled) pl = pool(h.n, slot);
+      else lp = plan(h.params, h.level, slot);
     } catch (const std::exception &e) {
       fail(batch, e.what());
       return true;
     }
-    // admission by device footprint: the whole budget minus the other arenas
-    const size_t others = arena_bytes - slot.device_bytes;
-    size_t b = batch.size();
-    while (b > 1 && others + footprint(p, *lp, h.op, h.level, b) > opt.hbm_budget_bytes) --b;
-    if (others + footprint(p, *lp, h.op, h.level, b) > opt.hbm_budget_bytes) {
-      // idle arenas of other slots hold budget without work: release them
-      for (Slot &o : slots)
-        if (&o != &slot && !o.busy && o.device) {
-          ok(ntt_hip_free(opt.device, o.device), "free");
-          arena_bytes -= o.device_bytes;
-          o.device = nullptr;
-          o.device_bytes = 0;
-        }
+    if (!pl && !lp) {  // no room for the plan itself
+      return wait_or_fail(batch, "the plan does not fit the device budget");
     }
-    if (arena_bytes - slot.device_bytes + footprint(p, *lp, h.op, h.level, b) > opt.hbm_budget_bytes) {
-      const bool others_busy = std::any_of(slots.begin(), slots.end(), [&](const Slot &o) { return o.busy; });
-      if (!others_busy) {
-        fail(batch, "request does not fit the device budget");
-        return true;
+    if (pooled) {
+      // a busy pool built before newer parameter sets may lack their primes:
+      // those requests wait for its rebuild
+      std::vector<std::shared_ptr<Request>> covered, later;
+      for (const auto &r : batch) {
+        const auto ps = param_set(r->params);
+        bool all = true;
+        for (size_t t = 0; t < r->level && all; ++t) all = pl->tower_of.count(ps->q[t]) != 0;
+        (all ? covered : later).push_back(r);
       }
-      requeue(batch);  // wait for in-flight batches to free memory
-      return false;
+      if (!later.empty()) requeue(later);
+      if (covered.empty()) return false;
+      batch = std::move(covered);
     }
+    Resident *res = pooled ? static_cast<Resident *>(pl.get()) : lp.get();
+    // per-request input/output extents (words)
+    std::vector<std::shared_ptr<const ParamSet>> sets;
+    for (const auto &r : batch) sets.push_back(param_set(r->params));
+    auto in_w = [&](size_t i) { return in_words(*sets[i], batch[i]->op, batch[i]->level); };
+    auto out_w = [&](size_t i) { return out_words(*sets[i], batch[i]->op, batch[i]->level); };
+    auto footprint = [&](size_t b) {
+      size_t words = 0;
+      for (size_t i = 0; i < b; ++i) words += in_w(i) + out_w(i) * (h.op == Op::KeySwitch ? 2 : 1);
+      size_t bytes = words * 8;
+      if (pooled) {
+        size_t towers = 0;
+        for (size_t i = 0; i < b; ++i) towers += batch[i]->level;
+        bytes += (towers * sizeof(ntt_hip_tower_job) + 255) / 256 * 256;
+      } else {
+        bytes += workspace_bytes(*lp, h.op, b);
+      }
+      return bytes;
+    };
+    // admission by the whole footprint: shrink the batch, then make room
+    size_t b = batch.size();
+    while (b > 1 && !fits(footprint(b), &slot)) --b;
+    if (!make_room(footprint(b), &slot, res)) return wait_or_fail(batch, "request does not fit the device budget");
     if (b < batch.size()) {
       requeue(std::vector<std::shared_ptr<Request>>(batch.begin() + b, batch.end()));
       batch.resize(b);
+      sets.resize(b);
     }
     try {
-      const size_t iw = in_words(p, h.op, h.level), ow = out_words(p, h.op, h.level);
-      const size_t ow2 = h.op == Op::KeySwitch ? ow : 0;
-      const size_t dev = footprint(p, *lp, h.op, h.level, b);
-      ensure_arena(slot, dev, h.device ? 0 : b * (iw + ow + ow2) * 8);
-      uint64_t *d_in = static_cast<uint64_t *>(slot.device), *d_out = d_in + b * iw, *d_out2 = d_out + b * ow;
-      void *d_ws = d_out2 + b * ow2;
+      size_t iw = 0, ow = 0, ow2 = 0;
+      std::vector<size_t> in_at(b), out_at(b), out2_at(b), out_len(b);
+      for (size_t i = 0; i < b; ++i) {
+        in_at[i] = iw;
+        out_at[i] = ow;
+        out_len[i] = out_w(i);
+        iw += in_w(i);
+        ow += out_len[i];
+      }
+      if (h.op == Op::KeySwitch) ow2 = ow;
+      for (size_t i = 0; i < b; ++i) out2_at[i] = ow + out_at[i];
+      const size_t dev = footprint(b);
+      ensure_arena(slot, dev, h.device ? 0 : (iw + ow + ow2) * 8);
+      uint64_t *d_in = static_cast<uint64_t *>(slot.device), *d_out = d_in + iw, *d_out2 = d_out + ow;
+      void *d_ws = d_out2 + ow2;
       uint64_t *h_in = static_cast<uint64_t *>(slot.pinned);
+      const bool direct = h.device && b == 1 && !pooled;  // run on the caller's buffers
       if (!h.device) {
-        for (size_t i = 0; i < b; ++i) std::memcpy(h_in + i * iw, batch[i]->in.data(), iw * 8);
-        ok(ntt_hip_memcpy(opt.device, d_in, h_in, b * iw * 8, NTT_HIP_H2D, slot.stream), "h2d");
-      } else if (b == 1) {  // run on the caller's buffers
+        for (size_t i = 0; i < b; ++i) std::memcpy(h_in + in_at[i], batch[i]->in.data(), in_w(i) * 8);
+        ok(ntt_hip_memcpy(opt.device, d_in, h_in, iw * 8, NTT_HIP_H2D, slot.stream), "h2d");
+      } else if (direct) {
         d_in = const_cast<uint64_t *>(h.d_in);
         d_out = h.d_out;
         d_out2 = h.d_out2;
       } else {
         for (size_t i = 0; i < b; ++i)
-          ok(ntt_hip_memcpy(opt.device, d_in + i * iw, batch[i]->d_in, iw * 8, NTT_HIP_D2D, slot.stream), "gather");
+          ok(ntt_hip_memcpy(opt.device, d_in + in_at[i], batch[i]->d_in, in_w(i) * 8, NTT_HIP_D2D, slot.stream),
+             "gather");
       }
-      const size_t L = h.level, n = p.n;
-      switch (h.op) {
-        case Op::KeySwitch:
-          ok(ntt_hip_keyswitch_run(lp->ks, d_in, b, d_out, d_out2, d_ws, slot.stream), "keyswitch");
-          break;
-        case Op::Forward:
-        case Op::Inverse:
-          ok(ntt_hip_transform_batch(lp->ctx, d_out, d_in, b, L * n, L * n, 0, L, h.op == Op::Inverse, slot.stream),
-             "transform");
-          break;
-        case Op::ModUp:
-          ok(ntt_hip_modup_run(lp->ks, d_in, 1, b, d_out, d_ws, slot.stream), "modup");
-          break;
-        case Op::ModDown:
-          ok(ntt_hip_moddown_run(lp->ks, d_in, b, d_out, d_ws, slot.stream), "moddown");
-          break;
+      if (pooled) {
+        // one job per tower of every request, each with its prime's pool tower
+        std::vector<ntt_hip_tower_job> jobs;
+        for (size_t i = 0; i < b; ++i)
+          for (size_t t = 0; t < batch[i]->level; ++t) {
+            const auto f = pl->tower_of.find(sets[i]->q[t]);
+            if (f == pl->tower_of.end()) throw std::runtime_error("prime missing from the ring's pool");
+            const uint64_t at = in_at[i] + t * h.n;
+            jobs.push_back({at, at, f->second, 0});
+          }
+        ok(ntt_hip_transform_jobs(pl->ctx, d_out, d_in, jobs.data(), jobs.size(), d_ws, h.op == Op::Inverse,
+                                  slot.stream),
+           "transform");
+      } else {
+        const size_t L = h.level;
+        switch (h.op) {
+          case Op::KeySwitch:
+            ok(ntt_hip_keyswitch_run(lp->ks, d_in, b, d_out, d_out2, d_ws, slot.stream), "keyswitch");
+            break;
+          case Op::ModUp:
+            ok(ntt_hip_modup_run(lp->ks, d_in, 1, b, d_out, d_ws, slot.stream), "modup");
+            break;
+          case Op::ModDown:
+            ok(ntt_hip_moddown_run(lp->ks, d_in, b, d_out, d_ws, slot.stream), "moddown");
+            break;
+          default:
+            (void)L;
+            throw std::logic_error("transform outside the pool path");
+        }
       }
       if (!h.device) {
-        uint64_t *h_out = h_in + b * iw;
-        ok(ntt_hip_memcpy(opt.device, h_out, d_out, b * (ow + ow2) * 8, NTT_HIP_D2H, slot.stream), "d2h");
-      } else if (b > 1) {
+        uint64_t *h_out = h_in + iw;
+        ok(ntt_hip_memcpy(opt.device, h_out, d_out, (ow + ow2) * 8, NTT_HIP_D2H, slot.stream), "d2h");
+      } else if (!direct) {
         for (size_t i = 0; i < b; ++i) {
-          ok(ntt_hip_memcpy(opt.device, batch[i]->d_out, d_out + i * ow, ow * 8, NTT_HIP_D2D, slot.stream), "scatter");
+          ok(ntt_hip_memcpy(opt.device, batch[i]->d_out, d_out + out_at[i], out_len[i] * 8, NTT_HIP_D2D, slot.stream),
+             "scatter");
           if (ow2)
-            ok(ntt_hip_memcpy(opt.device, batch[i]->d_out2, d_out2 + i * ow2, ow2 * 8, NTT_HIP_D2D, slot.stream),
+            ok(ntt_hip_memcpy(opt.device, batch[i]->d_out2, d_out2 + out_at[i], out_len[i] * 8, NTT_HIP_D2D,
+                              slot.stream),
                "scatter");
         }
       }
       ok(ntt_hip_event_record(opt.device, slot.event, slot.stream), "record");
       slot.batch = std::move(batch);
-      slot.in_words = iw;
-      slot.out_words = ow;
-      slot.out2_words = ow2;
+      slot.out_at = out_at;
+      slot.out_words = out_len;
+      slot.out2_at.assign(ow2 ? out2_at.begin() : out2_at.end(), out2_at.end());
+      slot.holder = pooled ? std::static_pointer_cast<Resident>(pl) : std::static_pointer_cast<Resident>(lp);
+      ++slot.holder->inflight;
+      slot.holder->last_use = ++tick;
       slot.busy = true;
+      publish_memory();
       std::lock_guard<std::mutex> lock(mu);
       ++stats.batches;
-      stats.device_bytes_in_flight = arena_bytes;
-      stats.peak_device_bytes = std::max(stats.peak_device_bytes, arena_bytes);
       size_t busy = 0;
       for (const Slot &s : slots) busy += s.busy;
       stats.max_in_flight = std::max(stats.max_in_flight, busy);
@@ -271,30 +467,45 @@ struct Server::Impl {
     return true;
   }
 
+  // A batch that cannot be admitted waits while others run, else fails.
+  bool wait_or_fail(const std::vector<std::shared_ptr<Request>> &batch, const char *why) {
+    const bool others_busy = std::any_of(slots.begin(), slots.end(), [](const Slot &o) { return o.busy; });
+    if (!others_busy) {
+      fail(batch, why);
+      return true;
+    }
+    requeue(batch);
+    return false;
+  }
+
   bool retire(Slot &slot) {
     if (!slot.busy) return false;
     const int q = ntt_hip_event_query(opt.device, slot.event);
     if (q == NTT_HIP_NOT_READY) return false;
     const auto now = Clock::now();
     const size_t b = slot.batch.size();
-    const uint64_t *h_out = static_cast<const uint64_t *>(slot.pinned) + b * slot.in_words;
-    std::lock_guard<std::mutex> lock(mu);
-    for (size_t i = 0; i < b; ++i) {
-      Request &r = *slot.batch[i];
-      if (q != NTT_HIP_OK) {
-        r.error = std::string("HIP: batch failed: ") + ntt_hip_last_error();
-      } else if (!r.device) {
-        r.result.out.assign(h_out + i * slot.out_words, h_out + (i + 1) * slot.out_words);
-        if (slot.out2_words) {
-          const uint64_t *o2 = h_out + b * slot.out_words;
-          r.result.out2.assign(o2 + i * slot.out2_words, o2 + (i + 1) * slot.out2_words);
+    size_t iw = 0;
+    for (const auto &r : slot.batch) iw += r->device ? 0 : r->in.size();
+    const uint64_t *h_out = static_cast<const uint64_t *>(slot.pinned) + iw;
+    {
+      std::lock_guard<std::mutex> lock(mu);
+      for (size_t i = 0; i < b; ++i) {
+        Request &r = *slot.batch[i];
+        if (q != NTT_HIP_OK) {
+          r.error = std::string("HIP: batch failed: ") + ntt_hip_last_error();
+        } else if (!r.device) {
+          r.result.out.assign(h_out + slot.out_at[i], h_out + slot.out_at[i] + slot.out_words[i]);
+          if (!slot.out2_at.empty())
+            r.result.out2.assign(h_out + slot.out2_at[i], h_out + slot.out2_at[i] + slot.out_words[i]);
         }
+        r.result.latency_ms = std::chrono::duration<double, std::milli>(now - r.t0).count();
+        r.result.batch_size = b;
+        r.done = true;
+        ++stats.completed;
       }
-      r.result.latency_ms = std::chrono::duration<double, std::milli>(now - r.t0).count();
-      r.result.batch_size = b;
-      r.done = true;
-      ++stats.completed;
     }
+    --slot.holder->inflight;
+    slot.holder.reset();
     slot.batch.clear();
     slot.busy = false;
     cv_done.notify_all();
@@ -359,6 +570,7 @@ Server::~Server() {
     if (s.stream) ntt_hip_stream_destroy(impl_->opt.device, s.stream);
   }
   impl_->plans.clear();
+  impl_->pools.clear();
 }
 
 uint32_t Server::add_params(const ParamSet &p) {
@@ -399,6 +611,7 @@ uint64_t Server::submit(uint32_t id, Op op, size_t level, std::span<const uint64
     r->params = id;
     r->op = op;
     r->level = level;
+    r->n = p.n;
     r->in.assign(input.begin(), input.end());
     r->t0 = Clock::now();
     impl_->pending.push_back(r);
@@ -423,6 +636,7 @@ uint64_t Server::submit_device(uint32_t id, Op op, size_t level, const uint64_t 
     r->params = id;
     r->op = op;
     r->level = level;
+    r->n = p.n;
     r->device = true;
     r->d_in = in;
     r->d_out = out;
@@ -572,6 +786,17 @@ int ntt_serve_wait(ntt_serve *s, uint64_t ticket, uint64_t *out, size_t out_word
     if (out2) std::copy(r.out2.begin(), r.out2.end(), out2);
     if (latency_ms) *latency_ms = r.latency_ms;
     if (batch_size) *batch_size = r.batch_size;
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_serve_stats_ex(ntt_serve *s, size_t *out, size_t count) {
+  return serve_guard([&] {
+    if (!s || !out) throw std::invalid_argument("null argument");
+    const auto st = s->server.stats();
+    const size_t v[9] = {st.submitted,        st.completed,      st.batches,         st.max_in_flight, st.device_bytes_in_flight,
+                         st.peak_device_bytes, st.resident_bytes, st.plans_resident, st.plans_evicted};
+    std::copy(v, v + std::min<size_t>(count, 9), out);
     return NTT_HIP_OK;
   });
 }

@@ -77,6 +77,17 @@ inline int arith_for(uint64_t max_q) {
   return kExact;
 }
 
+// Tuned launch shape, carried in the same int above the arithmetic bits:
+//   kRow16Bit: the N = 2^16 row pass keeps 16 residues per thread (ntt_pass_b)
+//              instead of 8 (ntt_pass_b8_*); the forward then uses headroom 0;
+//   bits kWgCapShift..+3: at most that many workgroups per CU for the
+//              N = 2^16 passes (0 = no cap), enforced with extra dynamic LDS.
+constexpr int kRow16Bit = 1 << 8, kWgCapShift = 12;
+inline int ntt_mode(int arith, bool row16, uint32_t wg_cap) {
+  if (row16) arith = (arith & kArithMask) | kRow16Bit;  // the 16-residue row pass runs headroom 0
+  return arith | (int)((wg_cap & 15u) << kWgCapShift);
+}
+
 hipError_t launch_ntt(const TowerMap &map, const PrimeConst *primes, uint32_t log_n, uint32_t towers, bool inverse,
                       int arith, hipStream_t stream, uint32_t chunk = 0);
 

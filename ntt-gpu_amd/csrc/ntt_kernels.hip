@@ -697,9 +697,18 @@ __global__ __launch_bounds__(256) void ntt_lds_generic(TowerMap map, const Prime
 }
 
 // ---------------------------------------------------------------- launcher
+// Extra dynamic LDS per workgroup that leaves room for at most `cap`
+// workgroups per CU (160 KiB of LDS; 0 = no cap).
+static uint32_t lds_pad_for(uint32_t cap) {
+  if (cap == 0) return 0;
+  constexpr uint32_t kLdsPerCu = 160 * 1024, kStatic = 17536;  // largest static LDS of the 2^16 passes
+  const uint32_t per_wg = kLdsPerCu / (cap + 1) + 1;
+  return per_wg > kStatic ? per_wg - kStatic : 0;
+}
+
 template <bool Inverse, int A, int H = 0>
 static hipError_t launch_t(const TowerMap &map, const PrimeConst *primes, uint32_t log_n, uint32_t towers,
-                           hipStream_t stream) {
+                           hipStream_t stream, bool row16 = false, uint32_t pad = 0) {
   const uint32_t n = 1u << log_n;
   if (log_n <= 12) {
     hipLaunchKernelGGL((ntt_lds_generic<Inverse, A>), dim3(1, towers), dim3(256), 0, stream, map, primes, log_n,
@@ -709,36 +718,39 @@ static hipError_t launch_t(const TowerMap &map, const PrimeConst *primes, uint32
   const uint32_t R = n / kRow, log_r = log_n - 8;
   auto pass_a = [&]() {
     if (R == 256) {
-      hipLaunchKernelGGL((ntt_pass_a256<Inverse, A, Inverse ? 0 : H>), dim3(kRow / 16, towers), dim3(256), 0, stream,
-                         map, primes);
+      hipLaunchKernelGGL((ntt_pass_a256<Inverse, A, Inverse ? 0 : H>), dim3(kRow / 16, towers), dim3(256), pad,
+                         stream, map, primes);
     } else {
       const uint32_t cols = 4096 / R;
       hipLaunchKernelGGL((ntt_lds_generic<Inverse, A>), dim3(kRow / cols, towers), dim3(256), 0, stream, map,
                          primes, log_r, (uint32_t)kRow, cols, (uint32_t)Inverse, (uint32_t)!Inverse);
     }
   };
+  const uint32_t pb = R == 256 ? pad : 0;
   auto pass_b = [&]() {
     if constexpr (Inverse && (A == kLazy || A == kNarrow)) {
-      if (R == 256) {
-        hipLaunchKernelGGL((ntt_pass_b8_inv<A>), dim3(R / 8, towers), dim3(256), 0, stream, map, primes, R);
+      if (R == 256 && !row16) {
+        hipLaunchKernelGGL((ntt_pass_b8_inv<A>), dim3(R / 8, towers), dim3(256), pb, stream, map, primes, R);
         return;
       }
     }
     if constexpr (!Inverse && (A == kLazy || A == kNarrow)) {
-      if (R == 256) {
+      if (R == 256 && !row16) {
         if (map.epi.sub)
-          hipLaunchKernelGGL((ntt_pass_b8_fwd<A, true, H>), dim3(R / 8, towers), dim3(256), 0, stream, map, primes,
+          hipLaunchKernelGGL((ntt_pass_b8_fwd<A, true, H>), dim3(R / 8, towers), dim3(256), pb, stream, map, primes,
                              R);
         else
-          hipLaunchKernelGGL((ntt_pass_b8_fwd<A, false, H>), dim3(R / 8, towers), dim3(256), 0, stream, map, primes,
+          hipLaunchKernelGGL((ntt_pass_b8_fwd<A, false, H>), dim3(R / 8, towers), dim3(256), pb, stream, map, primes,
                              R);
         return;
       }
     }
+    // the 16-residue row pass (R < 256, canonical arithmetic, or chosen by
+    // the tuner) keeps one reduction per stage: its forward pairs with H = 0
     if (!Inverse && map.epi.sub)
-      hipLaunchKernelGGL((ntt_pass_b<false, A, true>), dim3(R / 16, towers), dim3(256), 0, stream, map, primes, R);
+      hipLaunchKernelGGL((ntt_pass_b<false, A, true>), dim3(R / 16, towers), dim3(256), pb, stream, map, primes, R);
     else
-      hipLaunchKernelGGL((ntt_pass_b<Inverse, A, false>), dim3(R / 16, towers), dim3(256), 0, stream, map, primes,
+      hipLaunchKernelGGL((ntt_pass_b<Inverse, A, false>), dim3(R / 16, towers), dim3(256), pb, stream, map, primes,
                          R);
   };
   if (!Inverse) {
@@ -755,25 +767,28 @@ static hipError_t launch_t(const TowerMap &map, const PrimeConst *primes, uint32
 // for kLazy, the forward headroom class).
 static hipError_t dispatch(const TowerMap &m, const PrimeConst *primes, uint32_t log_n, uint32_t count, bool inverse,
                            int arith, hipStream_t stream) {
-  const int mode = arith & kArithMask, h = arith >> kArithShift;
+  const int mode = arith & kArithMask, h = (arith >> kArithShift) & 7;
+  const bool row16 = (arith & kRow16Bit) != 0;
+  const uint32_t pad = lds_pad_for((uint32_t)(arith >> kWgCapShift) & 15u);
+  if (row16 && h != 0) return hipErrorInvalidValue;
   if (mode == kLazy) {
-    if (inverse) return launch_t<true, kLazy>(m, primes, log_n, count, stream);
+    if (inverse) return launch_t<true, kLazy>(m, primes, log_n, count, stream, row16, pad);
     switch (h) {
-      case 0: return launch_t<false, kLazy, 0>(m, primes, log_n, count, stream);
-      case 1: return launch_t<false, kLazy, 1>(m, primes, log_n, count, stream);
-      case 2: return launch_t<false, kLazy, 2>(m, primes, log_n, count, stream);
-      case 3: return launch_t<false, kLazy, 3>(m, primes, log_n, count, stream);
-      case 4: return launch_t<false, kLazy, 4>(m, primes, log_n, count, stream);
-      case 5: return launch_t<false, kLazy, 5>(m, primes, log_n, count, stream);
+      case 0: return launch_t<false, kLazy, 0>(m, primes, log_n, count, stream, row16, pad);
+      case 1: return launch_t<false, kLazy, 1>(m, primes, log_n, count, stream, row16, pad);
+      case 2: return launch_t<false, kLazy, 2>(m, primes, log_n, count, stream, row16, pad);
+      case 3: return launch_t<false, kLazy, 3>(m, primes, log_n, count, stream, row16, pad);
+      case 4: return launch_t<false, kLazy, 4>(m, primes, log_n, count, stream, row16, pad);
+      case 5: return launch_t<false, kLazy, 5>(m, primes, log_n, count, stream, row16, pad);
       default: return hipErrorInvalidValue;
     }
   }
   if (h != 0) return hipErrorInvalidValue;
   switch (mode * 2 + (inverse ? 1 : 0)) {
-    case kNarrow * 2: return launch_t<false, kNarrow>(m, primes, log_n, count, stream);
-    case kNarrow * 2 + 1: return launch_t<true, kNarrow>(m, primes, log_n, count, stream);
-    case kExact * 2: return launch_t<false, kExact>(m, primes, log_n, count, stream);
-    case kExact * 2 + 1: return launch_t<true, kExact>(m, primes, log_n, count, stream);
+    case kNarrow * 2: return launch_t<false, kNarrow>(m, primes, log_n, count, stream, row16, pad);
+    case kNarrow * 2 + 1: return launch_t<true, kNarrow>(m, primes, log_n, count, stream, row16, pad);
+    case kExact * 2: return launch_t<false, kExact>(m, primes, log_n, count, stream, row16, pad);
+    case kExact * 2 + 1: return launch_t<true, kExact>(m, primes, log_n, count, stream, row16, pad);
     default: return hipErrorInvalidValue;
   }
 }

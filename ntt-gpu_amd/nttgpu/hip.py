@@ -69,6 +69,11 @@ SIGNATURES = {
     "ntt_hip_keyswitch_workspace_bytes": [VP, SZ, SZP],
     "ntt_hip_keyswitch_run": [VP, VP, SZ, VP, VP, VP, VP],
     "ntt_hip_keyswitch_run_static": [VP, VP, SZ, VP, VP, VP],
+    "ntt_hip_set_ntt_shape": [VP, INT, ctypes.c_uint32, ctypes.c_uint32],
+    "ntt_hip_context_device_bytes": [VP, ctypes.POINTER(SZ)],
+    "ntt_hip_keyswitch_device_bytes": [VP, ctypes.POINTER(SZ)],
+    "ntt_hip_transform_jobs": [VP, VP, VP, VP, SZ, VP, INT, VP],
+    "ntt_hip_get_ntt_shape": [VP, ctypes.POINTER(INT), ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)],
     "ntt_hip_keyswitch_capture": [VP, VP, SZ, VP, VP, VP, VP, ctypes.POINTER(VP)],
     "ntt_hip_graph_launch": [VP, VP],
     "ntt_hip_graph_destroy": [VP],
@@ -439,6 +444,17 @@ def autotune(ctx, towers):
 
 def set_tuning(ctx, ntt_chunk, conv_targets):
     check(lib().ntt_hip_set_tuning(ctx.handle, ntt_chunk, conv_targets), "set_tuning")
+
+
+def set_ntt_shape(ctx, fwd_headroom=-1, row_residues=8, wg_per_cu=0):
+    check(lib().ntt_hip_set_ntt_shape(ctx.handle, fwd_headroom, row_residues, wg_per_cu), "set_ntt_shape")
+
+
+def get_ntt_shape(ctx):
+    h, r, w = INT(0), ctypes.c_uint32(0), ctypes.c_uint32(0)
+    check(lib().ntt_hip_get_ntt_shape(ctx.handle, ctypes.byref(h), ctypes.byref(r), ctypes.byref(w)),
+          "get_ntt_shape")
+    return {"fwd_headroom": h.value, "row_residues": r.value, "wg_per_cu": w.value}
 
 
 def get_tuning(ctx):

@@ -25,6 +25,7 @@ SIGNATURES = {
     "ntt_serve_wait": [VP, ctypes.c_uint64, U64P, SZ, U64P, SZ, ctypes.POINTER(ctypes.c_double),
                        ctypes.POINTER(SZ)],
     "ntt_serve_stats": [VP, ctypes.POINTER(SZ)],
+    "ntt_serve_stats_ex": [VP, ctypes.POINTER(SZ), SZ],
 }
 _lib = None
 
@@ -119,9 +120,10 @@ class Server:
         return (out, out2 if op == KEYSWITCH else None), lat.value, bs.value
 
     def stats(self):
-        v = (SZ * 6)()
-        _check(lib().ntt_serve_stats(self.handle, v), "serve_stats")
-        keys = ("submitted", "completed", "batches", "max_in_flight", "device_bytes", "peak_device_bytes")
+        v = (SZ * 9)()
+        _check(lib().ntt_serve_stats_ex(self.handle, v, 9), "serve_stats")
+        keys = ("submitted", "completed", "batches", "max_in_flight", "device_bytes", "peak_device_bytes",
+                "resident_bytes", "plans_resident", "plans_evicted")
         return dict(zip(keys, list(v)))
 
     def close(self):

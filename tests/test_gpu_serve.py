@@ -168,3 +168,65 @@ def test_served_stream_at_the_stated_ranges():
     assert all(v >= 8 for v in checked.values()), checked
     srv.close()
     alone_srv.close()
+
+
+def test_transforms_of_different_levels_share_launches():
+    """NTT and INTT requests of one ring dimension but different levels and
+    parameter sets run in shared launches (one job table over the ring's
+    prime pool); every response equals the same request alone and the oracle."""
+    reqs, pools = workload.stream(40, rate_per_s=1e6, seed=21, sizes=(8192,), levels=(3, 20))
+    reqs = [r for r in reqs if r.op in (serve.FORWARD, serve.INVERSE)]
+    assert len({r.level for r in reqs}) > 3
+    srv = serve.Server(streams=2, max_batch=16, hbm_budget_bytes=8 << 30)
+    ms = workload.MixedServer(srv, pools)
+    for r in reqs:
+        ms.params_for(r)
+    log_case(f"pooled transforms: {len(reqs)} requests over {len({r.level for r in reqs})} levels")
+    tickets = [ms.submit(r) for r in reqs]
+    got = [ms.wait(t, r) for t, r in zip(tickets, reqs)]
+    st = srv.stats()
+    assert st["batches"] <= len(reqs) // 2, st
+    mixed_levels = False
+    alone = workload.MixedServer(serve.Server(streams=1, max_batch=1, hbm_budget_bytes=8 << 30), pools)
+    for i, r in enumerate(reqs):
+        (m, _), _, bs = got[i]
+        (a, _), _, _ = alone.wait(alone.submit(r), r)
+        assert np.array_equal(a, m), f"request {i}: pooled result differs from alone"
+        hp, _, _ = ms.sets[(r.n, r.level, r.dnum)]
+        assert np.array_equal(m, o.ntt_towers(r.data, r.n, hp.q, hp.q_psi, inverse=r.op == serve.INVERSE))
+        mixed_levels |= bs > 1
+    assert mixed_levels
+    srv.close()
+    alone.server.close()
+
+
+def test_resident_plans_count_and_evict_under_the_budget():
+    """Resident contexts and keys count against the budget: key switches
+    cycling over more parameter sets than fit evict idle plans (least recently
+    used first) instead of failing, the peak stays within the budget, and the
+    results equal the oracle."""
+    pools = {8192: workload.PrimePool(8192)}
+    levels = (10, 12, 14, 16, 18)
+    sets = [pools[8192].param_set(L, 2) for L in levels]
+    rng = np.random.default_rng(33)
+    srv = serve.Server(streams=2, max_batch=2, hbm_budget_bytes=40 << 20)
+    ids, keys = [], []
+    for hp in sets:
+        kb, ka = workload.make_key(hp, 1000 + len(hp.q))
+        ids.append(srv.add_params(hp, kb, ka))
+        keys.append((kb, ka))
+    log_case("serve eviction under a 40 MiB budget")
+    reqs = []
+    for k in range(15):
+        j = k % len(sets)
+        x = workload.seeded_residues(8192, sets[j].q, rng)
+        reqs.append((j, x, srv.submit(ids[j], serve.KEYSWITCH, len(sets[j].q), x)))
+    for j, x, t in reqs:
+        (b, a), _, _ = srv.wait(t, ids[j], serve.KEYSWITCH, len(sets[j].q))
+        hp = sets[j]
+        wb, wa = o.key_switch(x, hp.n, hp.q, hp.q_psi, hp.M, hp.p, hp.p_psi, hp.alpha, *keys[j])
+        assert np.array_equal(b, wb) and np.array_equal(a, wa)
+    st = srv.stats()
+    assert st["completed"] == 15 and st["plans_evicted"] > 0, st
+    assert st["peak_device_bytes"] <= 40 << 20 and st["resident_bytes"] > 0, st
+    srv.close()

@@ -586,9 +586,6 @@ def main():
     hp = params.openfhe_like(N, L, K, ALPHA)
     B = args.batch
     ctx = hip.Context(N, hp.q + hp.p, hp.q_psi + hp.p_psi)
-    ks_tuning = tuned(ctx, 512) if not args.no_autotune else None
-    if ks_tuning:
-        progress(f"autotune: {ks_tuning['chosen']}")
     ks = hip.KeySwitch(ctx, L, K, ALPHA, max_batch=B)
     full = hp.q + hp.p
     beta = hp.beta()
@@ -597,6 +594,13 @@ def main():
     progress("key generated")
     ks.upload_key(kb, ka, hp.M, hp.q)
     progress("key resident")
+    ks_tuning = None
+    if not args.no_autotune:  # on the batched key switch itself, before the graph is captured
+        rep = ks.autotune(B)
+        ks_tuning = {"batch": rep["batch"], "chosen": rep["chosen"],
+                     "default_ms": round(rep["keyswitch"]["default_ms"], 3),
+                     "tuned_ms": round(rep["keyswitch"]["tuned_ms"], 3)}
+        progress(f"autotune: {ks_tuning}")
     x = np.concatenate([seeded(N, hp.q, 5000 + c) for c in range(B)])
     comp = L * N
     d_in = hip.DeviceBuffer(B * comp).upload(x)

@@ -106,6 +106,11 @@ int ntt_hip_get_tuning(ntt_hip_context *ctx, uint32_t *ntt_chunk, uint32_t *conv
 /* fwd_headroom: -1 = from the moduli (default) or at most 0..5; row_residues:
  * 8 (default) or 16; wg_per_cu: 0 (no cap, default) or 1..15. */
 int ntt_hip_set_ntt_shape(ntt_hip_context *ctx, int fwd_headroom, uint32_t row_residues, uint32_t wg_per_cu);
+/* The same search timed on `batch` fused key switches of the plan (its key
+ * uploaded), over the context's NTT shape and conversion width and the
+ * plan's concurrent lanes and MAC/transform interleave group; allocates its
+ * own buffers for the duration of the call. */
+int ntt_hip_keyswitch_autotune(ntt_hip_keyswitch *ks, size_t batch, char *report, size_t capacity);
 int ntt_hip_get_ntt_shape(ntt_hip_context *ctx, int *fwd_headroom, uint32_t *row_residues, uint32_t *wg_per_cu);
 
 /* ---- Driver seam operations (include/ntt/resident_driver.hpp:13-57) on

@@ -1636,6 +1636,73 @@ int ntt_hip_get_ntt_shape(ntt_hip_context *ctx, int *fwd_headroom, uint32_t *row
   });
 }
 
+// The key switch's own tuner: the same coordinate search with alternating
+// pairs, timed on the batched key switch itself (zero inputs: the arithmetic
+// is data-independent), over the context's NTT shape and chunk, the plan's
+// lanes and MAC/transform interleave group, and the conversion width.
+int ntt_hip_keyswitch_autotune(ntt_hip_keyswitch *ks, size_t batch, char *report, size_t capacity) {
+  return guarded([&] {
+    require(ks && ks->key_loaded, "autotune needs a plan with its key uploaded");
+    require(batch >= 1 && batch <= ks->max_batch, "batch exceeds the plan capacity");
+    ntt_hip_context &c = *ks->ctx;
+    DeviceGuard g(c.device);
+    const size_t comp = (size_t)ks->L * c.n;
+    DeviceBlob io(c.device, 3 * batch * comp * sizeof(uint64_t));
+    DeviceBlob ws(c.device, batch * ks->per_component() * sizeof(uint64_t));
+    uint64_t *in = (uint64_t *)io.ptr, *ob = in + batch * comp, *oa = ob + batch * comp;
+    HIP_CHECK(hipMemsetAsync(in, 0, batch * comp * sizeof(uint64_t), c.stream));
+    auto run_ms = [&] {
+      return time_ms(c, [&] {
+        const int st = ntt_hip_keyswitch_run(ks, in, batch, ob, oa, ws.ptr, c.stream);
+        if (st != NTT_HIP_OK) throw RuntimeError(std::string("tune key switch: ") + g_last_error);
+      }, 2);
+    };
+    for (int w = 0; w < 3; ++w) (void)run_ms();
+    std::string rep = "{\"keyswitch\":{\"default_ms\":" + std::to_string(run_ms());
+    auto sweep = [&](const char *name, size_t count, const std::function<void(size_t)> &set, size_t incumbent) {
+      rep += ",\"" + std::string(name) + "\":{";
+      size_t pick = incumbent;
+      for (size_t i = 0; i < count; ++i) {
+        if (i == pick) continue;
+        float t_inc = 0, t_cand = 0;
+        for (int r = 0; r < 3; ++r) {
+          set(pick);
+          t_inc += run_ms();
+          set(i);
+          t_cand += run_ms();
+        }
+        rep += (rep.back() == '{' ? "" : ",") + std::string("\"") + std::to_string(i) + "\":[" +
+               std::to_string(t_cand / 3) + "," + std::to_string(t_inc / 3) + "]";
+        if (t_cand < 0.98f * t_inc) pick = i;
+      }
+      set(pick);
+      rep += "}";
+    };
+    if (c.log_n == 16 && (c.arith & kArithMask) != kExact) {
+      const int heads[2] = {-1, 0};
+      sweep("fwd_headroom", ((c.arith >> kArithShift) & 7) > 0 ? 2 : 1, [&](size_t i) { c.fwd_headroom = heads[i]; },
+            0);
+      sweep("row_residues", 2, [&](size_t i) { c.row16 = i == 1; }, 0);
+      const uint32_t caps[4] = {0, 6, 5, 4};
+      sweep("wg_per_cu", 4, [&](size_t i) { c.wg_cap = caps[i]; }, 0);
+    }
+    const uint32_t lanes[4] = {ntt_hip_keyswitch::kDefaultLanes, 1, 3, 4};
+    sweep("lanes", 4, [&](size_t i) { ks->lanes = lanes[i]; }, 0);
+    const uint32_t groups[5] = {ntt_hip_keyswitch::kDefaultMacGroup, 0, 1, 4, 8};
+    sweep("mac_group", 5, [&](size_t i) { ks->mac_group = groups[i]; }, 0);
+    const uint32_t targets[6] = {0, 2, 4, 8, 12, 24};
+    sweep("conv_targets", 6, [&](size_t i) { c.conv_targets = targets[i]; }, 0);
+    HIP_CHECK(hipStreamSynchronize(c.stream));
+    rep += ",\"tuned_ms\":" + std::to_string(run_ms()) + "}";
+    rep += ",\"chosen\":{\"fwd_headroom\":" + std::to_string(c.fwd_headroom) +
+           ",\"row_residues\":" + std::to_string(c.row16 ? 16 : 8) + ",\"wg_per_cu\":" + std::to_string(c.wg_cap) +
+           ",\"lanes\":" + std::to_string(ks->lanes) + ",\"mac_group\":" + std::to_string(ks->mac_group) +
+           ",\"conv_targets\":" + std::to_string(c.conv_targets) + "},\"batch\":" + std::to_string(batch) + "}";
+    if (report && capacity) std::snprintf(report, capacity, "%s", rep.c_str());
+    return NTT_HIP_OK;
+  });
+}
+
 // Coordinate search: each knob in turn over its candidates (the others held
 // at their best so far), a candidate replacing the incumbent only when it is
 // clearly faster. NTT knobs are timed on `towers` forward + inverse transforms

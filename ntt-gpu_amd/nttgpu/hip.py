@@ -71,6 +71,7 @@ SIGNATURES = {
     "ntt_hip_keyswitch_run_static": [VP, VP, SZ, VP, VP, VP],
     "ntt_hip_set_ntt_shape": [VP, INT, ctypes.c_uint32, ctypes.c_uint32],
     "ntt_hip_context_device_bytes": [VP, ctypes.POINTER(SZ)],
+    "ntt_hip_keyswitch_autotune": [VP, SZ, ctypes.c_char_p, SZ],
     "ntt_hip_keyswitch_device_bytes": [VP, ctypes.POINTER(SZ)],
     "ntt_hip_transform_jobs": [VP, VP, VP, VP, SZ, VP, INT, VP],
     "ntt_hip_get_ntt_shape": [VP, ctypes.POINTER(INT), ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)],
@@ -355,6 +356,13 @@ class KeySwitch:
     def run(self, in_ptr, batch, out_b_ptr, out_a_ptr, workspace_ptr, stream=None):
         check(lib().ntt_hip_keyswitch_run(self.handle, in_ptr, batch, out_b_ptr, out_a_ptr, workspace_ptr, stream),
               "keyswitch_run")
+
+    def autotune(self, batch):
+        """Tune the plan and its context on `batch` key switches; returns the report."""
+        import json
+        buf = ctypes.create_string_buffer(8192)
+        check(lib().ntt_hip_keyswitch_autotune(self.handle, batch, buf, 8192), "keyswitch_autotune")
+        return json.loads(buf.value.decode())
 
     def run_static(self, in_ptr, batch, out_b_ptr, out_a_ptr, stream=None):
         """The key switch over the library's static device workspace (no

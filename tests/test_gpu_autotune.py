@@ -121,3 +121,52 @@ def test_tuned_choice_is_exact_and_not_slower():
     ctx.transform_batch(buf.ptr, 2, N16, 0, 1, False)
     ctx.synchronize()
     assert np.array_equal(buf.download(), o.ntt_towers(x, N16, [q] * 2, [r] * 2))
+
+
+def test_keyswitch_autotune_is_exact_and_not_slower():
+    """The key switch's own tuner (NTT shape, lanes, interleave group,
+    conversion width timed on the batched key switch): afterwards the key
+    switch is bit-exact against the oracle and not slower than the defaults."""
+    hp = params.openfhe_like(N16, 24, 6, 6)
+    ctx = hip.Context(N16, hp.q + hp.p, hp.q_psi + hp.p_psi)
+    B = 8
+    ks = hip.KeySwitch(ctx, 24, 6, 6, B)
+    full = hp.q + hp.p
+    kb = np.concatenate([uniform(N16, full, 70 + d) for d in range(hp.beta())])
+    ka = np.concatenate([uniform(N16, full, 80 + d) for d in range(hp.beta())])
+    ks.upload_key(kb, ka, hp.M, hp.q)
+    log_case("keyswitch autotune L=24 batch 8")
+    rep = ks.autotune(B)
+    assert {"lanes", "mac_group", "conv_targets"} <= set(rep["chosen"])
+    x = np.concatenate([uniform(N16, hp.q, 90 + c) for c in range(B)])
+    comp = 24 * N16
+    d_in = hip.DeviceBuffer(x.size).upload(x)
+    ob, oa = hip.DeviceBuffer(x.size), hip.DeviceBuffer(x.size)
+    ws = hip.DeviceBuffer(ks.workspace_words(B))
+    ks.run(d_in.ptr, B, ob.ptr, oa.ptr, ws.ptr)
+    ctx.synchronize()
+    gb, ga = ob.download(), oa.download()
+    for c in (0, B - 1):
+        wb, wa = o.key_switch(x[c * comp:(c + 1) * comp], N16, hp.q, hp.q_psi, hp.M, hp.p, hp.p_psi, hp.alpha, kb, ka)
+        assert np.array_equal(gb[c * comp:(c + 1) * comp], wb) and np.array_equal(ga[c * comp:(c + 1) * comp], wa)
+    # tuned vs a fresh default plan on the same data, alternating
+    ctx2 = hip.Context(N16, hp.q + hp.p, hp.q_psi + hp.p_psi)
+    ks2 = hip.KeySwitch(ctx2, 24, 6, 6, B)
+    ks2.upload_key(kb, ka, hp.M, hp.q)
+    st = hip.Stream(0)
+    e0, e1 = hip.Event(), hip.Event()
+
+    def timed(plan):
+        plan.run(d_in.ptr, B, ob.ptr, oa.ptr, ws.ptr, st.handle)
+        e0.record(st.handle)
+        for _ in range(3):
+            plan.run(d_in.ptr, B, ob.ptr, oa.ptr, ws.ptr, st.handle)
+        e1.record(st.handle)
+        e1.synchronize()
+        return e0.elapsed_ms(e1)
+
+    td, tt = [], []
+    for _ in range(4):
+        td.append(timed(ks2))
+        tt.append(timed(ks))
+    assert statistics.median(tt) <= 1.05 * statistics.median(td), (rep, td, tt)

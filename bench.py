@@ -8,6 +8,8 @@ A "step" is one batched key switch of `--batch` components already resident
 in HBM. Prints one JSON line.
 """
 import argparse
+import contextlib
+import ctypes
 import json
 import os
 import statistics
@@ -38,6 +40,32 @@ T0 = time.perf_counter()
 
 def progress(msg):
     print(f"[bench {time.perf_counter() - T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
+
+
+_ROCTX = []
+
+
+@contextlib.contextmanager
+def trace_range(name):
+    """A roctx range around a timed section, so that a kernel trace taken with
+    rocprofv3 --marker-trace can pick out the dispatches of that section from
+    the tuner's and the warm-up's (a no-op when no profiler is attached or the
+    marker library is absent)."""
+    if not _ROCTX:
+        try:
+            lib = ctypes.CDLL("librocprofiler-sdk-roctx.so.1")
+            lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
+            _ROCTX.append(lib)
+        except OSError:
+            _ROCTX.append(None)
+    lib = _ROCTX[0]
+    if lib is not None:
+        lib.roctxRangePushA(name.encode())
+    try:
+        yield
+    finally:
+        if lib is not None:
+            lib.roctxRangePop()
 
 
 def pct(values, q):
@@ -72,17 +100,21 @@ def ntt_roofline(towers, reps, stream, bits=59, autotune=True):
     chunk = 256
     for t0 in range(0, towers, chunk):
         data.upload(seeded(N, [q] * min(chunk, towers - t0), 17 + t0), offset=t0 * N)
-    e0, e1 = hip.Event(), hip.Event()
+    events = [hip.Event() for _ in range(reps + 1)]
     for w in range(6):  # warm: steady-state clocks and caches before the timed launches
         ctx.transform_batch(data.ptr, towers, N, 0, 1, w % 2 == 1, stream=stream)
     hip.Stream(0, stream).synchronize()
-    times = []
-    for r in range(reps):
-        e0.record(stream)
-        ctx.transform_batch(data.ptr, towers, N, 0, 1, r % 2 == 1, stream=stream)
-        e1.record(stream)
-        e1.synchronize()
-        times.append(e0.elapsed_ms(e1))
+    # One more untimed launch keeps the device busy while the timed ones are
+    # enqueued back to back, so each event interval is device time of one
+    # launch, not the host's submission latency.
+    ctx.transform_batch(data.ptr, towers, N, 0, 1, True, stream=stream)
+    with trace_range(f"ntt roofline {bits}-bit timed"):
+        events[0].record(stream)
+        for r in range(reps):
+            ctx.transform_batch(data.ptr, towers, N, 0, 1, r % 2 == 1, stream=stream)
+            events[r + 1].record(stream)
+        events[-1].synchronize()
+    times = [events[r].elapsed_ms(events[r + 1]) for r in range(reps)]
     fwd = [t for i, t in enumerate(times) if i % 2 == 0]
     inv = [t for i, t in enumerate(times) if i % 2 == 1]
     alg_bytes = 16 * N * towers  # read + write residues once per direction
@@ -616,13 +648,14 @@ def main():
         graph.launch(s)
     stream.synchronize()
     events = [hip.Event() for _ in range(args.steps + 1)]
-    t0 = time.perf_counter()
-    events[0].record(s)
-    for i in range(args.steps):
-        graph.launch(s)
-        events[i + 1].record(s)
-    stream.synchronize()
-    wall = time.perf_counter() - t0
+    with trace_range("batched key switch timed"):
+        t0 = time.perf_counter()
+        events[0].record(s)
+        for i in range(args.steps):
+            graph.launch(s)
+            events[i + 1].record(s)
+        stream.synchronize()
+        wall = time.perf_counter() - t0
     step_ms = [events[i].elapsed_ms(events[i + 1]) for i in range(args.steps)]
     batched_kss = B * args.steps / wall
     gpu_b = d_b.download(comp, 0)

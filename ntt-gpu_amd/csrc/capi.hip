@@ -278,19 +278,26 @@ std::shared_ptr<BasisPlanDev> basis_plan(ntt_hip_context &c, const std::vector<s
   static_assert(sizeof(BasisConst) <= 64, "plan header");
   const size_t off_src = 64, off_dst = off_src + s * sizeof(SourceConst), off_hat = off_dst + t * sizeof(TargetConst);
   const size_t off_limb = off_hat + s * t * sizeof(uint64_t);
-  const size_t bytes = off_limb + t * s * 3 * sizeof(uint32_t);
+  const size_t bytes = off_limb + t * s * 6 * sizeof(uint32_t);
   plan->blob = std::make_unique<DeviceBlob>(c.device, bytes);
   char *base = (char *)plan->blob->ptr;
   BasisConst head{(uint32_t)s, (uint32_t)t, (const SourceConst *)(base + off_src),
                   (const TargetConst *)(base + off_dst), (const uint64_t *)(base + off_hat),
                   (const uint32_t *)(base + off_limb)};
-  // register kernel's copy: target-major, each constant (< 2^63) as 21-bit limbs
-  std::vector<uint32_t> limbs(t * s * 3);
-  for (size_t m = 0; m < t; ++m)
+  // register kernel's copy: target-major; for the source's 21-bit limb j the
+  // constant hat * 2^(21 j) mod m, as {low, high} 32-bit halves
+  std::vector<uint32_t> limbs(t * s * 6);
+  for (size_t m = 0; m < t; ++m) {
+    const uint64_t qm = tc[m].q;
     for (size_t k = 0; k < s; ++k) {
-      const uint64_t h = hats[k * t + m];
-      for (int l = 0; l < 3; ++l) limbs[(m * s + k) * 3 + l] = (uint32_t)((h >> (21 * l)) & ((1u << 21) - 1));
+      uint64_t h = hats[k * t + m];
+      for (int j = 0; j < 3; ++j) {
+        limbs[((m * s + k) * 3 + j) * 2] = (uint32_t)h;
+        limbs[((m * s + k) * 3 + j) * 2 + 1] = (uint32_t)(h >> 32);
+        h = mulmod(h, (uint64_t{1} << 21) % qm, qm);
+      }
     }
+  }
   std::vector<char> host(bytes);
   std::memcpy(host.data(), &head, sizeof(head));
   std::memcpy(host.data() + off_src, sc.data(), s * sizeof(SourceConst));

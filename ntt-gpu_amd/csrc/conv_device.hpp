@@ -16,7 +16,9 @@ struct BasisConst {
   const SourceConst *src;    // [s]
   const TargetConst *dst;    // [t]
   const uint64_t *hat_mont;  // [s * t]: (Bhat_i mod m) * 2^64 mod m, source-major
-  const uint32_t *hat_limbs; // [t][s][3]: the same constants as three 21-bit limbs, target-major
+  // [t][s][3][2], target-major: for limb j, hat_mont * 2^(21 j) mod m as
+  // {low, high} 32-bit halves (the register kernel's form)
+  const uint32_t *hat_limbs;
 };
 
 struct BasisLaunch {

@@ -11,13 +11,17 @@
 // m * 2^64 (very wide or very many source primes).
 //
 // Up to 16 sources (every ModUp digit and ModDown of the key-switch shapes)
-// take a register kernel: the scaled sources y_i stay in VGPRs, the plan is
-// read through scalar loads, and each constant is split into three 21-bit
-// limbs so the six partial products y_{0,1} * h_{0,1,2} of every term are
-// plain v_mad_u64_u32 accumulations that cannot overflow 64 bits (each sum
-// is below 16 * 2^53); one 128-bit combine and one Montgomery reduction
-// finish each output.
+// take a register kernel specialised on the source count: each scaled source
+// y_i is split into three 21-bit limbs y_ij (kept in VGPRs), and the plan
+// holds, per (target, source, limb), h_ij = Bhat_i * 2^64 * 2^(21 j) mod m as
+// two 32-bit halves (read through scalar loads). Then
+//   sum_i y_i Bhat_i 2^64 = sum_ij y_ij h_ij = C_lo + 2^32 C_hi (mod m),
+// C_lo = sum y_ij lo32(h_ij), C_hi = sum y_ij hi32(h_ij): 6s v_mad_u64_u32
+// into two 64-bit columns that cannot overflow (each below 48 * 2^53), one
+// shifted add and one Montgomery reduction per output, exact for any
+// source primes (the sum is below 3 s 2^21 m < m 2^64).
 #include <algorithm>
+#include <utility>
 
 #include "conv_device.hpp"
 
@@ -69,48 +73,52 @@ typedef const __attribute__((address_space(4))) uint32_t *CLimbs;
 
 constexpr uint32_t kRegSources = 16;
 
-// Same grid as basis_switch_kernel; s <= kRegSources and sum_i b_i < 2^64.
+// Same grid as basis_switch_kernel; S = s source towers (1..kRegSources).
+template <int S>
 __global__ __launch_bounds__(256) void basis_switch_regs(BasisLaunch a) {
   const BasisConst &p = *a.plan;
   const CSrc src = (CSrc)(const void *)p.src;
   const CDst dst = (CDst)(const void *)p.dst;
   const CLimbs limbs = (CLimbs)(const void *)p.hat_limbs;
-  const uint32_t s = a.plan_host_s, t = a.plan_host_t;
+  const uint32_t t = a.plan_host_t;
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   const uint32_t chunks = (t + a.targets_per_block - 1) / a.targets_per_block;
   const uint32_t poly = blockIdx.y / chunks, chunk = blockIdx.y - poly * chunks;
   const uint64_t *in = a.in + (uint64_t)poly * a.in_poly_stride;
   uint64_t *out = a.out + (uint64_t)poly * a.out_poly_stride;
   if (i >= a.n) return;
-  uint64_t y[kRegSources];
+  constexpr uint32_t kMask21 = (1u << 21) - 1;
+  uint32_t y[3 * S];
 #pragma unroll
-  for (uint32_t k = 0; k < kRegSources; ++k)
-    if (k < s) y[k] = in[(uint64_t)k * a.n + i];
-#pragma unroll
-  for (uint32_t k = 0; k < kRegSources; ++k)
-    if (k < s) y[k] = shoup(y[k], src[k].inv, src[k].inv_shoup, src[k].q);
+  for (int k = 0; k < S; ++k) {
+    const uint64_t v = shoup(in[(uint64_t)k * a.n + i], src[k].inv, src[k].inv_shoup, src[k].q);
+    y[3 * k] = (uint32_t)v & kMask21;
+    y[3 * k + 1] = (uint32_t)(v >> 21) & kMask21;
+    y[3 * k + 2] = (uint32_t)(v >> 42);
+  }
   const uint32_t m0 = chunk * a.targets_per_block;
   const uint32_t m1 = min(t, m0 + a.targets_per_block);
   for (uint32_t m = m0; m < m1; ++m) {
-    const CLimbs h = limbs + (uint64_t)m * s * 3;
-    uint64_t p00 = 0, p01 = 0, p02 = 0, p10 = 0, p11 = 0, p12 = 0;
+    const CLimbs h = limbs + (uint64_t)m * (6 * S);
+    uint64_t lo = 0, hi = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < kRegSources; ++k) {
-      if (k < s) {
-        const uint32_t y0 = (uint32_t)y[k], y1 = (uint32_t)(y[k] >> 32);
-        const uint32_t h0 = h[3 * k], h1 = h[3 * k + 1], h2 = h[3 * k + 2];
-        p00 += (uint64_t)y0 * h0;
-        p01 += (uint64_t)y0 * h1;
-        p02 += (uint64_t)y0 * h2;
-        p10 += (uint64_t)y1 * h0;
-        p11 += (uint64_t)y1 * h1;
-        p12 += (uint64_t)y1 * h2;
-      }
+    for (int j = 0; j < 3 * S; ++j) {
+      lo += (uint64_t)y[j] * h[2 * j];
+      hi += (uint64_t)y[j] * h[2 * j + 1];
     }
-    const uint64_t r = mont_reduce(combine_limbs(p00, p01, p02, p10, p11, p12), dst[m].q, dst[m].qinv_neg);
+    U128 sum{lo + (hi << 32), hi >> 32};
+    sum.hi += sum.lo < lo ? 1u : 0u;
+    const uint64_t r = mont_reduce(sum, dst[m].q, dst[m].qinv_neg);
     const uint32_t ot = m < a.skip_at ? m : m + a.skip_len;
     out[(uint64_t)ot * a.n + i] = r;
   }
+}
+
+typedef void (*BasisKernel)(BasisLaunch);
+template <int... S>
+static BasisKernel regs_kernel(uint32_t s, std::integer_sequence<int, S...>) {
+  static constexpr BasisKernel table[] = {basis_switch_regs<S + 1>...};
+  return table[s - 1];
 }
 
 // Targets per workgroup when the caller leaves it to the launcher: as many as
@@ -127,8 +135,9 @@ hipError_t launch_basis_switch(const BasisLaunch &in, bool per_term, uint32_t po
   if (a.targets_per_block == 0) a.targets_per_block = auto_targets(a.plan_host_t, a.n, polys);
   const uint32_t chunks = (a.plan_host_t + a.targets_per_block - 1) / a.targets_per_block;
   dim3 grid((a.n + 255) / 256, chunks * polys);
-  if (!per_term && a.plan_host_s <= kRegSources) {
-    hipLaunchKernelGGL(basis_switch_regs, grid, dim3(256), 0, stream, a);
+  if (a.plan_host_s >= 1 && a.plan_host_s <= kRegSources) {
+    hipLaunchKernelGGL(regs_kernel(a.plan_host_s, std::make_integer_sequence<int, kRegSources>()), grid, dim3(256),
+                       0, stream, a);
     return hipGetLastError();
   }
   const size_t lds = (size_t)a.plan_host_s * 256 * sizeof(uint64_t);
@@ -218,8 +227,8 @@ __global__ __launch_bounds__(256) void mac_fused_kernel(MacLaunch m, uint32_t po
     U128 sb{0, 0}, sa{0, 0};
     for (uint32_t d = 0; d < m.digits_count; ++d) {
       const uint64_t x = d == own_digit ? own : dg[d * step];
-      add_wide(sb, mul_wide(x, kb[d * step]));
-      add_wide(sa, mul_wide(x, ka[d * step]));
+      add_wide(sb, mul_wide63(x, kb[d * step]));
+      add_wide(sa, mul_wide63(x, ka[d * step]));
     }
     rb = mont_reduce(sb, pc.q, pc.qinv_neg);
     ra = mont_reduce(sa, pc.q, pc.qinv_neg);

@@ -51,11 +51,26 @@ __device__ __forceinline__ uint64_t sub_mod(uint64_t a, uint64_t b, uint64_t q) 
   return a >= b ? a - b : a + (q - b);
 }
 
+// a * b for any a < 2^64 and b < 2^63 from four 32x32 products chained
+// through v_mad_u64_u32: with b1 = b >> 32 < 2^31 none of the partial sums
+// can overflow 64 bits, so no carries are tracked (the generic a * b plus
+// __umul64hi recomputes partial products and propagates carries).
+//   t = a0 b1 + hi32(a0 b0) < 2^63 + 2^32,  u = a1 b0 + lo32(t) < 2^64,
+//   a b = lo32(a0 b0) + 2^32 lo32(u) + 2^64 (a1 b1 + hi32(t) + hi32(u)).
+__host__ __device__ __forceinline__ U128 mul_wide63(uint64_t a, uint64_t b) {
+  const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+  const uint64_t p00 = (uint64_t)a0 * b0;
+  const uint64_t t = (uint64_t)a0 * b1 + (p00 >> 32);
+  const uint64_t u = (uint64_t)a1 * b0 + (uint32_t)t;
+  const uint64_t hi = (uint64_t)a1 * b1 + ((t >> 32) + (u >> 32));
+  return {(u << 32) | (uint32_t)p00, hi};
+}
+
 // Montgomery reduction: S = hi:lo with S < q * 2^64 -> S * 2^-64 mod q, canonical.
-// qinv_neg = -q^-1 mod 2^64.
+// qinv_neg = -q^-1 mod 2^64; q < 2^63 (every prime of this library).
 __device__ __forceinline__ uint64_t mont_reduce(U128 s, uint64_t q, uint64_t qinv_neg) {
   const uint64_t m = s.lo * qinv_neg;
-  const uint64_t t = s.hi + mulhi64(m, q) + (s.lo != 0 ? 1u : 0u);
+  const uint64_t t = s.hi + mul_wide63(m, q).hi + (s.lo != 0 ? 1u : 0u);
   return t >= q ? t - q : t;
 }
 

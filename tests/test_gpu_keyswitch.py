@@ -43,7 +43,8 @@ def test_switch_basis(small, src, dst):
 
 
 def test_switch_basis_per_term_path():
-    """Four 63-bit sources: sum b_i >= 2^64 selects per-term reduction."""
+    """Four 63-bit sources (sum b_i >= 2^64): the register kernel's limb
+    form is exact whatever the source widths."""
     n = 256
     qs = params.find_primes(n, 6, 63)
     ctx = hip.Context(n, qs, [params.find_psi(q, n) for q in qs])
@@ -55,6 +56,42 @@ def test_switch_basis_per_term_path():
     ctx.switch_basis(out.ptr, dx.ptr, [0, 1, 2, 3], [4, 5])
     ctx.synchronize()
     assert np.array_equal(out.download(), o.basis_switch(x, n, qs[:4], qs[4:]))
+
+
+@pytest.mark.parametrize("s", [1, 3, 7, 11, 16])
+@pytest.mark.parametrize("bits", [63, 50, 31])
+def test_switch_basis_register_kernel_widths(s, bits):
+    """Every source count of the register kernel (1..16, one specialisation
+    each) at 63-, 50- and 31-bit primes, extreme residues included."""
+    n = 256
+    qs = params.find_primes(n, s + 3, bits)
+    ctx = hip.Context(n, qs, [params.find_psi(q, n) for q in qs])
+    x = uniform(n, qs[:s], 1000 + 31 * s + bits)
+    x[::5] = np.repeat(np.array(qs[:s], np.uint64) - 1, n)[::5]
+    x[1::9] = 0
+    log_case(f"switch_basis register kernel s={s} {bits}-bit")
+    out = hip.DeviceBuffer(3 * n)
+    dx = _dev(x)
+    ctx.switch_basis(out.ptr, dx.ptr, list(range(s)), [s, s + 1, s + 2])
+    ctx.synchronize()
+    assert np.array_equal(out.download(), o.basis_switch(x, n, qs[:s], qs[s:]))
+
+
+@pytest.mark.parametrize("bits,s", [(30, 20), (62, 18)])
+def test_switch_basis_wide_source_sets(bits, s):
+    """More than 16 sources: the LDS kernel, with one reduction per output
+    (20 x 30-bit, sum b_i < 2^64) or per-term reductions (18 x 62-bit)."""
+    n = 256
+    qs = params.find_primes(n, s + 4, bits)
+    ctx = hip.Context(n, qs, [params.find_psi(q, n) for q in qs])
+    x = uniform(n, qs[:s], 77 + s)
+    x[::3] = np.repeat(np.array(qs[:s], np.uint64) - 1, n)[::3]
+    log_case(f"switch_basis LDS kernel s={s} {bits}-bit")
+    out = hip.DeviceBuffer(4 * n)
+    dx = _dev(x)
+    ctx.switch_basis(out.ptr, dx.ptr, list(range(s)), list(range(s, s + 4)))
+    ctx.synchronize()
+    assert np.array_equal(out.download(), o.basis_switch(x, n, qs[:s], qs[s:]))
 
 
 def test_switch_basis_rejections(small):

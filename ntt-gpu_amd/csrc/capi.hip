@@ -692,6 +692,9 @@ int ntt_hip_context_create(ntt_hip_context **out, int device, uint64_t n, const 
       const u128 mu = ~(u128)0 / qt;  // = floor(2^128 / q): q is odd, so it does not divide 2^128
       p.mu_lo = (uint64_t)mu;
       p.mu_hi = (uint64_t)(mu >> 64);
+      const int bits = 64 - __builtin_clzll(qt);
+      p.est_shift = bits >= 24 ? (uint32_t)(bits - 24) : 0;
+      p.est_mul = bits >= 24 ? (uint32_t)(((u128)1 << (bits + 31)) / qt) : 0;
       p.fwd = dt + (t * 2) * (size_t)n;
       p.inv = dt + (t * 2 + 1) * (size_t)n;
     }

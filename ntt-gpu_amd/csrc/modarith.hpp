@@ -68,7 +68,7 @@ __host__ __device__ __forceinline__ U128 mul_wide63(uint64_t a, uint64_t b) {
 
 // Montgomery reduction: S = hi:lo with S < q * 2^64 -> S * 2^-64 mod q, canonical.
 // qinv_neg = -q^-1 mod 2^64; q < 2^63 (every prime of this library).
-__device__ __forceinline__ uint64_t mont_reduce(U128 s, uint64_t q, uint64_t qinv_neg) {
+__host__ __device__ __forceinline__ uint64_t mont_reduce(U128 s, uint64_t q, uint64_t qinv_neg) {
   const uint64_t m = s.lo * qinv_neg;
   const uint64_t t = s.hi + mul_wide63(m, q).hi + (s.lo != 0 ? 1u : 0u);
   return t >= q ? t - q : t;
@@ -77,20 +77,6 @@ __device__ __forceinline__ uint64_t mont_reduce(U128 s, uint64_t q, uint64_t qin
 // Montgomery product of canonical a and b_mont = b * 2^64 mod q -> a * b mod q.
 __device__ __forceinline__ uint64_t mont_mul(uint64_t a, uint64_t b_mont, uint64_t q, uint64_t qinv_neg) {
   return mont_reduce(mul_wide(a, b_mont), q, qinv_neg);
-}
-
-// sum_k y_k h_k from limb sums p_ab = sum_k y_a h_b, where y = y1 2^32 + y0 and
-// h = h2 2^42 + h1 2^21 + h0 (21-bit limbs): Q_a = p_a0 + p_a1 2^21 + p_a2 2^42,
-// S = Q_0 + Q_1 2^32. Exact whenever S < 2^127 (then Q_1 < 2^95).
-__host__ __device__ __forceinline__ U128 combine_limbs(uint64_t p00, uint64_t p01, uint64_t p02, uint64_t p10,
-                                                       uint64_t p11, uint64_t p12) {
-  U128 q0{p00, 0}, q1{p10, 0};
-  add_wide(q0, {p01 << 21, p01 >> 43});
-  add_wide(q0, {p02 << 42, p02 >> 22});
-  add_wide(q1, {p11 << 21, p11 >> 43});
-  add_wide(q1, {p12 << 42, p12 >> 22});
-  add_wide(q0, {q1.lo << 32, (q1.hi << 32) | (q1.lo >> 32)});
-  return q0;
 }
 
 // ---------------------------------------------------------------- wide-lazy butterflies (q < 2^61)
@@ -194,13 +180,19 @@ __host__ __device__ __forceinline__ void gs8(uint64_t &x, uint64_t &y, uint64_t 
   y = shoup4(d, w, wp, m);
 }
 
-// Any x < 2^64 -> [0, 4q), congruent (q < 2^61): x - e q with e the quotient
-// estimate of x v / 2^64, v = floor(2^64 / q). floor(x v / 2^64) is at most
-// one short of floor(x / q) for x < 2^64, mulhi_est at most two more, so the
-// remainder is below 4q. Cheaper than a chain of conditional subtracts from
-// bounds of 32q and more.
-__host__ __device__ __forceinline__ uint64_t reduce4(uint64_t x, uint64_t v, const Lazy8 &m) {
-  return mad_lo(x, 0, 0, mulhi_est(x, v), m.nq);
+// x < 2^7 q -> [0, 2q), congruent, for q of bit length b >= 24 (every
+// residue of a planned wide-lazy forward transform is below 2^7 q): x - e q
+// with e = floor(x' r / 2^55), x' = floor(x / 2^(b - 24)) < 2^31 and
+// r = floor(2^(b + 31) / q) in [2^31, 2^32) (shift = b - 24, mul = r).
+// x' r / 2^55 <= x / q, and it exceeds x / q - 2^-24 - 2^-23 (the truncations
+// of x' and r), so e is floor(x / q) or one less. One 64-bit shift, one
+// v_mul_hi_u32 and a shift for e, then x + e (2^64 - q): about 6 VALU
+// instructions against 11 for a 64-bit quotient estimate or 4 per
+// conditional subtract.
+__host__ __device__ __forceinline__ uint64_t reduce2_est(uint64_t x, uint32_t shift, uint32_t mul, uint64_t nq) {
+  const uint32_t xs = (uint32_t)(x >> shift);
+  const uint32_t e = (uint32_t)(((uint64_t)xs * mul) >> 55);
+  return x + (uint64_t)e * nq;
 }
 
 // [0, 8q) -> [0, q).

@@ -15,6 +15,7 @@ struct PrimeConst {
   uint64_t r2;                       // 2^128 mod q (to enter Montgomery form)
   uint64_t nq, nq2, nq4;             // 2^64 - q, 2^64 - 2q, 2^64 - 4q (precomputed: see Lazy8)
   uint64_t mu_lo, mu_hi;             // floor(2^128 / q) (Barrett, variable x variable products)
+  uint32_t est_shift, est_mul;       // b - 24 and floor(2^(b+31) / q), b = bit length of q >= 24 (reduce2_est)
   const ulonglong2 *fwd;
   const ulonglong2 *inv;
 };

@@ -224,15 +224,16 @@ __device__ __forceinline__ void ct_planned(Val<A> &x, Val<A> &y, tw2 w, const La
   }
 }
 
-// Forward output of plan H -> canonical: conditional subtracts from the
-// plan's bound, or (bounds of 32q and more) the quotient-estimate reduction
-// to [0, 4q) first (v = floor(2^64 / q)).
+// Forward output of plan H -> canonical: conditional subtracts from bounds
+// up to 8q, from 16q and more the quotient estimate (reduce2_est, to [0, 2q))
+// and one subtract.
 template <int A, int H>
-__device__ __forceinline__ Val<A> canon_planned(Val<A> x, const Lazy8 &m, const NegMultiples &neg, uint64_t v) {
+__device__ __forceinline__ Val<A> canon_planned(Val<A> x, const Lazy8 &m, const NegMultiples &neg,
+                                                const PrimeConst &pc) {
   if constexpr (A == kLazy && H > 0) {
     constexpr int bl = FwdPlan<H>::d.bound_log;
-    if constexpr (bl >= 5) {
-      return sub_if_ge_neg(sub_if_ge_neg(reduce4(x, v, m), neg.c[1]), neg.c[0]);
+    if constexpr (bl >= 4) {
+      return sub_if_ge_neg(reduce2_est(x, pc.est_shift, pc.est_mul, m.nq), neg.c[0]);
     } else {
 #pragma unroll
       for (int k = bl - 1; k >= 0; --k) x = sub_if_ge_neg(x, neg.c[k]);
@@ -245,11 +246,12 @@ __device__ __forceinline__ Val<A> canon_planned(Val<A> x, const Lazy8 &m, const 
 
 // Forward output of plan H -> below 8q (the input range of epi_diff).
 template <int A, int H>
-__device__ __forceinline__ Val<A> below8_planned(Val<A> x, const Lazy8 &m, const NegMultiples &neg, uint64_t v) {
+__device__ __forceinline__ Val<A> below8_planned(Val<A> x, const Lazy8 &m, const NegMultiples &neg,
+                                                 const PrimeConst &pc) {
   if constexpr (A == kLazy && H > 0) {
     constexpr int bl = FwdPlan<H>::d.bound_log;
-    if constexpr (bl >= 6) {
-      return reduce4(x, v, m);
+    if constexpr (bl >= 5) {
+      return reduce2_est(x, pc.est_shift, pc.est_mul, m.nq);
     } else {
 #pragma unroll
       for (int k = bl - 1; k >= 3; --k) x = sub_if_ge_neg(x, neg.c[k]);
@@ -528,19 +530,19 @@ __global__ __launch_bounds__(256) void ntt_pass_b8_fwd(TowerMap map, const Prime
     if (acc) {
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
-        const uint64_t d = epi_diff<A>(sub[u + 32 * k], below8_planned<A, H>(v[k], m, neg, pc.mu_hi), m, pc.q);
+        const uint64_t d = epi_diff<A>(sub[u + 32 * k], below8_planned<A, H>(v[k], m, neg, pc), m, pc.q);
         out[u + 32 * k] = add_mod(out[u + 32 * k], shoup(d, c.x, c.y, pc.q), pc.q);
       }
       return;
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const uint64_t d = epi_diff<A>(sub[u + 32 * k], below8_planned<A, H>(v[k], m, neg, pc.mu_hi), m, pc.q);
+      const uint64_t d = epi_diff<A>(sub[u + 32 * k], below8_planned<A, H>(v[k], m, neg, pc), m, pc.q);
       out[u + 32 * k] = shoup(d, c.x, c.y, pc.q);
     }
   } else {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) buf[pad16((u << 3) | k)] = canon_planned<A, H>(v[k], m, neg, pc.mu_hi);
+    for (int k = 0; k < 8; ++k) buf[pad16((u << 3) | k)] = canon_planned<A, H>(v[k], m, neg, pc);
     wave_sync8();
 #pragma unroll
     for (int k = 0; k < 8; ++k) row[u + 32 * k] = buf[pad16(u + 32 * k)];

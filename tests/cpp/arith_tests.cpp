@@ -3,6 +3,7 @@
 // gfx950 kernels inline. Every claimed range is checked on the extremes
 // (y = 2^64 - 1, x = 8q - 1, w = q - 1, q just below 2^61) and on seeded
 // random operands for OpenFHE-sized and adversarial primes.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <initializer_list>
@@ -26,6 +27,12 @@ void check(bool ok, const std::string &why) {
 }
 uint64_t mulmod(uint64_t a, uint64_t b, uint64_t q) { return (uint64_t)((u128)a * b % q); }
 uint64_t shoup_companion(uint64_t w, uint64_t q) { return (uint64_t)(((u128)w << 64) / q); }
+// reduce2_est's per-prime constants, as the library computes them
+void est_constants(uint64_t q, uint32_t &shift, uint32_t &mul) {
+  const int bits = 64 - __builtin_clzll(q);
+  shift = (uint32_t)(bits - 24);
+  mul = (uint32_t)(((u128)1 << (bits + 31)) / q);
+}
 
 void one(uint64_t q, uint64_t w, uint64_t x, uint64_t y) {
   const Lazy8 m = lazy8_of(q);
@@ -141,11 +148,12 @@ void planned_forward_transforms(std::mt19937_64 &rng) {
         }
       }
       check(bound <= (uint64_t{1} << plan.bound_log), "plan bound_log covers the output");
-      const uint64_t v = (uint64_t)((~(u128)0 / q) >> 64);
+      uint32_t shift, mul;
+      est_constants(q, shift, mul);
       for (int i = 0; i < n; ++i) {
         uint64_t x = a[i], y = a[i];
         for (int k = plan.bound_log - 1; k >= 0; --k) x = sub_if_ge_neg(x, 0 - (q << k));
-        y = sub_if_ge_neg(sub_if_ge_neg(reduce4(y, v, m), 0 - 2 * q), 0 - q);
+        y = sub_if_ge_neg(reduce2_est(y, shift, mul, m.nq), 0 - q);
         if (x != want[i] || y != want[i])
           check(false, "planned transform equals the exact transform, H = " + std::to_string(H));
       }
@@ -172,14 +180,26 @@ int main() {
       one(q, w, x, y);
     }
   }
-  // quotient-estimate reduction of any 64-bit value to [0, 4q)
-  for (int i = 0; i < 2000000; ++i) {
-    uint64_t q = (rng() >> (3 + i % 50)) | 1;
-    if (i % 7 == 0) q = (uint64_t{1} << 61) - 1 - 2 * (rng() & 0xffff);
-    if (q < 3) q = 3;
-    const uint64_t x = (i % 5 == 0) ? ~0ULL - (rng() & 0xff) : rng();
-    const uint64_t r = reduce4(x, (uint64_t)((~(u128)0 / q) >> 64), lazy8_of(q));
-    check(r < 4 * q && r % q == x % q, "reduce4 in [0, 4q) and congruent");
+  // quotient-estimate reduction of x < 2^7 q (and x < 2^64) to [0, 2q), bit
+  // lengths 24..63, at the edges (k q - 1, k q, the bound - 1) and at random
+  for (int i = 0; i < 3000000; ++i) {
+    const int bits = 24 + i % 40;
+    uint64_t q = (uint64_t{1} << (bits - 1)) | (rng() >> (65 - bits)) | 1;
+    if (i % 9 == 0) q = (bits == 64 ? ~0ULL : (uint64_t{1} << bits) - 1) - 2 * (rng() & 0xff);
+    if (i % 11 == 0) q = (uint64_t{1} << (bits - 1)) + 1 + 2 * (rng() & 0xff);
+    uint32_t shift, mul;
+    est_constants(q, shift, mul);
+    const u128 bound = std::min((u128)q << 7, (u128)1 << 64);
+    const uint64_t k = 1 + rng() % 127;
+    uint64_t x;
+    switch (i % 6) {
+      case 0: x = (uint64_t)(bound - 1); break;
+      case 1: x = (u128)k * q < bound ? k * q : 0; break;
+      case 2: x = (u128)k * q - 1 < bound ? k * q - 1 : q - 1; break;
+      default: x = (uint64_t)((((u128)rng() << 64) | rng()) % bound);
+    }
+    const uint64_t r = reduce2_est(x, shift, mul, 0 - q);
+    check(mul >= (1u << 31) && r < 2 * q && r % q == x % q, "reduce2_est in [0, 2q) and congruent");
   }
   // mulhi_est on arbitrary 64-bit operands (not only Shoup companions)
   for (int i = 0; i < 1000000; ++i) {
@@ -197,25 +217,48 @@ int main() {
   for (const uint64_t c : edges)
     for (const uint64_t x : {uint64_t{0}, c - 1, c, 2 * c - 1})
       check(sub_if_ge_neg(x, 0 - c) == (x >= c ? x - c : x), "sub_if_ge_neg at the edges");
-  // limb-split dot product of the register conversion kernel: up to 16 terms,
-  // y below its source prime, constants below 2^63, total below 2^127
-  for (int i = 0; i < 200000; ++i) {
+  // register conversion kernel: 21-bit source limbs y_kj against
+  // h_kj = hat_k 2^(21 j) mod m in two 32-bit-aligned columns; up to 16
+  // sources below 2^63, m below 2^63; the columns never overflow and the
+  // Montgomery reduction of C_lo + 2^32 C_hi is (sum_k y_k hat_k) 2^-64 mod m
+  for (int i = 0; i < 300000; ++i) {
     const int s = 1 + (int)(rng() % 16);
     const bool extreme = (i % 5) == 0;
-    uint64_t p[6] = {0, 0, 0, 0, 0, 0};
-    u128 want = 0;
+    uint64_t m = ((rng() >> (1 + i % 33)) | 1) | (uint64_t{1} << 24);
+    if (i % 7 == 0) m = (uint64_t{1} << 63) - 1 - 2 * (rng() & 0xffff);
+    u128 lo = 0, hi = 0, want = 0;
     for (int k = 0; k < s; ++k) {
-      // sum of source primes < 2^64: each below 2^64 / 16
-      const uint64_t y = extreme ? (~0ULL >> 4) - 1 : rng() >> 4;
-      const uint64_t h = extreme ? (~0ULL >> 1) : rng() >> 1;
-      want += (u128)y * h;
-      const uint32_t ya[2] = {(uint32_t)y, (uint32_t)(y >> 32)};
-      const uint32_t hb[3] = {(uint32_t)(h & 0x1fffff), (uint32_t)((h >> 21) & 0x1fffff), (uint32_t)(h >> 42)};
-      for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 3; ++b) p[a * 3 + b] += (uint64_t)ya[a] * hb[b];
+      const uint64_t y = extreme ? (uint64_t{1} << 63) - 1 : rng() >> 1;
+      uint64_t h = extreme ? m - 1 : rng() % m;
+      want = (want + (u128)y * h) % m;
+      for (int j = 0; j < 3; ++j) {
+        const uint32_t yj = j < 2 ? (uint32_t)(y >> (21 * j)) & 0x1fffff : (uint32_t)(y >> 42);
+        lo += (u128)yj * (uint32_t)h;
+        hi += (u128)yj * (uint32_t)(h >> 32);
+        h = (uint64_t)(((u128)h << 21) % m);
+      }
     }
-    const U128 got = combine_limbs(p[0], p[1], p[2], p[3], p[4], p[5]);
-    check(got.lo == (uint64_t)want && got.hi == (uint64_t)(want >> 64), "limb-split dot product is exact");
+    check(lo < ((u128)1 << 64) && hi < ((u128)1 << 64), "conversion columns fit 64 bits");
+    const uint64_t l64 = (uint64_t)lo, h64 = (uint64_t)hi;
+    U128 sum{l64 + (h64 << 32), h64 >> 32};
+    sum.hi += sum.lo < l64 ? 1u : 0u;
+    check(((((u128)sum.hi << 64) | sum.lo) % m) == want && (((u128)sum.hi << 64) | sum.lo) < ((u128)m << 64),
+          "conversion columns congruent and below m 2^64");
+    const uint64_t qinv_neg = [&] {
+      uint64_t inv = 1;
+      for (int b = 0; b < 6; ++b) inv *= 2 - m * inv;
+      return 0 - inv;
+    }();
+    const uint64_t r = mont_reduce(sum, m, qinv_neg);
+    check(((u128)r << 64) % m == want && r < m, "Montgomery reduction of the columns");
+  }
+  // 64 x 63-bit products of the Montgomery reduction and the key inner product
+  for (int i = 0; i < 2000000; ++i) {
+    const uint64_t a = (i % 5 == 0) ? ~0ULL - (rng() & 0xff) : rng();
+    const uint64_t b = (i % 7 == 0) ? (uint64_t{1} << 63) - 1 - (rng() & 0xff) : rng() >> 1;
+    const U128 p = mul_wide63(a, b);
+    const u128 w = (u128)a * b;
+    check(p.lo == (uint64_t)w && p.hi == (uint64_t)(w >> 64), "mul_wide63");
   }
   // Barrett variable x variable products: lazy form below 2^62, exact form below 2^63
   for (int i = 0; i < 3000000; ++i) {

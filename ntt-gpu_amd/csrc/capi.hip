@@ -183,6 +183,22 @@ struct ntt_hip_context {
     }
     return nttamd::ntt_mode(a, row16, wg_cap);
   }
+  // The mode of a forward launch over towers [first, first + count) only:
+  // the headroom class of their own largest modulus (at most the tuned
+  // fwd_headroom), so narrower towers skip reductions a wider tower of the
+  // basis would need. Inverse launches ignore the class.
+  int ntt_mode_for_q(uint64_t max_q) const {
+    int a = arith;
+    if ((a & kArithMask) == kLazy) {
+      int h = headroom_for(max_q);
+      if (fwd_headroom >= 0) h = std::min(h, fwd_headroom);
+      a = kLazy | (h << kArithShift);
+    }
+    return nttamd::ntt_mode(a, row16, wg_cap);
+  }
+  int ntt_mode_for(size_t first, size_t count) const {
+    return count ? ntt_mode_for_q(*std::max_element(q.begin() + first, q.begin() + first + count)) : ntt_mode();
+  }
   // Standalone transforms (ntt_hip_transform*) with no tuned chunk run the
   // towers in groups whose pass-A output (the hand-off pass B reads back)
   // fits the 256 MB Infinity Cache: measured at N = 2^16 on 2048 towers,
@@ -752,7 +768,9 @@ int ntt_hip_transform_jobs(ntt_hip_context *ctx, uint64_t *data, const uint64_t 
     m.src_poly_stride = 0;
     m.tower_stride = ctx->n;
     m.towers_per_poly = (uint32_t)count;
-    hip_ok(launch_ntt(m, ctx->dprimes(), ctx->log_n, (uint32_t)count, inverse != 0, ctx->ntt_mode(), s,
+    uint64_t max_q = 0;
+    for (size_t j = 0; j < count; ++j) max_q = std::max(max_q, ctx->q[jobs[j].tower]);
+    hip_ok(launch_ntt(m, ctx->dprimes(), ctx->log_n, (uint32_t)count, inverse != 0, ctx->ntt_mode_for_q(max_q), s,
                       ctx->transform_chunk()),
            "ntt job launch");
     return NTT_HIP_OK;
@@ -775,8 +793,8 @@ int ntt_hip_transform_batch(ntt_hip_context *ctx, uint64_t *data, const uint64_t
             "polynomial stride shorter than its towers");
     DeviceGuard g(ctx->device);
     const TowerMap m = affine(data, src, ctx->n, towers, first, poly_stride, src_poly_stride);
-    hip_ok(launch_ntt(m, ctx->dprimes(), ctx->log_n, (uint32_t)(polys * towers), inverse != 0, ctx->ntt_mode(),
-                      ctx->pick(stream), ctx->transform_chunk()),
+    hip_ok(launch_ntt(m, ctx->dprimes(), ctx->log_n, (uint32_t)(polys * towers), inverse != 0,
+                      ctx->ntt_mode_for(first, towers), ctx->pick(stream), ctx->transform_chunk()),
            "ntt launch");
     return NTT_HIP_OK;
   });
@@ -1160,7 +1178,7 @@ void ks_modup_mac(const ntt_hip_keyswitch &key, const ntt_hip_keyswitch &plan, c
     const uint32_t t1 = std::min(T, t0 + G);
     const uint32_t k0 = plan.tower_job_begin[t0], k1 = plan.tower_job_begin[t1];
     hip_ok(launch_ntt_span(m, c.dprimes(), c.log_n, (uint32_t)B, k0 * (uint32_t)B, (k1 - k0) * (uint32_t)B, false,
-                           c.ntt_mode(), s),
+                           c.ntt_mode_for(t0, t1 - t0), s),
            "modup ntt group");
     ks_mac(key, plan, in, B, w, z, s, t0, t1 - t0);
   }
@@ -1195,8 +1213,23 @@ void ks_moddown(const ntt_hip_keyswitch &ks, const uint64_t *z, size_t B, uint64
   TowerMap fin = with_correction(affine(w.corr, nullptr, n, L, 0, ks.corr_words(), 0), z, ks.z_words(), out_b, out_a,
                                  (uint32_t)B, L * n, ks.pinv);
   fin.epi.accumulate = accumulate;
-  hip_ok(launch_ntt(fin, pr, c.log_n, (uint32_t)(2 * B * L), false, c.ntt_mode(), s, c.ntt_chunk),
-         "moddown ntt + correction");
+  if (c.ntt_chunk) {
+    hip_ok(launch_ntt(fin, pr, c.log_n, (uint32_t)(2 * B * L), false, c.ntt_mode(), s, c.ntt_chunk),
+           "moddown ntt + correction");
+    return;
+  }
+  // one launch per run of towers of one headroom class (tower-major index:
+  // towers [t0, t1) of all 2B polynomials are indices [t0 2B, t1 2B))
+  const uint32_t polys = (uint32_t)(2 * B);
+  for (size_t t0 = 0; t0 < L;) {
+    const int mode = c.ntt_mode_for(t0, 1);
+    size_t t1 = t0 + 1;
+    while (t1 < L && c.ntt_mode_for(t1, 1) == mode) ++t1;
+    hip_ok(launch_ntt_span(fin, pr, c.log_n, polys, (uint32_t)(t0 * polys), (uint32_t)((t1 - t0) * polys), false, mode,
+                           s),
+           "moddown ntt + correction");
+    t0 = t1;
+  }
 }
 }  // namespace
 

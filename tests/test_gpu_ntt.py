@@ -222,6 +222,23 @@ def _top_primes(bits, count, n=N16):
     return qs
 
 
+def test_forward_class_follows_the_transformed_towers():
+    """A context of 60-, 50-, 40- and 59-bit top-of-class primes: a launch
+    over a range of towers takes the headroom class of that range's own widest
+    prime (ranges [0,4), [1,3), [2,3), [1,2), [3,4)), bit-exact vs the oracle
+    with every residue at q - 1 and seeded."""
+    qs = [_top_primes(60, 1)[0], _top_primes(50, 1)[0], _top_primes(40, 1)[0], _top_primes(59, 1)[0]]
+    rs = [params.min_root(q, N16) for q in qs]
+    ctx = hip.Context(N16, qs, rs)
+    for first, towers in [(0, 4), (1, 2), (2, 1), (1, 1), (3, 1)]:
+        sub_q, sub_r = qs[first:first + towers], rs[first:first + towers]
+        for name, x in [("q-1", np.concatenate([np.full(N16, q - 1, np.uint64) for q in sub_q])),
+                        ("uniform", uniform(N16, sub_q, 31 * first + towers))]:
+            log_case(f"mixed-class forward towers [{first}, {first + towers}) {name}")
+            got = _gpu_round(ctx, x, first, towers, False)  # the buffer holds the range's towers
+            assert np.array_equal(got, o.ntt_towers(x, N16, sub_q, sub_r)), f"[{first}, {first + towers}) {name}"
+
+
 @pytest.mark.parametrize("bits", [61, 60, 59, 58, 57, 56, 50])
 def test_forward_headroom_classes(bits):
     """The forward transform at N=2^16 skips x reductions while residues stay

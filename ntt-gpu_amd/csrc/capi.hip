@@ -1102,9 +1102,24 @@ void ks_modup_convert(const ntt_hip_keyswitch &ks, const uint64_t *in, size_t B,
 void ks_modup(const ntt_hip_keyswitch &ks, const uint64_t *in, size_t B, const KsBuffers &w, hipStream_t s) {
   const ntt_hip_context &c = *ks.ctx;
   ks_modup_convert(ks, in, B, w, s);
-  hip_ok(launch_ntt(complement_jobs(ks, w), c.dprimes(), c.log_n, (uint32_t)(B * ks.jobs_per_poly), false, c.ntt_mode(),
-                    s, c.ntt_chunk),
-         "modup ntt");
+  const TowerMap m = complement_jobs(ks, w);
+  if (c.ntt_chunk) {
+    hip_ok(launch_ntt(m, c.dprimes(), c.log_n, (uint32_t)(B * ks.jobs_per_poly), false, c.ntt_mode(), s, c.ntt_chunk),
+           "modup ntt");
+    return;
+  }
+  // jobs are ordered by tower: one launch per run of towers of one headroom class
+  const uint32_t T = ks.L + ks.K;
+  for (uint32_t t0 = 0; t0 < T;) {
+    const int mode = c.ntt_mode_for(t0, 1);
+    uint32_t t1 = t0 + 1;
+    while (t1 < T && c.ntt_mode_for(t1, 1) == mode) ++t1;
+    const uint32_t k0 = ks.tower_job_begin[t0], k1 = ks.tower_job_begin[t1];
+    hip_ok(launch_ntt_span(m, c.dprimes(), c.log_n, (uint32_t)B, k0 * (uint32_t)B, (k1 - k0) * (uint32_t)B, false, mode,
+                           s),
+           "modup ntt");
+    t0 = t1;
+  }
 }
 
 void ks_modup_convert(const ntt_hip_keyswitch &ks, const uint64_t *in, size_t B, const KsBuffers &w, hipStream_t s) {

@@ -245,10 +245,66 @@ __global__ __launch_bounds__(256) void mac_fused_kernel(MacLaunch m, uint32_t po
   m.out_a[o] = ra;
 }
 
+// mac_fused_kernel<false> for a digit count D fixed at compile time: the
+// digit loop unrolls, so every digit and key load is issued before the first
+// product waits on one (with a runtime count each iteration waited for its
+// own loads, one memory round trip per digit).
+template <int D>
+__global__ __launch_bounds__(256) void mac_fused_fixed(MacLaunch m, uint32_t polys) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t tower = m.first_tower + blockIdx.y / polys, poly = blockIdx.y % polys;
+  if (i >= m.n) return;
+  // the modulus through scalar loads, issued with the others
+  typedef const __attribute__((address_space(4))) PrimeConst *CPrime;
+  typedef const __attribute__((address_space(4))) uint32_t *CIndex;
+  const uint32_t prime = ((CIndex)(const void *)m.tower_prime)[tower];
+  const uint64_t q = ((CPrime)(const void *)m.primes)[prime].q;
+  const uint64_t qinv_neg = ((CPrime)(const void *)m.primes)[prime].qinv_neg;
+  const uint64_t *dg = m.digits + (uint64_t)poly * m.digit_poly_stride + (uint64_t)tower * m.n + i;
+  const uint64_t *kb = m.key_b + (uint64_t)tower * m.n + i;
+  const uint64_t *ka = m.key_a + (uint64_t)tower * m.n + i;
+  const uint64_t step = (uint64_t)m.towers * m.n;
+  const uint32_t own_digit = tower < m.active_q ? tower / m.alpha : 0xffffffffu;
+  // digit d's own towers come from the input: the address is selected, not
+  // the value, so no load waits behind a branch (P towers, tower >= active_q,
+  // have no own digit)
+  const uint64_t *ownp = m.own + (uint64_t)poly * m.own_poly_stride + (uint64_t)tower * m.n + i;
+  uint64_t x[D], b[D], a[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    x[d] = *((uint32_t)d == own_digit ? ownp : dg + d * step);
+    b[d] = kb[d * step];
+    a[d] = ka[d * step];
+  }
+  U128 sb{0, 0}, sa{0, 0};
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    add_wide(sb, mul_wide63(x[d], b[d]));
+    add_wide(sa, mul_wide63(x[d], a[d]));
+  }
+  const uint64_t o = (uint64_t)poly * m.out_poly_stride + (uint64_t)tower * m.n + i;
+  m.out_b[o] = mont_reduce(sb, q, qinv_neg);
+  m.out_a[o] = mont_reduce(sa, q, qinv_neg);
+}
+
+constexpr uint32_t kMacFixedDigits = 8;
+typedef void (*MacKernel)(MacLaunch, uint32_t);
+template <int... D>
+static MacKernel mac_kernel(uint32_t digits, std::integer_sequence<int, D...>) {
+  static constexpr MacKernel table[] = {mac_fused_fixed<D + 1>...};
+  return table[digits - 1];
+}
+
 hipError_t launch_mac_fused(const MacLaunch &m, bool per_term, uint32_t polys, hipStream_t stream) {
   dim3 grid((m.n + 255) / 256, (m.launch_towers ? m.launch_towers : m.towers) * polys);
-  if (per_term) hipLaunchKernelGGL(mac_fused_kernel<true>, grid, dim3(256), 0, stream, m, polys);
-  else hipLaunchKernelGGL(mac_fused_kernel<false>, grid, dim3(256), 0, stream, m, polys);
+  if (per_term) {
+    hipLaunchKernelGGL(mac_fused_kernel<true>, grid, dim3(256), 0, stream, m, polys);
+  } else if (m.digits_count >= 1 && m.digits_count <= kMacFixedDigits) {
+    hipLaunchKernelGGL(mac_kernel(m.digits_count, std::make_integer_sequence<int, kMacFixedDigits>()), grid, dim3(256),
+                       0, stream, m, polys);
+  } else {
+    hipLaunchKernelGGL(mac_fused_kernel<false>, grid, dim3(256), 0, stream, m, polys);
+  }
   return hipGetLastError();
 }
 

@@ -106,8 +106,11 @@ __global__ __launch_bounds__(256) void basis_switch_regs(BasisLaunch a) {
       lo += (uint64_t)y[j] * h[2 * j];
       hi += (uint64_t)y[j] * h[2 * j + 1];
     }
-    U128 sum{lo + (hi << 32), hi >> 32};
-    sum.hi += sum.lo < lo ? 1u : 0u;
+    // lo + 2^32 hi: the shifted column's low word is zero, so one add with
+    // carry on the middle words (hi >> 32 < 2^28 takes the carry)
+    uint32_t carry;
+    const uint32_t mid = __builtin_addc((uint32_t)(lo >> 32), (uint32_t)hi, 0u, &carry);
+    const U128 sum{((uint64_t)mid << 32) | (uint32_t)lo, (hi >> 32) + carry};
     const uint64_t r = mont_reduce(sum, dst[m].q, dst[m].qinv_neg);
     const uint32_t ot = m < a.skip_at ? m : m + a.skip_len;
     out[(uint64_t)ot * a.n + i] = r;

@@ -45,6 +45,10 @@ struct TowerMap {
   uint32_t towers_per_poly, prime_offset;
   uint32_t y_offset;  // launch index of blockIdx.y == 0 (chunked launches)
   uint32_t polys;     // towers / towers_per_poly
+  // ceil(2^32 / polys) when (y_offset + count) * polys <= 2^32 for the launch
+  // (then y / polys = y * polys_mul >> 32 exactly), else 0 (divide); set by
+  // the launcher
+  uint64_t polys_mul;
   // Optional epilogue of a forward transform (ModDown correction, affine maps
   // only): the last pass writes (sub - NTT(x)) * scale[k] mod q to
   // out[poly < split ? 0 : 1] + (poly mod split) * out_poly_stride + k * tower_stride

@@ -306,7 +306,7 @@ __device__ __forceinline__ void reg_stages_fwd(Val<A> (&v)[16], TwPtr tw, uint32
 __device__ __forceinline__ void tower_of(const TowerMap &m, uint32_t y, bool first_pass, uint64_t *&data,
                                          const uint64_t *&src, uint32_t &prime, uint32_t &k, uint32_t &poly) {
   y += m.y_offset;
-  k = y / m.polys;
+  k = m.polys_mul ? (uint32_t)(((uint64_t)y * m.polys_mul) >> 32) : y / m.polys;
   poly = y - k * m.polys;
   uint64_t off, src_off;
   if (m.jobs) {
@@ -767,8 +767,14 @@ static hipError_t launch_t(const TowerMap &map, const PrimeConst *primes, uint32
 
 // One launch of `count` launch indices in the arithmetic `arith` (mode and,
 // for kLazy, the forward headroom class).
-static hipError_t dispatch(const TowerMap &m, const PrimeConst *primes, uint32_t log_n, uint32_t count, bool inverse,
+static hipError_t dispatch(const TowerMap &map, const PrimeConst *primes, uint32_t log_n, uint32_t count, bool inverse,
                            int arith, hipStream_t stream) {
+  // tower_of's y / polys as a multiply: exact for y * polys <= 2^32 (the
+  // reciprocal's excess e = mul polys - 2^32 < polys keeps y e / 2^32 below
+  // the distance of y / polys to the next integer)
+  TowerMap m = map;
+  const uint64_t y_end = (uint64_t)m.y_offset + count;
+  m.polys_mul = m.polys && y_end * m.polys <= (uint64_t{1} << 32) ? ((uint64_t{1} << 32) + m.polys - 1) / m.polys : 0;
   const int mode = arith & kArithMask, h = (arith >> kArithShift) & 7;
   const bool row16 = (arith & kRow16Bit) != 0;
   const uint32_t pad = lds_pad_for((uint32_t)(arith >> kWgCapShift) & 15u);

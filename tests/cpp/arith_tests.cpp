@@ -252,6 +252,15 @@ int main() {
     const uint64_t r = mont_reduce(sum, m, qinv_neg);
     check(((u128)r << 64) % m == want && r < m, "Montgomery reduction of the columns");
   }
+  // the transform kernels' launch-index split y / polys as y * ceil(2^32 / polys) >> 32,
+  // used whenever (largest y + 1) * polys <= 2^32
+  for (int i = 0; i < 3000000; ++i) {
+    const uint64_t p = 1 + (i % 3 == 0 ? rng() % 64 : rng() % 70000);
+    const uint64_t y_end = (uint64_t{1} << 32) / p;  // the largest range the launcher allows
+    const uint64_t y = (i % 4 == 0) ? y_end - 1 - (rng() % 3 < y_end ? rng() % 3 : 0) : rng() % y_end;
+    const uint64_t mul = ((uint64_t{1} << 32) + p - 1) / p;
+    check(((y * mul) >> 32) == y / p, "launch index split by reciprocal");
+  }
   // 64 x 63-bit products of the Montgomery reduction and the key inner product
   for (int i = 0; i < 2000000; ++i) {
     const uint64_t a = (i % 5 == 0) ? ~0ULL - (rng() & 0xff) : rng();

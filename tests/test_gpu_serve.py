@@ -230,3 +230,57 @@ def test_resident_plans_count_and_evict_under_the_budget():
     assert st["completed"] == 15 and st["plans_evicted"] > 0, st
     assert st["peak_device_bytes"] <= 40 << 20 and st["resident_bytes"] > 0, st
     srv.close()
+
+
+def test_submit_and_wait_on_different_threads():
+    """One thread submits while two others wait on the tickets as they come
+    (the C calls release the GIL, so they overlap): every result equals the
+    oracle, and a ticket that has been collected is unknown afterwards."""
+    import queue
+    import threading
+
+    pools = {8192: workload.PrimePool(8192)}
+    hp = pools[8192].param_set(6, 2)
+    kb, ka = workload.make_key(hp, 77)
+    srv = serve.Server(streams=2, max_batch=8, hbm_budget_bytes=4 << 30)
+    pid = srv.add_params(hp, kb, ka)
+    rng = np.random.default_rng(91)
+    inputs = [workload.seeded_residues(8192, hp.q, rng) for _ in range(48)]
+    log_case("serve: submit and wait on different threads, 48 transforms")
+    work, results, errors = queue.Queue(), {}, []
+
+    def submitter():
+        try:
+            for i, x in enumerate(inputs):
+                work.put((i, srv.submit(pid, serve.FORWARD, len(hp.q), x)))
+        except Exception as e:  # noqa: BLE001 - reported by the main thread
+            errors.append(e)
+        finally:
+            for _ in range(2):
+                work.put(None)
+
+    def waiter():
+        try:
+            while True:
+                item = work.get()
+                if item is None:
+                    return
+                i, t = item
+                (out, _), _, _ = srv.wait(t, pid, serve.FORWARD, len(hp.q))
+                results[i] = (t, out)
+        except Exception as e:  # noqa: BLE001
+            errors.append(e)
+
+    threads = [threading.Thread(target=submitter)] + [threading.Thread(target=waiter) for _ in range(2)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=120)
+    assert not any(th.is_alive() for th in threads), "a submitting or waiting thread did not finish"
+    assert not errors, errors
+    assert sorted(results) == list(range(len(inputs)))
+    for i, x in enumerate(inputs):
+        assert np.array_equal(results[i][1], o.ntt_towers(x, 8192, hp.q, hp.q_psi)), f"request {i}"
+    with pytest.raises(Exception, match="unknown ticket"):
+        srv.wait(results[0][0], pid, serve.FORWARD, len(hp.q))
+    srv.close()

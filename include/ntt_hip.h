@@ -139,6 +139,27 @@ typedef struct ntt_hip_tower_job {
 } ntt_hip_tower_job;
 int ntt_hip_transform_jobs(ntt_hip_context *ctx, uint64_t *data, const uint64_t *src, const ntt_hip_tower_job *jobs,
                            size_t count, void *jobs_workspace, int inverse, void *stream);
+/* Transforms of towers of different ring dimensions in one launch per pass
+ * (no reference counterpart: the reference transforms one basis of one N per
+ * call). Job j transforms the N residues at jobs[j].data (its first pass
+ * reading jobs[j].src instead when non-NULL) with tower jobs[j].tower of context
+ * jobs[j].ctx; the contexts may differ in N (2^13 .. 2^16) and basis but
+ * share one device, and all their moduli are below 2^61 (towers below 2^30
+ * and wider ones run as two launches). The kernels are persistent: a fixed
+ * grid whose workgroups stride over (tower, block) work items. The table is
+ * checked on the host and staged into `jobs_workspace` (device,
+ * ntt_hip_transform_mixed_workspace_bytes(count) bytes) on `stream` (NULL:
+ * the first job's context stream), which must stay untouched until the
+ * launch has run. */
+typedef struct ntt_hip_mixed_job {
+  ntt_hip_context *ctx;
+  uint64_t *data;
+  const uint64_t *src;
+  uint32_t tower, reserved;
+} ntt_hip_mixed_job;
+int ntt_hip_transform_mixed_workspace_bytes(size_t count, size_t *bytes);
+int ntt_hip_transform_mixed(const ntt_hip_mixed_job *jobs, size_t count, void *jobs_workspace, int inverse,
+                            void *stream);
 /* Driver::pointwise (resident_adapter.inc:204-213): out[i] = out[i] * right[i] mod q */
 int ntt_hip_pointwise(ntt_hip_context *ctx, uint64_t *out, const uint64_t *right, size_t first, size_t towers,
                       void *stream);

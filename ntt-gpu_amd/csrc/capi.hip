@@ -777,6 +777,60 @@ int ntt_hip_transform_jobs(ntt_hip_context *ctx, uint64_t *data, const uint64_t 
   });
 }
 
+int ntt_hip_transform_mixed_workspace_bytes(size_t count, size_t *bytes) {
+  return guarded([&] {
+    require(bytes, "null argument");
+    require(count >= 1 && count <= (size_t{1} << 20), "1..2^20 mixed jobs per launch");
+    *bytes = count * sizeof(MixedJob);
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_transform_mixed(const ntt_hip_mixed_job *jobs, size_t count, void *jobs_workspace, int inverse,
+                            void *stream) {
+  return guarded([&] {
+    require(jobs && jobs_workspace, "null argument");
+    require(count >= 1 && count <= (size_t{1} << 20), "1..2^20 mixed jobs per launch");
+    const ntt_hip_context *first = jobs[0].ctx;
+    require(first != nullptr, "null context");
+    // the table, one run per arithmetic (32-bit towers, then wide-lazy ones)
+    std::vector<MixedJob> table;
+    table.reserve(count);
+    uint32_t runs[2][2] = {{0, 0}, {0, 0}};  // [narrow, lazy] x {jobs, blocks}
+    for (int pass = 0; pass < 2; ++pass) {
+      const int want = pass == 0 ? kNarrow : kLazy;
+      for (size_t j = 0; j < count; ++j) {
+        const ntt_hip_context *c = jobs[j].ctx;
+        require(c != nullptr && jobs[j].data != nullptr, "null context or buffer");
+        require(c->device == first->device, "mixed jobs on one device");
+        require(c->log_n >= 13 && c->log_n <= 16, "mixed launches take N = 2^13 .. 2^16");
+        require(jobs[j].tower < c->q.size(), "job tower outside its context basis");
+        const int a = c->arith & kArithMask;
+        require(a == kLazy || a == kNarrow, "mixed launches take moduli below 2^61");
+        if (a != want) continue;
+        MixedJob m{};
+        m.data = jobs[j].data;
+        m.src = jobs[j].src ? jobs[j].src : jobs[j].data;
+        m.prime = c->dprimes() + jobs[j].tower;
+        m.log_n = c->log_n;
+        m.first = runs[pass][1];
+        runs[pass][0] += 1;
+        runs[pass][1] += (uint32_t)(c->n / 4096);
+        table.push_back(m);
+      }
+    }
+    const hipStream_t s = first->pick(stream);
+    DeviceGuard g(first->device);
+    HIP_CHECK(hipMemcpyAsync(jobs_workspace, table.data(), table.size() * sizeof(MixedJob), hipMemcpyHostToDevice, s));
+    const MixedJob *dev = (const MixedJob *)jobs_workspace;
+    if (runs[0][0])
+      hip_ok(launch_ntt_mixed(dev, runs[0][0], runs[0][1], inverse != 0, kNarrow, s), "mixed ntt launch");
+    if (runs[1][0])
+      hip_ok(launch_ntt_mixed(dev + runs[0][0], runs[1][0], runs[1][1], inverse != 0, kLazy, s), "mixed ntt launch");
+    return NTT_HIP_OK;
+  });
+}
+
 int ntt_hip_transform(ntt_hip_context *ctx, uint64_t *data, const uint64_t *src, size_t first, size_t towers,
                       int inverse, void *stream) {
   return ntt_hip_transform_batch(ctx, data, src, 1, 0, 0, first, towers, inverse, stream);

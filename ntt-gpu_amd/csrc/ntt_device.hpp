@@ -96,6 +96,22 @@ inline int ntt_mode(int arith, bool row16, uint32_t wg_cap) {
 hipError_t launch_ntt(const TowerMap &map, const PrimeConst *primes, uint32_t log_n, uint32_t towers, bool inverse,
                       int arith, hipStream_t stream, uint32_t chunk = 0);
 
+// One tower of a launch that mixes ring dimensions N = 2^13 .. 2^16 (each job
+// its own length and prime constants, e.g. towers of contexts of different N).
+struct MixedJob {
+  uint64_t *data;           // N residues, transformed in place
+  const uint64_t *src;      // forward input (== data when in place)
+  const PrimeConst *prime;  // device constants of the tower's prime (its context's table)
+  uint32_t log_n;           // 13 .. 16
+  uint32_t first;           // 4096-residue blocks of the jobs before this one
+};
+// Forward or inverse transforms of `count` jobs (device table) holding
+// `blocks` 4096-residue blocks in all, in one persistent launch per pass:
+// a fixed grid whose workgroups stride over (job, block) work items. Wide-lazy
+// (kLazy) and 32-bit (kNarrow) towers, one arithmetic per launch.
+hipError_t launch_ntt_mixed(const MixedJob *jobs, uint32_t count, uint32_t blocks, bool inverse, int arith,
+                            hipStream_t stream);
+
 // Launch indices [y0, y0 + count) of a map over `polys` polynomials (both
 // passes): a range of the tower-major index space, e.g. the jobs of a few
 // towers for every polynomial.

@@ -334,18 +334,15 @@ __device__ __forceinline__ uint64_t *epi_dest(const TowerMap &m, uint32_t k, uin
 }
 
 // ---------------------------------------------------------------- pass A, R = 256
-template <bool Inverse, int A, int H = 0>
-__global__ __launch_bounds__(256) void ntt_pass_a256(TowerMap map, const PrimeConst *__restrict__ primes) {
-  __shared__ Val<A> lds[16 * kHalfRow];
-  uint64_t *data;
-  const uint64_t *src;
-  uint32_t prime, tk, tpoly;
-  tower_of(map, blockIdx.y, !Inverse, data, src, prime, tk, tpoly);
-  const PrimeConst pc = primes[prime];
+// Column block bx (16 columns of 256) of one tower: data in place, src the
+// forward input (== data unless the first pass reads elsewhere).
+template <bool Inverse, int A, int H>
+__device__ __forceinline__ void a256_body(uint64_t *data, const uint64_t *src, const PrimeConst &pc, uint32_t bx,
+                                          Val<A> *lds) {
   const TwPtr tw = tw_table(Inverse ? pc.inv : pc.fwd);
   const Lazy8 m = lazy8_of(pc.q, pc.nq, pc.nq2, pc.nq4);
   const int t = threadIdx.x, cl = t & 15, u = t >> 4;
-  const uint32_t c = blockIdx.x * 16 + cl;  // column
+  const uint32_t c = bx * 16 + cl;  // column
   Val<A> v[16];
   if constexpr (!Inverse) {
 #pragma unroll
@@ -377,6 +374,17 @@ __global__ __launch_bounds__(256) void ntt_pass_a256(TowerMap map, const PrimeCo
 #pragma unroll
     for (int k = 0; k < 16; ++k) data[(uint64_t)(u + 16 * k) * kRow + c] = v[k];
   }
+}
+
+template <bool Inverse, int A, int H = 0>
+__global__ __launch_bounds__(256) void ntt_pass_a256(TowerMap map, const PrimeConst *__restrict__ primes) {
+  __shared__ Val<A> lds[16 * kHalfRow];
+  uint64_t *data;
+  const uint64_t *src;
+  uint32_t prime, tk, tpoly;
+  tower_of(map, blockIdx.y, !Inverse, data, src, prime, tk, tpoly);
+  const PrimeConst pc = primes[prime];
+  a256_body<Inverse, A, H>(data, src, pc, blockIdx.x, lds);
 }
 
 // ---------------------------------------------------------------- pass B (rows of 256)
@@ -481,23 +489,11 @@ __device__ __forceinline__ void b8_fwd_stages(Val<A> (&v)[8], TwPtr tw, uint32_t
   }
 }
 
-template <int A, bool Epi, int H = 0>
-__global__ __launch_bounds__(256) void ntt_pass_b8_fwd(TowerMap map, const PrimeConst *__restrict__ primes, uint32_t R) {
-  __shared__ Val<A> lds[8 * kLdsRow];
-  uint64_t *data;
-  const uint64_t *src;
-  uint32_t prime, tk, tpoly;
-  tower_of(map, blockIdx.y, false, data, src, prime, tk, tpoly);
-  const PrimeConst pc = primes[prime];
-  const TwPtr tw = tw_table(pc.fwd);
-  const Lazy8 m = lazy8_of(pc.q, pc.nq, pc.nq2, pc.nq4);
-  const int t = threadIdx.x, u = t & 31, rl = t >> 5;
-  const uint32_t r = blockIdx.x * 8 + rl;
-  uint64_t *row = data + (uint64_t)r * kRow;
-  const uint32_t rf = R + r;
-  Val<A> *buf = lds + rl * kLdsRow;
-  Val<A> v[8];
-  const NegMultiples neg = neg_multiples(m);
+// Rows of one workgroup (8 of 256, row index r = 8 bx + thread / 32):
+// loads in P1 and all 8 stages; v ends in layout P3.
+template <int A, int H>
+__device__ __forceinline__ void b8_fwd_core(Val<A> (&v)[8], const uint64_t *row, TwPtr tw, uint32_t rf, int u,
+                                            const Lazy8 &m, const NegMultiples &neg, Val<A> *buf) {
 #pragma unroll
   for (int k = 0; k < 8; ++k) v[k] = (Val<A>)row[u + 32 * k];
   b8_fwd_stages<A, H, 0, 3>(v, tw, rf, u, m, neg);
@@ -516,6 +512,37 @@ __global__ __launch_bounds__(256) void ntt_pass_b8_fwd(TowerMap map, const Prime
   for (int k = 0; k < 8; ++k) v[k] = buf[pad16((u << 3) | k)];
   wave_sync8();
   b8_fwd_stages<A, H, 6, 8>(v, tw, rf, u, m, neg);
+}
+
+// Canonical output of b8_fwd_core, back to P1 for a coalesced store.
+template <int A, int H>
+__device__ __forceinline__ void b8_fwd_store(Val<A> (&v)[8], uint64_t *row, int u, const Lazy8 &m,
+                                             const NegMultiples &neg, const PrimeConst &pc, Val<A> *buf) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k) buf[pad16((u << 3) | k)] = canon_planned<A, H>(v[k], m, neg, pc);
+  wave_sync8();
+#pragma unroll
+  for (int k = 0; k < 8; ++k) row[u + 32 * k] = buf[pad16(u + 32 * k)];
+}
+
+template <int A, bool Epi, int H = 0>
+__global__ __launch_bounds__(256) void ntt_pass_b8_fwd(TowerMap map, const PrimeConst *__restrict__ primes, uint32_t R) {
+  __shared__ Val<A> lds[8 * kLdsRow];
+  uint64_t *data;
+  const uint64_t *src;
+  uint32_t prime, tk, tpoly;
+  tower_of(map, blockIdx.y, false, data, src, prime, tk, tpoly);
+  const PrimeConst pc = primes[prime];
+  const TwPtr tw = tw_table(pc.fwd);
+  const Lazy8 m = lazy8_of(pc.q, pc.nq, pc.nq2, pc.nq4);
+  const int t = threadIdx.x, u = t & 31, rl = t >> 5;
+  const uint32_t r = blockIdx.x * 8 + rl;
+  uint64_t *row = data + (uint64_t)r * kRow;
+  const uint32_t rf = R + r;
+  Val<A> *buf = lds + rl * kLdsRow;
+  Val<A> v[8];
+  const NegMultiples neg = neg_multiples(m);
+  b8_fwd_core<A, H>(v, row, tw, rf, u, m, neg, buf);
   if constexpr (Epi) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) buf[pad16((u << 3) | k)] = v[k];
@@ -541,29 +568,21 @@ __global__ __launch_bounds__(256) void ntt_pass_b8_fwd(TowerMap map, const Prime
       out[u + 32 * k] = shoup(d, c.x, c.y, pc.q);
     }
   } else {
-#pragma unroll
-    for (int k = 0; k < 8; ++k) buf[pad16((u << 3) | k)] = canon_planned<A, H>(v[k], m, neg, pc);
-    wave_sync8();
-#pragma unroll
-    for (int k = 0; k < 8; ++k) row[u + 32 * k] = buf[pad16(u + 32 * k)];
+    b8_fwd_store<A, H>(v, row, u, m, neg, pc, buf);
   }
 }
 
 // Inverse row pass, 8 residues per thread (first pass of an inverse
 // transform): load in P1, then P3 (distances 1, 2, 4), P2' with
 // j = (u >> 3) << 6 | k << 3 | (u & 7) (distances 8, 16, 32), P1 (64, 128).
+// One row per 32 threads (row 8 bx + thread / 32): reads src, writes data.
 template <int A>
-__global__ __launch_bounds__(256) void ntt_pass_b8_inv(TowerMap map, const PrimeConst *__restrict__ primes, uint32_t R) {
-  __shared__ Val<A> lds[8 * kLdsRow];
-  uint64_t *data;
-  const uint64_t *src;
-  uint32_t prime, tk, tpoly;
-  tower_of(map, blockIdx.y, true, data, src, prime, tk, tpoly);
-  const PrimeConst pc = primes[prime];
+__device__ __forceinline__ void b8_inv_body(uint64_t *data, const uint64_t *src, const PrimeConst &pc, uint32_t R,
+                                            uint32_t bx, Val<A> *lds) {
   const TwPtr tw = tw_table(pc.inv);
   const Lazy8 m = lazy8_of(pc.q, pc.nq, pc.nq2, pc.nq4);
   const int t = threadIdx.x, u = t & 31, rl = t >> 5;
-  const uint32_t r = blockIdx.x * 8 + rl;
+  const uint32_t r = bx * 8 + rl;
   uint64_t *row = data + (uint64_t)r * kRow;
   const uint64_t *src_row = src + (uint64_t)r * kRow;
   const uint32_t rf = R + r;
@@ -625,6 +644,17 @@ __global__ __launch_bounds__(256) void ntt_pass_b8_inv(TowerMap map, const Prime
   for (int k = 0; k < 8; ++k) row[u + 32 * k] = v[k];
 }
 
+template <int A>
+__global__ __launch_bounds__(256) void ntt_pass_b8_inv(TowerMap map, const PrimeConst *__restrict__ primes, uint32_t R) {
+  __shared__ Val<A> lds[8 * kLdsRow];
+  uint64_t *data;
+  const uint64_t *src;
+  uint32_t prime, tk, tpoly;
+  tower_of(map, blockIdx.y, true, data, src, prime, tk, tpoly);
+  const PrimeConst pc = primes[prime];
+  b8_inv_body<A>(data, src, pc, R, blockIdx.x, lds);
+}
+
 // ---------------------------------------------------------------- generic LDS kernel
 // Runs `stages` radix-2 stages on `sub` independent sub-transforms of length
 // L = 2^log_len held in LDS: element j of sub-transform b is at
@@ -633,20 +663,13 @@ __global__ __launch_bounds__(256) void ntt_pass_b8_inv(TowerMap map, const Prime
 // 4096/R columns per workgroup). Forward runs ml = 1..L/2, inverse the mirror
 // with n^-1 folded into the last stage when `last` is set.
 template <bool Inverse, int A>
-__global__ __launch_bounds__(256) void ntt_lds_generic(TowerMap map, const PrimeConst *__restrict__ primes,
-                                                       uint32_t log_len, uint32_t stride, uint32_t cols_per_block,
-                                                       uint32_t finish, uint32_t first_pass) {
-  __shared__ Val<A> lds[4096];
-  uint64_t *data;
-  const uint64_t *src;
-  uint32_t prime, tk, tpoly;
-  tower_of(map, blockIdx.y, first_pass != 0, data, src, prime, tk, tpoly);
-  const PrimeConst pc = primes[prime];
+__device__ __forceinline__ void lds_generic_stages(const uint64_t *src, const PrimeConst &pc, uint32_t log_len,
+                                                   uint32_t stride, uint32_t cols_per_block, uint32_t col0,
+                                                   bool finish, Val<A> *lds) {
   const TwPtr tw = tw_table(Inverse ? pc.inv : pc.fwd);
   const Lazy8 m = lazy8_of(pc.q, pc.nq, pc.nq2, pc.nq4);
   const uint32_t L = 1u << log_len;
   const uint32_t total = L * cols_per_block;  // <= 4096
-  const uint32_t col0 = blockIdx.x * cols_per_block;
   // LDS layout: lds[col * L + j]; global: data[(col0 + col) + j * stride] (stride > 1)
   // or data[j] (stride == 1, cols_per_block == 1).
   for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
@@ -675,6 +698,23 @@ __global__ __launch_bounds__(256) void ntt_lds_generic(TowerMap map, const Prime
     }
     __syncthreads();
   }
+}
+
+template <bool Inverse, int A>
+__global__ __launch_bounds__(256) void ntt_lds_generic(TowerMap map, const PrimeConst *__restrict__ primes,
+                                                       uint32_t log_len, uint32_t stride, uint32_t cols_per_block,
+                                                       uint32_t finish, uint32_t first_pass) {
+  __shared__ Val<A> lds[4096];
+  uint64_t *data;
+  const uint64_t *src;
+  uint32_t prime, tk, tpoly;
+  tower_of(map, blockIdx.y, first_pass != 0, data, src, prime, tk, tpoly);
+  const PrimeConst pc = primes[prime];
+  const Lazy8 m = lazy8_of(pc.q, pc.nq, pc.nq2, pc.nq4);
+  const uint32_t L = 1u << log_len;
+  const uint32_t total = L * cols_per_block;
+  const uint32_t col0 = blockIdx.x * cols_per_block;
+  lds_generic_stages<Inverse, A>(src, pc, log_len, stride, cols_per_block, col0, finish != 0, lds);
   const bool epilogue = !Inverse && finish && map.epi.sub;
   const uint64_t *sub = nullptr;
   uint64_t *out = data;
@@ -695,6 +735,112 @@ __global__ __launch_bounds__(256) void ntt_lds_generic(TowerMap map, const Prime
       if (acc) x = add_mod(out[at], x, pc.q);
     }
     out[at] = x;
+  }
+}
+
+// ---------------------------------------------------------------- mixed ring dimensions
+// Work items are 4096-residue blocks (pass A: 16 columns of 256 at N = 2^16,
+// 4096/R columns of R below) or half blocks (pass B: 8 rows of 256). The job
+// of block b is found by a binary search over the jobs' prefix counts, read
+// through the scalar cache (the item index is uniform in a workgroup).
+__device__ __forceinline__ uint32_t mixed_job_of(const MixedJob *jobs, uint32_t count, uint32_t block) {
+  typedef const __attribute__((address_space(4))) MixedJob *CJob;
+  const CJob cj = (CJob)(const void *)jobs;
+  uint32_t lo = 0, hi = count;  // jobs[lo].first <= block < (lo + 1 < count ? jobs[lo + 1].first : blocks)
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (cj[mid].first <= block) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <bool Inverse, int A>
+__global__ __launch_bounds__(256) void ntt_mixed_pass_a(const MixedJob *jobs, uint32_t count, uint32_t items) {
+  __shared__ Val<A> lds[4096];  // the generic column pass's block; the 2^16 one needs 16 * kHalfRow
+  typedef const __attribute__((address_space(4))) MixedJob *CJob;
+  const CJob cj = (CJob)(const void *)jobs;
+  for (uint32_t it = blockIdx.x; it < items; it += gridDim.x) {
+    const uint32_t j = mixed_job_of(jobs, count, it);
+    uint64_t *data = cj[j].data;
+    const uint64_t *src = Inverse ? data : cj[j].src;  // forward: first pass
+    const uint32_t log_n = cj[j].log_n, bx = it - cj[j].first;
+    const PrimeConst pc = *cj[j].prime;
+    if (log_n == 16) {
+      a256_body<Inverse, A, 0>(data, src, pc, bx, lds);
+    } else {
+      const uint32_t log_r = log_n - 8, cols = 4096u >> log_r, col0 = bx * cols, L = 1u << log_r;
+      lds_generic_stages<Inverse, A>(src, pc, log_r, kRow, cols, col0, Inverse, lds);
+      for (uint32_t e = threadIdx.x; e < 4096u; e += blockDim.x) {
+        const uint32_t col = e % cols, jr = e / cols;
+        data[(uint64_t)jr * kRow + col0 + col] = lds[col * L + jr];
+      }
+    }
+    __syncthreads();  // the next item reuses the LDS
+  }
+}
+
+template <bool Inverse, int A>
+__global__ __launch_bounds__(256) void ntt_mixed_pass_b(const MixedJob *jobs, uint32_t count, uint32_t items) {
+  __shared__ Val<A> lds[8 * kLdsRow];
+  typedef const __attribute__((address_space(4))) MixedJob *CJob;
+  const CJob cj = (CJob)(const void *)jobs;
+  for (uint32_t it = blockIdx.x; it < items; it += gridDim.x) {
+    const uint32_t j = mixed_job_of(jobs, count, it >> 1);  // two row blocks per 4096-residue block
+    uint64_t *data = cj[j].data;
+    const uint32_t R = 1u << (cj[j].log_n - 8), bx = it - 2 * cj[j].first;
+    const PrimeConst pc = *cj[j].prime;
+    if constexpr (Inverse) {
+      b8_inv_body<A>(data, cj[j].src, pc, R, bx, lds);  // inverse: first pass
+    } else {
+      const TwPtr tw = tw_table(pc.fwd);
+      const Lazy8 m = lazy8_of(pc.q, pc.nq, pc.nq2, pc.nq4);
+      const NegMultiples neg = neg_multiples(m);
+      const int u = threadIdx.x & 31, rl = threadIdx.x >> 5;
+      const uint32_t r = bx * 8 + rl;
+      uint64_t *row = data + (uint64_t)r * kRow;
+      Val<A> *buf = lds + rl * kLdsRow;
+      Val<A> v[8];
+      b8_fwd_core<A, 0>(v, row, tw, R + r, u, m, neg, buf);
+      b8_fwd_store<A, 0>(v, row, u, m, neg, pc, buf);
+    }
+    wave_sync8();  // a wave's exchange rows are its own; order them against its next item
+  }
+}
+
+template <int A>
+static hipError_t launch_mixed_t(const MixedJob *jobs, uint32_t count, uint32_t blocks, bool inverse,
+                                 hipStream_t stream) {
+  int device = 0, cus = 0;
+  hipError_t e = hipGetDevice(&device);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  if (e != hipSuccess) return e;
+  // persistent grid: a few workgroups per CU (both passes hold at least 5 per CU)
+  const uint32_t slots = (uint32_t)(cus > 0 ? cus : 1) * 5u;
+  auto pass_a = [&]() {
+    hipLaunchKernelGGL((ntt_mixed_pass_a<true, A>), dim3(blocks < slots ? blocks : slots), dim3(256), 0, stream, jobs,
+                       count, blocks);
+  };
+  if (!inverse) {
+    hipLaunchKernelGGL((ntt_mixed_pass_a<false, A>), dim3(blocks < slots ? blocks : slots), dim3(256), 0, stream,
+                       jobs, count, blocks);
+    hipLaunchKernelGGL((ntt_mixed_pass_b<false, A>), dim3(2 * blocks < slots ? 2 * blocks : slots), dim3(256), 0,
+                       stream, jobs, count, 2 * blocks);
+  } else {
+    hipLaunchKernelGGL((ntt_mixed_pass_b<true, A>), dim3(2 * blocks < slots ? 2 * blocks : slots), dim3(256), 0,
+                       stream, jobs, count, 2 * blocks);
+    pass_a();
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_ntt_mixed(const MixedJob *jobs, uint32_t count, uint32_t blocks, bool inverse, int arith,
+                            hipStream_t stream) {
+  if (count == 0 || blocks == 0) return hipSuccess;
+  switch (arith & kArithMask) {
+    case kLazy: return launch_mixed_t<kLazy>(jobs, count, blocks, inverse, stream);
+    case kNarrow: return launch_mixed_t<kNarrow>(jobs, count, blocks, inverse, stream);
+    default: return hipErrorInvalidValue;
   }
 }
 

@@ -74,6 +74,8 @@ SIGNATURES = {
     "ntt_hip_keyswitch_autotune": [VP, SZ, ctypes.c_char_p, SZ],
     "ntt_hip_keyswitch_device_bytes": [VP, ctypes.POINTER(SZ)],
     "ntt_hip_transform_jobs": [VP, VP, VP, VP, SZ, VP, INT, VP],
+    "ntt_hip_transform_mixed_workspace_bytes": [SZ, ctypes.POINTER(SZ)],
+    "ntt_hip_transform_mixed": [VP, SZ, VP, INT, VP],
     "ntt_hip_get_ntt_shape": [VP, ctypes.POINTER(INT), ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)],
     "ntt_hip_keyswitch_capture": [VP, VP, SZ, VP, VP, VP, VP, ctypes.POINTER(VP)],
     "ntt_hip_graph_launch": [VP, VP],
@@ -440,6 +442,29 @@ class KeySwitch:
             self.close()
         except Exception:
             pass
+
+
+class MixedJob(ctypes.Structure):
+    """ntt_hip_mixed_job: one tower of a mixed-ring-dimension transform."""
+    _fields_ = [("ctx", VP), ("data", VP), ("src", VP), ("tower", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+def transform_mixed(jobs, inverse, stream=None, workspace=None):
+    """Transforms of towers of different ring dimensions in one launch per
+    pass. jobs: (Context, data_ptr, tower) or (Context, data_ptr, tower,
+    src_ptr) tuples. Returns the device workspace holding the job table; it
+    must outlive the launch (pass it back in as `workspace` to reuse it)."""
+    arr = (MixedJob * len(jobs))()
+    for i, job in enumerate(jobs):
+        ctx, data, tower = job[0], job[1], job[2]
+        arr[i] = MixedJob(ctx.handle, data, job[3] if len(job) > 3 else None, tower, 0)
+    need = SZ(0)
+    check(lib().ntt_hip_transform_mixed_workspace_bytes(len(jobs), ctypes.byref(need)), "transform_mixed_workspace")
+    if workspace is None or workspace.words * 8 < need.value:
+        workspace = DeviceBuffer((need.value + 7) // 8)
+    check(lib().ntt_hip_transform_mixed(ctypes.cast(arr, VP), len(jobs), workspace.ptr, int(inverse), stream),
+          "transform_mixed")
+    return workspace
 
 
 def autotune(ctx, towers):

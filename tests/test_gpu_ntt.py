@@ -256,3 +256,92 @@ def test_forward_headroom_classes(bits):
         got = _gpu_round(ctx, x, 0, 3, False)
         assert np.array_equal(got, o.ntt_towers(x, N16, qs, rs)), f"{bits}-bit forward ({name})"
         assert np.array_equal(_gpu_round(ctx, got, 0, 3, True), x), f"{bits}-bit round trip ({name})"
+
+
+def _mixed_contexts():
+    """Contexts of every two-pass ring dimension, wide-lazy (60- and 50-bit
+    top-of-range primes) and 32-bit (30-bit primes)."""
+    ctxs = []
+    for n in (8192, 16384, 32768, 65536):
+        qs = [params.last_prime(60, 2 * n), params.last_prime(50, 2 * n)]
+        ctxs.append((n, qs, [params.min_root(q, n) for q in qs]))
+    for n in (8192, 65536):
+        qs = [params.last_prime(30, 2 * n), params.previous_prime(params.last_prime(30, 2 * n), 2 * n)]
+        ctxs.append((n, qs, [params.min_root(q, n) for q in qs]))
+    return [(n, qs, rs, hip.Context(n, qs, rs)) for n, qs, rs in ctxs]
+
+
+def test_mixed_ring_dimensions_in_one_launch():
+    """Towers of N = 2^13 .. 2^16 from six contexts (wide-lazy and 32-bit) in
+    one mixed launch per pass, job order shuffled, some read out of place:
+    every tower equals the oracle's transform of its own N, and the inverse
+    launch restores the inputs."""
+    ctxs = _mixed_contexts()
+    rng = np.random.default_rng(2024)
+    jobs, cases = [], []
+    for ci, (n, qs, rs, ctx) in enumerate(ctxs):
+        for t in range(len(qs)):
+            for rep in range(2):
+                x = uniform(n, [qs[t]], 100 * ci + 10 * t + rep)
+                if rep == 1:
+                    x[:] = qs[t] - 1  # every residue at its largest value
+                buf = hip.DeviceBuffer(n).upload(x)
+                src = hip.DeviceBuffer(n).upload(x) if (ci + t + rep) % 3 == 0 else None
+                cases.append((n, qs[t], rs[t], x, buf, src))
+                jobs.append((ctx, buf.ptr, t, src.ptr if src else None))
+    order = rng.permutation(len(jobs))
+    jobs = [jobs[i] for i in order]
+    cases = [cases[i] for i in order]
+    log_case(f"mixed launch: {len(jobs)} towers over N = 2^13..2^16, 60/50/30-bit")
+    st = hip.Stream(0)
+    ws = hip.transform_mixed(jobs, inverse=False, stream=st.handle)
+    st.synchronize()
+    for n, q, r, x, buf, src in cases:
+        assert np.array_equal(buf.download(), o.ntt_towers(x, n, [q], [r])), f"forward N={n} q={q}"
+    hip.transform_mixed([(j[0], j[1], j[2]) for j in jobs], inverse=True, stream=st.handle, workspace=ws)
+    st.synchronize()
+    for n, q, r, x, buf, src in cases:
+        assert np.array_equal(buf.download(), x), f"round trip N={n} q={q}"
+
+
+def test_mixed_launch_strides_over_more_work_than_its_grid():
+    """300 towers (2^16 and 2^14, interleaved): the persistent grid holds far
+    fewer workgroups than the 4000+ work items, so every workgroup strides."""
+    ctxs = _mixed_contexts()
+    (n1, q1, r1, c1), (n2, q2, r2, c2) = ctxs[3], ctxs[1]
+    towers = []
+    for k in range(300):
+        n, qs, rs, ctx = (n1, q1, r1, c1) if k % 3 else (n2, q2, r2, c2)
+        t = k % 2
+        towers.append((n, qs[t], rs[t], ctx, t, uniform(n, [qs[t]], 7000 + k)))
+    big = hip.DeviceBuffer(sum(n for n, *_ in towers))
+    offs, off = [], 0
+    for n, q, r, ctx, t, x in towers:
+        big.upload(x, offset=off)
+        offs.append(off)
+        off += n
+    log_case("mixed launch: 300 towers, persistent striding")
+    st = hip.Stream(0)
+    ws = hip.transform_mixed([(ctx, big.at(o_), t) for (n, q, r, ctx, t, x), o_ in zip(towers, offs)],
+                             inverse=False, stream=st.handle)
+    st.synchronize()  # the job table in `ws` stays allocated until the launch has run
+    del ws
+    got = big.download()
+    for (n, q, r, ctx, t, x), o_ in zip(towers, offs):
+        assert np.array_equal(got[o_:o_ + n], o.ntt_towers(x, n, [q], [r])), f"tower at {o_} (N={n})"
+
+
+def test_mixed_launch_rejections():
+    small_q = [params.last_prime(50, 2 * 4096)]
+    small = hip.Context(4096, small_q, [params.min_root(small_q[0], 4096)])
+    wide_q = params.find_primes(8192, 1, 62)
+    wide = hip.Context(8192, wide_q, [params.find_psi(wide_q[0], 8192)])
+    ok_q = [params.last_prime(50, 2 * 8192)]
+    ok = hip.Context(8192, ok_q, [params.min_root(ok_q[0], 8192)])
+    buf = hip.DeviceBuffer(8192)
+    with pytest.raises(hip.ContractError, match="2\\^13"):
+        hip.transform_mixed([(small, buf.ptr, 0)], inverse=False)
+    with pytest.raises(hip.ContractError, match="below 2\\^61"):
+        hip.transform_mixed([(ok, buf.ptr, 0), (wide, buf.ptr, 0)], inverse=False)
+    with pytest.raises(hip.ContractError, match="outside"):
+        hip.transform_mixed([(ok, buf.ptr, 1)], inverse=False)

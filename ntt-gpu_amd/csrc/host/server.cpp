@@ -99,7 +99,7 @@ struct Slot {
   size_t pinned_bytes = 0;
   std::vector<std::shared_ptr<Request>> batch;
   std::vector<size_t> out_at, out_words, out2_at;  // per request, words into the staged outputs
-  std::shared_ptr<Resident> holder;                // the plan or pool the batch runs on
+  std::vector<std::shared_ptr<Resident>> holders;  // the plan or pools the batch runs on
   bool busy = false;
 };
 
@@ -290,6 +290,15 @@ struct Server::Impl {
   }
 
   static bool is_transform(Op op) { return op == Op::Forward || op == Op::Inverse; }
+  // Transforms of parameter sets whose ring dimension is 2^13 .. 2^16 and
+  // whose moduli are below 2^61 share launches across ring dimensions
+  // (ntt_hip_transform_mixed); others group by ring dimension.
+  static bool mixable(const ParamSet &p) {
+    if (p.n < 8192 || p.n > 65536) return false;
+    for (uint64_t q : p.q)
+      if (q >= (uint64_t{1} << 61)) return false;
+    return true;
+  }
 
   // Take the head request's group off the queue (at most max_batch).
   std::vector<std::shared_ptr<Request>> take_group() {
@@ -297,10 +306,13 @@ struct Server::Impl {
     std::vector<std::shared_ptr<Request>> batch;
     if (pending.empty()) return batch;
     const auto head = pending.front();
+    const bool head_mixable = is_transform(head->op) && mixable(*params.at(head->params));
     for (auto it = pending.begin(); it != pending.end() && batch.size() < opt.max_batch;) {
       const Request &r = **it;
-      const bool same = r.op == head->op && r.device == head->device &&
-                        (is_transform(head->op) ? r.n == head->n : r.params == head->params && r.level == head->level);
+      const bool same =
+          r.op == head->op && r.device == head->device &&
+          (is_transform(head->op) ? r.n == head->n || (head_mixable && mixable(*params.at(r.params)))
+                                  : r.params == head->params && r.level == head->level);
       if (same) {
         batch.push_back(*it);
         it = pending.erase(it);
@@ -319,32 +331,55 @@ struct Server::Impl {
     const auto hset = param_set(h.params);
     const bool pooled = is_transform(h.op);
     std::shared_ptr<LevelPlan> lp;
-    std::shared_ptr<Pool> pl;
+    // the pool of every ring dimension in the batch, each held (in flight) from
+    // the moment it is fetched, so fetching or admitting another cannot evict it
+    std::map<uint64_t, std::shared_ptr<Pool>> pls;
+    auto unhold = [&]() {
+      for (auto &kv : pls)
+        if (kv.second) --kv.second->inflight;
+      pls.clear();
+    };
     try {
-      if (pooled) pl = pool(h.n, slot);
-      else lp = plan(h.params, h.level, slot);
+      if (pooled) {
+        for (const auto &r : batch)
+          if (!pls.count(r->n)) {
+            auto pl = pool(r->n, slot);
+            if (pl) ++pl->inflight;
+            pls[r->n] = pl;
+          }
+      } else {
+        lp = plan(h.params, h.level, slot);
+      }
     } catch (const std::exception &e) {
+      unhold();
       fail(batch, e.what());
       return true;
     }
-    if (!pl && !lp) {  // no room for the plan itself
-      return wait_or_fail(batch, "the plan does not fit the device budget");
-    }
     if (pooled) {
-      // a busy pool built before newer parameter sets may lack their primes:
-      // those requests wait for its rebuild
+      // a busy pool built before newer parameter sets may lack their primes,
+      // and a pool may not fit yet: those requests wait
       std::vector<std::shared_ptr<Request>> covered, later;
       for (const auto &r : batch) {
+        const auto &pl = pls[r->n];
         const auto ps = param_set(r->params);
-        bool all = true;
+        bool all = pl != nullptr;
         for (size_t t = 0; t < r->level && all; ++t) all = pl->tower_of.count(ps->q[t]) != 0;
         (all ? covered : later).push_back(r);
       }
+      if (covered.empty()) {
+        const bool no_room = pls.size() == 1 && !pls.begin()->second;
+        unhold();
+        if (no_room) return wait_or_fail(batch, "the plan does not fit the device budget");
+        requeue(later);
+        return false;
+      }
       if (!later.empty()) requeue(later);
-      if (covered.empty()) return false;
       batch = std::move(covered);
+    } else if (!lp) {  // no room for the plan itself
+      return wait_or_fail(batch, "the plan does not fit the device budget");
     }
-    Resident *res = pooled ? static_cast<Resident *>(pl.get()) : lp.get();
+    // pools are held in flight, so make_room cannot evict them; a plan is kept explicitly
+    Resident *res = pooled ? nullptr : lp.get();
     // per-request input/output extents (words)
     std::vector<std::shared_ptr<const ParamSet>> sets;
     for (const auto &r : batch) sets.push_back(param_set(r->params));
@@ -357,7 +392,9 @@ struct Server::Impl {
       if (pooled) {
         size_t towers = 0;
         for (size_t i = 0; i < b; ++i) towers += batch[i]->level;
-        bytes += (towers * sizeof(ntt_hip_tower_job) + 255) / 256 * 256;
+        size_t table = 0;
+        if (towers) ok(ntt_hip_transform_mixed_workspace_bytes(towers, &table), "mixed workspace");
+        bytes += (std::max(towers * sizeof(ntt_hip_tower_job), table) + 255) / 256 * 256;
       } else {
         bytes += workspace_bytes(*lp, h.op, b);
       }
@@ -366,11 +403,31 @@ struct Server::Impl {
     // admission by the whole footprint: shrink the batch, then make room
     size_t b = batch.size();
     while (b > 1 && !fits(footprint(b), &slot)) --b;
-    if (!make_room(footprint(b), &slot, res)) return wait_or_fail(batch, "request does not fit the device budget");
+    if (!make_room(footprint(b), &slot, res)) {
+      unhold();
+      return wait_or_fail(batch, "request does not fit the device budget");
+    }
     if (b < batch.size()) {
       requeue(std::vector<std::shared_ptr<Request>>(batch.begin() + b, batch.end()));
       batch.resize(b);
       sets.resize(b);
+    }
+    // the residents the admitted batch runs on (pools no longer needed are released)
+    std::vector<std::shared_ptr<Resident>> holders;
+    if (pooled) {
+      for (auto it = pls.begin(); it != pls.end();) {
+        bool used = false;
+        for (const auto &r : batch) used |= r->n == it->first;
+        if (used) {
+          holders.push_back(it->second);
+          ++it;
+        } else {
+          if (it->second) --it->second->inflight;
+          it = pls.erase(it);
+        }
+      }
+    } else {
+      holders.push_back(lp);
     }
     try {
       size_t iw = 0, ow = 0, ow2 = 0;
@@ -402,19 +459,33 @@ struct Server::Impl {
           ok(ntt_hip_memcpy(opt.device, d_in + in_at[i], batch[i]->d_in, in_w(i) * 8, NTT_HIP_D2D, slot.stream),
              "gather");
       }
-      if (pooled) {
+      if (pooled && pls.size() == 1) {
         // one job per tower of every request, each with its prime's pool tower
+        const auto &pl = pls.begin()->second;
         std::vector<ntt_hip_tower_job> jobs;
         for (size_t i = 0; i < b; ++i)
           for (size_t t = 0; t < batch[i]->level; ++t) {
             const auto f = pl->tower_of.find(sets[i]->q[t]);
             if (f == pl->tower_of.end()) throw std::runtime_error("prime missing from the ring's pool");
-            const uint64_t at = in_at[i] + t * h.n;
+            const uint64_t at = in_at[i] + t * batch[i]->n;  // the head may have been requeued
             jobs.push_back({at, at, f->second, 0});
           }
         ok(ntt_hip_transform_jobs(pl->ctx, d_out, d_in, jobs.data(), jobs.size(), d_ws, h.op == Op::Inverse,
                                   slot.stream),
            "transform");
+      } else if (pooled) {
+        // several ring dimensions: one mixed launch per pass over the pools
+        std::vector<ntt_hip_mixed_job> jobs;
+        for (size_t i = 0; i < b; ++i) {
+          const auto &pl = pls[batch[i]->n];
+          for (size_t t = 0; t < batch[i]->level; ++t) {
+            const auto f = pl->tower_of.find(sets[i]->q[t]);
+            if (f == pl->tower_of.end()) throw std::runtime_error("prime missing from the ring's pool");
+            const uint64_t at = in_at[i] + t * batch[i]->n;
+            jobs.push_back({pl->ctx, d_out + at, d_in + at, f->second, 0});
+          }
+        }
+        ok(ntt_hip_transform_mixed(jobs.data(), jobs.size(), d_ws, h.op == Op::Inverse, slot.stream), "transform");
       } else {
         const size_t L = h.level;
         switch (h.op) {
@@ -450,9 +521,11 @@ struct Server::Impl {
       slot.out_at = out_at;
       slot.out_words = out_len;
       slot.out2_at.assign(ow2 ? out2_at.begin() : out2_at.end(), out2_at.end());
-      slot.holder = pooled ? std::static_pointer_cast<Resident>(pl) : std::static_pointer_cast<Resident>(lp);
-      ++slot.holder->inflight;
-      slot.holder->last_use = ++tick;
+      slot.holders = holders;
+      for (const auto &r : slot.holders) {
+        if (!pooled) ++r->inflight;  // pools were held since they were fetched
+        r->last_use = ++tick;
+      }
       slot.busy = true;
       publish_memory();
       std::lock_guard<std::mutex> lock(mu);
@@ -462,6 +535,7 @@ struct Server::Impl {
       stats.max_in_flight = std::max(stats.max_in_flight, busy);
     } catch (const std::exception &e) {
       (void)ntt_hip_stream_synchronize(opt.device, slot.stream);
+      unhold();
       fail(batch, e.what());
     }
     return true;
@@ -504,8 +578,8 @@ struct Server::Impl {
         ++stats.completed;
       }
     }
-    --slot.holder->inflight;
-    slot.holder.reset();
+    for (const auto &r : slot.holders) --r->inflight;
+    slot.holders.clear();
     slot.batch.clear();
     slot.busy = false;
     cv_done.notify_all();

@@ -284,3 +284,35 @@ def test_submit_and_wait_on_different_threads():
     with pytest.raises(Exception, match="unknown ticket"):
         srv.wait(results[0][0], pid, serve.FORWARD, len(hp.q))
     srv.close()
+
+
+def test_transforms_of_different_ring_dimensions_share_launches():
+    """NTT and INTT requests of N = 2^13 .. 2^16 (three per ring dimension)
+    submitted back to back on one slot: a batch of four or more must span ring
+    dimensions (one mixed launch per pass); every response equals the oracle
+    and the same request served alone."""
+    sizes = (8192, 16384, 32768, 65536)
+    pools = {n: workload.PrimePool(n) for n in sizes}
+    srv = serve.Server(streams=1, max_batch=16, hbm_budget_bytes=8 << 30)
+    alone = serve.Server(streams=1, max_batch=1, hbm_budget_bytes=8 << 30)
+    rng = np.random.default_rng(64)
+    reqs = []
+    for k in range(12):
+        n = sizes[k % 4]
+        hp = pools[n].param_set(3 + k % 3, 2)
+        kb, ka = workload.make_key(hp, 500 + k)
+        op = serve.FORWARD if k % 2 == 0 else serve.INVERSE
+        reqs.append((hp, op, srv.add_params(hp, kb, ka), alone.add_params(hp, kb, ka),
+                     workload.seeded_residues(n, hp.q, rng)))
+    log_case("serve: transforms over four ring dimensions")
+    tickets = [srv.submit(pid, op, len(hp.q), x) for hp, op, pid, _, x in reqs]
+    got = [srv.wait(t, pid, op, len(hp.q)) for t, (hp, op, pid, _, x) in zip(tickets, reqs)]
+    sizes_seen = [bs for _, _, bs in got]
+    assert max(sizes_seen) >= 4, f"no batch spans ring dimensions: batch sizes {sizes_seen}"
+    for (hp, op, pid, apid, x), ((out, _), _, _) in zip(reqs, got):
+        want = o.ntt_towers(x, hp.n, hp.q, hp.q_psi, inverse=op == serve.INVERSE)
+        assert np.array_equal(out, want), f"N={hp.n} L={len(hp.q)} op={op}"
+        (a, _), _, _ = alone.wait(alone.submit(apid, op, len(hp.q), x), apid, op, len(hp.q))
+        assert np.array_equal(a, out)
+    srv.close()
+    alone.close()

@@ -462,6 +462,53 @@ CLI = os.path.join(ROOT, "ntt-gpu_amd", "bin", "ntt_hip_cli")
 CONVERSION_BENCH = os.path.join(ROOT, "ntt-gpu_amd", "bin", "session_conversion_bench")
 
 
+def mixed_ring_dimensions(towers_per_n=64, reps=10):
+    """Forward transforms of towers of four ring dimensions (N = 2^13 .. 2^16,
+    `towers_per_n` towers each, 60/50-bit primes): one mixed launch per pass
+    (ntt_hip_transform_mixed) against one launch per ring dimension, on the
+    same stream and data; the two results are compared bit for bit."""
+    sizes = (8192, 16384, 32768, 65536)
+    st = hip.Stream(0)
+    runs = []
+    for n in sizes:
+        qs = [params.last_prime(60, 2 * n), params.last_prime(50, 2 * n)]
+        ctx = hip.Context(n, qs, [params.min_root(q, n) for q in qs])
+        x = np.concatenate([seeded(n, qs, 40 + n + t) for t in range(towers_per_n // 2)])
+        a, b = hip.DeviceBuffer(x.size).upload(x), hip.DeviceBuffer(x.size).upload(x)
+        runs.append((n, ctx, a, b))
+    jobs = [(ctx, a.at(t * n), t % 2) for n, ctx, a, b in runs for t in range(towers_per_n)]
+    ws = hip.transform_mixed(jobs, inverse=False, stream=st.handle)  # warm
+    for n, ctx, a, b in runs:
+        ctx.transform_batch(b.ptr, towers_per_n // 2, 2 * n, 0, 2, False, stream=st.handle)
+    st.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+
+    def timed(fn):
+        e0.record(st.handle)
+        for _ in range(reps):
+            fn()
+        e1.record(st.handle)
+        e1.synchronize()
+        return e0.elapsed_ms(e1) / reps
+
+    mixed_ms = timed(lambda: hip.transform_mixed(jobs, inverse=False, stream=st.handle, workspace=ws))
+    split_ms = timed(lambda: [ctx.transform_batch(b.ptr, towers_per_n // 2, 2 * n, 0, 2, False, stream=st.handle)
+                              for n, ctx, a, b in runs])
+    equal = all(np.array_equal(a.download(), b.download()) for n, ctx, a, b in runs)
+    total = sum(n for n in sizes) * towers_per_n
+    out = {"towers_per_ring_dimension": towers_per_n, "ring_dimensions": list(sizes),
+           "mixed_launch_ms": round(mixed_ms, 4), "launch_per_ring_dimension_ms": round(split_ms, 4),
+           "mixed_residues_per_s": round(total / (mixed_ms * 1e-3), 1),
+           "outputs_equal": bool(equal),
+           "note": "forward transforms; both forms transform the same towers repeatedly in place (results compared "
+                   "after the same number of transforms)"}
+    for n, ctx, a, b in runs:
+        a.free()
+        b.free()
+        ctx.close()
+    return out
+
+
 def reference_workloads():
     """The reference's own benchmark commands on its largest workload, run
     through this build's drop-in tools: ntt_hip_cli events-jsonl (resident,
@@ -695,6 +742,9 @@ def main():
     if valu:
         progress(f"valu roofline: {valu['achieved']} of {valu['peak']} T butterfly/s")
     refw = reference_workloads() if not args.no_reference_workloads else None
+    mixed = mixed_ring_dimensions()
+    progress(f"mixed ring dimensions: {mixed['mixed_launch_ms']} ms mixed, "
+             f"{mixed['launch_per_ring_dimension_ms']} ms per-N, equal={mixed['outputs_equal']}")
     served_ks = None
     if args.serve_keyswitch_requests:
         served_ks = served_keyswitch(hp, kb, ka, d_in, d_b, d_a, B, args.serve_keyswitch_requests,
@@ -743,6 +793,7 @@ def main():
         "served_stream": served,
         "served_keyswitch": served_ks,
         "reference_workloads": refw,
+        "mixed_ring_dimensions": mixed,
         "device": hip.device_name(0),
     }
     print(json.dumps(out), flush=True)

@@ -270,22 +270,29 @@ struct Server::Impl {
     return bytes;
   }
 
+  // Arenas grow geometrically (x1.5, the device part only while the extra
+  // fits the budget): each regrowth frees and allocates, which waits for the
+  // device, so a stream whose batches grow reallocates a few times rather
+  // than once per new batch size.
   void ensure_arena(Slot &s, size_t device_bytes, size_t pinned_bytes) {
     if (s.device_bytes < device_bytes) {
+      size_t want = std::max(device_bytes, s.device_bytes + s.device_bytes / 2);
+      if (want > device_bytes && held() - s.device_bytes + want > opt.hbm_budget_bytes) want = device_bytes;
       if (s.device) ok(ntt_hip_free(opt.device, s.device), "free");
       arena_bytes -= s.device_bytes;
       s.device = nullptr;
       s.device_bytes = 0;
-      ok(ntt_hip_malloc(opt.device, &s.device, device_bytes), "arena");
-      s.device_bytes = device_bytes;
-      arena_bytes += device_bytes;
+      ok(ntt_hip_malloc(opt.device, &s.device, want), "arena");
+      s.device_bytes = want;
+      arena_bytes += want;
     }
     if (s.pinned_bytes < pinned_bytes) {
+      const size_t want = std::max(pinned_bytes, s.pinned_bytes + s.pinned_bytes / 2);
       if (s.pinned) ok(ntt_hip_host_free(s.pinned), "host free");
       s.pinned = nullptr;
       s.pinned_bytes = 0;
-      ok(ntt_hip_host_alloc(&s.pinned, pinned_bytes), "pinned staging");
-      s.pinned_bytes = pinned_bytes;
+      ok(ntt_hip_host_alloc(&s.pinned, want), "pinned staging");
+      s.pinned_bytes = want;
     }
   }
 

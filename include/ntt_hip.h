@@ -214,6 +214,17 @@ int ntt_hip_keyswitch_workspace_bytes(ntt_hip_keyswitch *ks, size_t batch, size_
  * All pointers are device pointers; workspace of the size above. */
 int ntt_hip_keyswitch_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batch, uint64_t *out_b, uint64_t *out_a,
                           void *workspace, void *stream);
+/* The same batch with every component at its own device addresses (no
+ * counterpart in the reference; for callers whose components live in separate
+ * buffers, e.g. a server batching the requests of different clients):
+ * in[c] = L * n evaluation residues, out_b[c] / out_a[c] = L * n each. The
+ * pointer arrays are host arrays of device pointers, staged behind the
+ * workspace on `stream`; the workspace holds
+ * ntt_hip_keyswitch_gather_workspace_bytes(ks, batch) bytes. Outputs equal
+ * ntt_hip_keyswitch_run on the same components gathered into one buffer. */
+int ntt_hip_keyswitch_gather_workspace_bytes(ntt_hip_keyswitch *ks, size_t batch, size_t *bytes);
+int ntt_hip_keyswitch_run_gather(ntt_hip_keyswitch *ks, const uint64_t *const *in, size_t batch,
+                                 uint64_t *const *out_b, uint64_t *const *out_a, void *workspace, void *stream);
 /* Allocation-free form of ntt_hip_keyswitch_run (no counterpart in the
  * reference). For callers that cannot allocate device memory: code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc. The

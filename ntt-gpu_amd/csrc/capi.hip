@@ -1126,6 +1126,15 @@ struct KsBuffers {
   uint64_t *coeff, *digits, *z, *pc, *corr;
 };
 
+// The components' evaluation-form input: contiguous (base + c * L * n) or,
+// when `table` is set, at per-component device addresses table[c].
+struct KsIn {
+  const uint64_t *base = nullptr;
+  const uint64_t *const *table = nullptr;
+  KsIn(const uint64_t *b) : base(b) {}
+  KsIn(const uint64_t *const *t, int) : table(t) {}
+};
+
 KsBuffers ks_buffers(const ntt_hip_keyswitch &ks, void *workspace, size_t B) {
   KsBuffers w;
   w.coeff = (uint64_t *)workspace;                   // B * coeff_words
@@ -1151,9 +1160,9 @@ TowerMap complement_jobs(const ntt_hip_keyswitch &ks, const KsBuffers &w) {
 
 // INTT of the Q towers and, per digit, the unreduced CRT into its complement
 // (coefficient form, in the digit buffers).
-void ks_modup_convert(const ntt_hip_keyswitch &ks, const uint64_t *in, size_t B, const KsBuffers &w, hipStream_t s);
+void ks_modup_convert(const ntt_hip_keyswitch &ks, KsIn in, size_t B, const KsBuffers &w, hipStream_t s);
 
-void ks_modup(const ntt_hip_keyswitch &ks, const uint64_t *in, size_t B, const KsBuffers &w, hipStream_t s) {
+void ks_modup(const ntt_hip_keyswitch &ks, KsIn in, size_t B, const KsBuffers &w, hipStream_t s) {
   const ntt_hip_context &c = *ks.ctx;
   ks_modup_convert(ks, in, B, w, s);
   const TowerMap m = complement_jobs(ks, w);
@@ -1176,13 +1185,13 @@ void ks_modup(const ntt_hip_keyswitch &ks, const uint64_t *in, size_t B, const K
   }
 }
 
-void ks_modup_convert(const ntt_hip_keyswitch &ks, const uint64_t *in, size_t B, const KsBuffers &w, hipStream_t s) {
+void ks_modup_convert(const ntt_hip_keyswitch &ks, KsIn in, size_t B, const KsBuffers &w, hipStream_t s) {
   const ntt_hip_context &c = *ks.ctx;
   const size_t L = ks.L, T = ks.L + ks.K, n = c.n;
   const PrimeConst *pr = c.dprimes();
-  hip_ok(launch_ntt(affine(w.coeff, in, n, L, 0, ks.coeff_words(), L * n), pr, c.log_n, (uint32_t)(B * L), true,
-                    c.ntt_mode(), s, c.ntt_chunk),
-         "intt");
+  TowerMap im = affine(w.coeff, in.base, n, L, 0, ks.coeff_words(), L * n);
+  im.src_table = in.table;
+  hip_ok(launch_ntt(im, pr, c.log_n, (uint32_t)(B * L), true, c.ntt_mode(), s, c.ntt_chunk), "intt");
   for (uint32_t d = 0; d < ks.beta; ++d) {
     const auto &plan = *ks.up[d];
     const uint32_t first = d * ks.alpha, count = std::min<uint32_t>(ks.alpha, ks.L - first);
@@ -1204,11 +1213,12 @@ void ks_modup_convert(const ntt_hip_keyswitch &ks, const uint64_t *in, size_t B,
 
 // Key inner product over all digits of `plan`'s workspace with `key`'s
 // resident key, both channels, into z ([2][B] * z_words).
-void ks_mac(const ntt_hip_keyswitch &key, const ntt_hip_keyswitch &plan, const uint64_t *in, size_t B,
-            const KsBuffers &w, uint64_t *z, hipStream_t s, uint32_t first_tower = 0, uint32_t towers = 0) {
+void ks_mac(const ntt_hip_keyswitch &key, const ntt_hip_keyswitch &plan, KsIn in, size_t B, const KsBuffers &w,
+            uint64_t *z, hipStream_t s, uint32_t first_tower = 0, uint32_t towers = 0) {
   const size_t T = plan.L + plan.K, n = plan.n();
   MacLaunch m{};
-  m.own = in;
+  m.own = in.base;
+  m.own_table = in.table;
   m.own_poly_stride = plan.L * n;
   m.alpha = plan.alpha;
   m.active_q = plan.L;
@@ -1232,7 +1242,7 @@ void ks_mac(const ntt_hip_keyswitch &key, const ntt_hip_keyswitch &plan, const u
 // ModUp and the key inner product of `key` into z, interleaved by tower
 // groups (mac_group) when the plan asks for it.
 constexpr size_t kMinGroupBatch = 8;
-void ks_modup_mac(const ntt_hip_keyswitch &key, const ntt_hip_keyswitch &plan, const uint64_t *in, size_t B,
+void ks_modup_mac(const ntt_hip_keyswitch &key, const ntt_hip_keyswitch &plan, KsIn in, size_t B,
                   const KsBuffers &w, uint64_t *z, hipStream_t s) {
   const ntt_hip_context &c = *plan.ctx;
   const uint32_t T = plan.L + plan.K, G = plan.mac_group;
@@ -1258,8 +1268,10 @@ void ks_modup_mac(const ntt_hip_keyswitch &key, const ntt_hip_keyswitch &plan, c
 // channel b lands in out_b, channel a in out_a (B * L * n each).
 // accumulate: bit 0 adds the B channel into out_b, bit 1 the A channel into
 // out_a (mod q) instead of writing them.
+// out_table (when set): per-component output addresses, the B channel of
+// component c at out_table[c], the A channel at out_table[B + c].
 void ks_moddown(const ntt_hip_keyswitch &ks, const uint64_t *z, size_t B, uint64_t *out_b, uint64_t *out_a,
-                const KsBuffers &w, hipStream_t s, uint32_t accumulate = 0) {
+                const KsBuffers &w, hipStream_t s, uint32_t accumulate = 0, uint64_t *const *out_table = nullptr) {
   const ntt_hip_context &c = *ks.ctx;
   const size_t L = ks.L, K = ks.K, n = c.n;
   const PrimeConst *pr = c.dprimes();
@@ -1282,6 +1294,7 @@ void ks_moddown(const ntt_hip_keyswitch &ks, const uint64_t *z, size_t B, uint64
   TowerMap fin = with_correction(affine(w.corr, nullptr, n, L, 0, ks.corr_words(), 0), z, ks.z_words(), out_b, out_a,
                                  (uint32_t)B, L * n, ks.pinv);
   fin.epi.accumulate = accumulate;
+  fin.epi.out_table = out_table;
   if (c.ntt_chunk) {
     hip_ok(launch_ntt(fin, pr, c.log_n, (uint32_t)(2 * B * L), false, c.ntt_mode(), s, c.ntt_chunk),
            "moddown ntt + correction");
@@ -1300,6 +1313,40 @@ void ks_moddown(const ntt_hip_keyswitch &ks, const uint64_t *z, size_t B, uint64
     t0 = t1;
   }
 }
+// Lanes of a batch: up to ks.lanes sub-batches of at least kMinLaneBatch.
+uint32_t lane_count(const ntt_hip_keyswitch &ks, size_t batch) {
+  return (uint32_t)std::max<size_t>(1, std::min<size_t>(ks.lanes, batch / ntt_hip_keyswitch::kMinLaneBatch));
+}
+
+// Run fn(first, count, lane, stream) per lane: lane j takes components
+// [first_j, first_j + count_j); lane 0 runs on the caller's stream, the others
+// on side streams forked from it and joined back, so the caller's stream (or
+// its capture) orders the whole batch.
+template <class F>
+void run_lanes(ntt_hip_keyswitch &ks, size_t batch, hipStream_t s, F &&fn) {
+  const uint32_t lanes = lane_count(ks, batch);
+  if (lanes <= 1) {
+    fn(0, batch, 0u, s);
+    return;
+  }
+  ks.ensure_lanes(lanes);
+  HIP_CHECK(hipEventRecord(ks.fork, s));
+  size_t first = 0;
+  for (uint32_t j = 0; j < lanes; ++j) {
+    const size_t count = batch / lanes + (j < batch % lanes ? 1 : 0);
+    const hipStream_t st = j == 0 ? s : ks.side[j - 1];
+    if (j) HIP_CHECK(hipStreamWaitEvent(st, ks.fork, 0));
+    fn(first, count, j, st);
+    first += count;
+  }
+  for (uint32_t j = 1; j < lanes; ++j) {
+    HIP_CHECK(hipEventRecord(ks.joins[j - 1], ks.side[j - 1]));
+    HIP_CHECK(hipStreamWaitEvent(s, ks.joins[j - 1], 0));
+  }
+}
+
+// Pointer tables of ntt_hip_keyswitch_run_gather (3 per component), 256-byte padded.
+size_t gather_table_bytes(size_t batch) { return (3 * batch * sizeof(void *) + 255) / 256 * 256; }
 }  // namespace
 
 extern "C" {
@@ -1321,34 +1368,58 @@ int ntt_hip_keyswitch_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batc
     require(batch >= 1 && batch <= ks->max_batch, "batch exceeds the plan capacity");
     const hipStream_t s = ks->ctx->pick(stream);
     DeviceGuard g(ks->ctx->device);
-    auto chain = [&](size_t first, size_t count, hipStream_t st) {
+    run_lanes(*ks, batch, s, [&](size_t first, size_t count, uint32_t, hipStream_t st) {
       const size_t off = first * ks->L * ks->n();
       const KsBuffers w = ks_buffers(*ks, (uint64_t *)workspace + first * ks->per_component(), count);
       ks_modup_mac(*ks, *ks, in + off, count, w, w.z, st);
       ks_moddown(*ks, w.z, count, out_b + off, out_a + off, w, st);
-    };
-    const uint32_t lanes = (uint32_t)std::max<size_t>(1, std::min<size_t>(ks->lanes, batch / ntt_hip_keyswitch::kMinLaneBatch));
-    if (lanes <= 1) {
-      chain(0, batch, s);
-      return NTT_HIP_OK;
-    }
-    // lane j runs components [first_j, first_j + count_j) in its own part of
-    // the workspace; lane 0 on the caller's stream, the others forked from it
-    // and joined back, so the caller's stream (or its capture) orders the whole batch
-    ks->ensure_lanes(lanes);
-    HIP_CHECK(hipEventRecord(ks->fork, s));
+    });
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_keyswitch_gather_workspace_bytes(ntt_hip_keyswitch *ks, size_t batch, size_t *bytes) {
+  return guarded([&] {
+    require(ks && bytes, "null argument");
+    require(batch >= 1 && batch <= ks->max_batch, "batch exceeds the plan capacity");
+    *bytes = batch * ks->per_component() * sizeof(uint64_t) + gather_table_bytes(batch);
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_keyswitch_run_gather(ntt_hip_keyswitch *ks, const uint64_t *const *in, size_t batch,
+                                 uint64_t *const *out_b, uint64_t *const *out_a, void *workspace, void *stream) {
+  return guarded([&] {
+    require(ks && in && out_b && out_a && workspace, "null argument");
+    require(ks->key_loaded, "evaluation key not uploaded");
+    require(batch >= 1 && batch <= ks->max_batch, "batch exceeds the plan capacity");
+    for (size_t c = 0; c < batch; ++c) require(in[c] && out_b[c] && out_a[c], "null component pointer");
+    const hipStream_t s = ks->ctx->pick(stream);
+    DeviceGuard g(ks->ctx->device);
+    // the tables behind the workspace proper: the inputs, then per lane its
+    // components' B outputs followed by their A outputs (the order of the
+    // ModDown epilogue's polynomials)
+    const uint32_t lanes = lane_count(*ks, batch);
+    std::vector<const void *> table(3 * batch);
+    for (size_t c = 0; c < batch; ++c) table[c] = in[c];
     size_t first = 0;
     for (uint32_t j = 0; j < lanes; ++j) {
       const size_t count = batch / lanes + (j < batch % lanes ? 1 : 0);
-      const hipStream_t st = j == 0 ? s : ks->side[j - 1];
-      if (j) HIP_CHECK(hipStreamWaitEvent(st, ks->fork, 0));
-      chain(first, count, st);
+      for (size_t c = 0; c < count; ++c) {
+        table[batch + 2 * first + c] = out_b[first + c];
+        table[batch + 2 * first + count + c] = out_a[first + c];
+      }
       first += count;
     }
-    for (uint32_t j = 1; j < lanes; ++j) {
-      HIP_CHECK(hipEventRecord(ks->joins[j - 1], ks->side[j - 1]));
-      HIP_CHECK(hipStreamWaitEvent(s, ks->joins[j - 1], 0));
-    }
+    char *dev_table = (char *)workspace + batch * ks->per_component() * sizeof(uint64_t);
+    HIP_CHECK(hipMemcpyAsync(dev_table, table.data(), table.size() * sizeof(void *), hipMemcpyHostToDevice, s));
+    const uint64_t *const *in_tab = (const uint64_t *const *)dev_table;
+    uint64_t *const *out_tab = (uint64_t *const *)(dev_table + batch * sizeof(void *));
+    run_lanes(*ks, batch, s, [&](size_t first_c, size_t count, uint32_t, hipStream_t st) {
+      const KsBuffers w = ks_buffers(*ks, (uint64_t *)workspace + first_c * ks->per_component(), count);
+      ks_modup_mac(*ks, *ks, KsIn(in_tab + first_c, 0), count, w, w.z, st);
+      ks_moddown(*ks, w.z, count, nullptr, nullptr, w, st, 0, out_tab + 2 * first_c);
+    });
     return NTT_HIP_OK;
   });
 }

@@ -38,6 +38,7 @@ struct BasisLaunch {
 struct MacLaunch {
   const uint64_t *own;     // (poly, tower, i)
   uint64_t own_poly_stride;
+  const uint64_t *const *own_table;  // when set: polynomial p's own towers at own_table[p]
   uint32_t alpha, active_q;
   const uint64_t *digits;  // (poly, digit, tower, i)
   const uint64_t *key_b, *key_a;  // (digit, tower, i), Montgomery form

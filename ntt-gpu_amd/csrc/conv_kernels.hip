@@ -224,7 +224,8 @@ __global__ __launch_bounds__(256) void mac_fused_kernel(MacLaunch m, uint32_t po
   const uint64_t *ka = m.key_a + (uint64_t)tower * m.n + i;
   const uint64_t step = (uint64_t)m.towers * m.n;
   const uint32_t own_digit = tower < m.active_q ? tower / m.alpha : 0xffffffffu;
-  const uint64_t own = tower < m.active_q ? m.own[(uint64_t)poly * m.own_poly_stride + (uint64_t)tower * m.n + i] : 0;
+  const uint64_t *own_base = m.own_table ? m.own_table[poly] : m.own + (uint64_t)poly * m.own_poly_stride;
+  const uint64_t own = tower < m.active_q ? own_base[(uint64_t)tower * m.n + i] : 0;
   uint64_t rb, ra;
   if constexpr (!PerTerm) {
     U128 sb{0, 0}, sa{0, 0};
@@ -271,7 +272,8 @@ __global__ __launch_bounds__(256) void mac_fused_fixed(MacLaunch m, uint32_t pol
   // digit d's own towers come from the input: the address is selected, not
   // the value, so no load waits behind a branch (P towers, tower >= active_q,
   // have no own digit)
-  const uint64_t *ownp = m.own + (uint64_t)poly * m.own_poly_stride + (uint64_t)tower * m.n + i;
+  const uint64_t *ownp = (m.own_table ? m.own_table[poly] : m.own + (uint64_t)poly * m.own_poly_stride) +
+                        (uint64_t)tower * m.n + i;
   uint64_t x[D], b[D], a[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) {

@@ -49,6 +49,10 @@ struct TowerMap {
   // (then y / polys = y * polys_mul >> 32 exactly), else 0 (divide); set by
   // the launcher
   uint64_t polys_mul;
+  // When set, polynomial p's first-pass source is src_table[p] (+ the tower's
+  // offset) instead of src + p * src_poly_stride (components at scattered
+  // device addresses, e.g. the buffers of separate requests).
+  const uint64_t *const *src_table;
   // Optional epilogue of a forward transform (ModDown correction, affine maps
   // only): the last pass writes (sub - NTT(x)) * scale[k] mod q to
   // out[poly < split ? 0 : 1] + (poly mod split) * out_poly_stride + k * tower_stride
@@ -61,6 +65,9 @@ struct TowerMap {
     uint64_t sub_poly_stride, out_poly_stride;
     uint32_t split;
     uint32_t accumulate;  // bit j: out[j] += result (mod q) instead of out[j] = result
+    // When set, polynomial p's output is out_table[p] (+ k * tower_stride)
+    // instead of out[p < split ? 0 : 1] + ...
+    uint64_t *const *out_table;
   } epi;
 };
 

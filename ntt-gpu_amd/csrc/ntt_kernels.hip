@@ -319,7 +319,8 @@ __device__ __forceinline__ void tower_of(const TowerMap &m, uint32_t y, bool fir
     prime = m.prime_offset + k;
   }
   data = m.base + (uint64_t)poly * m.poly_stride + off;
-  src = (first_pass && m.src) ? m.src + (uint64_t)poly * m.src_poly_stride + src_off : data;
+  if (first_pass && m.src_table) src = m.src_table[poly] + src_off;
+  else src = (first_pass && m.src) ? m.src + (uint64_t)poly * m.src_poly_stride + src_off : data;
 }
 
 // Epilogue destinations of tower (k, poly): the subtrahend and the output;
@@ -329,6 +330,7 @@ __device__ __forceinline__ uint64_t *epi_dest(const TowerMap &m, uint32_t k, uin
   sub = m.epi.sub + (uint64_t)poly * m.epi.sub_poly_stride + (uint64_t)k * m.tower_stride;
   const bool second = poly >= m.epi.split;
   acc = (m.epi.accumulate >> (second ? 1 : 0)) & 1;
+  if (m.epi.out_table) return m.epi.out_table[poly] + (uint64_t)k * m.tower_stride;
   return m.epi.out[second ? 1 : 0] + (uint64_t)(second ? poly - m.epi.split : poly) * m.epi.out_poly_stride +
          (uint64_t)k * m.tower_stride;
 }

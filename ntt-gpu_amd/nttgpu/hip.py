@@ -75,6 +75,8 @@ SIGNATURES = {
     "ntt_hip_keyswitch_device_bytes": [VP, ctypes.POINTER(SZ)],
     "ntt_hip_transform_jobs": [VP, VP, VP, VP, SZ, VP, INT, VP],
     "ntt_hip_transform_mixed_workspace_bytes": [SZ, ctypes.POINTER(SZ)],
+    "ntt_hip_keyswitch_gather_workspace_bytes": [VP, SZ, ctypes.POINTER(SZ)],
+    "ntt_hip_keyswitch_run_gather": [VP, VP, SZ, VP, VP, VP, VP],
     "ntt_hip_transform_mixed": [VP, SZ, VP, INT, VP],
     "ntt_hip_get_ntt_shape": [VP, ctypes.POINTER(INT), ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)],
     "ntt_hip_keyswitch_capture": [VP, VP, SZ, VP, VP, VP, VP, ctypes.POINTER(VP)],
@@ -358,6 +360,23 @@ class KeySwitch:
     def run(self, in_ptr, batch, out_b_ptr, out_a_ptr, workspace_ptr, stream=None):
         check(lib().ntt_hip_keyswitch_run(self.handle, in_ptr, batch, out_b_ptr, out_a_ptr, workspace_ptr, stream),
               "keyswitch_run")
+
+    def gather_workspace_words(self, batch):
+        out = SZ(0)
+        check(lib().ntt_hip_keyswitch_gather_workspace_bytes(self.handle, batch, ctypes.byref(out)),
+              "gather_workspace_bytes")
+        return out.value // 8
+
+    def run_gather(self, in_ptrs, out_b_ptrs, out_a_ptrs, workspace_ptr, stream=None):
+        """Key-switch len(in_ptrs) components, each at its own device addresses
+        (lists of device pointers as ints); equal to run() on the same
+        components gathered into contiguous buffers."""
+        b = len(in_ptrs)
+        if len(out_b_ptrs) != b or len(out_a_ptrs) != b:
+            raise ValueError("run_gather: one output pair per input component")
+        arr = lambda ps: (ctypes.c_void_p * max(b, 1))(*[ctypes.c_void_p(int(p)) for p in ps])
+        check(lib().ntt_hip_keyswitch_run_gather(self.handle, arr(in_ptrs), b, arr(out_b_ptrs), arr(out_a_ptrs),
+                                                 workspace_ptr, stream), "keyswitch_run_gather")
 
     def autotune(self, batch):
         """Tune the plan and its context on `batch` key switches; returns the report."""

@@ -443,6 +443,73 @@ def test_keyswitch_full_batch_64_graph():
         buf.free()
 
 
+def _scattered_keyswitch(hp, n, L, B, seed):
+    """Key-switch B components once from one contiguous buffer (run) and once
+    from separately allocated per-component buffers, allocated in a shuffled
+    order (run_gather); returns both results per component."""
+    ctx = hip.Context(n, hp.q[:L] + hp.p, hp.q_psi[:L] + hp.p_psi)
+    kb, ka = _key(hp, seed)
+    ks = hip.KeySwitch(ctx, L, hp.K, hp.alpha, B)
+    ks.upload_key(kb, ka, hp.M, hp.q)
+    x = np.concatenate([uniform(n, hp.q[:L], seed + 1 + c) for c in range(B)])
+    w = L * n
+    d_in, d_b, d_a = _dev(x), hip.DeviceBuffer(B * w), hip.DeviceBuffer(B * w)
+    ws = hip.DeviceBuffer(ks.workspace_words(B))
+    ks.run(d_in.ptr, B, d_b.ptr, d_a.ptr, ws.ptr)
+    ctx.synchronize()
+    gb, ga = d_b.download(), d_a.download()
+    for buf in (d_in, d_b, d_a, ws):
+        buf.free()
+    order = np.random.default_rng(seed).permutation(B)
+    ins, obs, oas = [None] * B, [None] * B, [None] * B
+    for c in order:
+        ins[c] = _dev(x[c * w:(c + 1) * w])
+        obs[c], oas[c] = hip.DeviceBuffer(w), hip.DeviceBuffer(w)
+    wsg = hip.DeviceBuffer(ks.gather_workspace_words(B))
+    ks.run_gather([b.ptr for b in ins], [b.ptr for b in obs], [b.ptr for b in oas], wsg.ptr)
+    ctx.synchronize()
+    out = [(obs[c].download(), oas[c].download()) for c in range(B)]
+    return x, kb, ka, gb, ga, out, ks, ctx
+
+
+@pytest.mark.parametrize("n,M,K,alpha,B", [(4096, 8, 3, 3, 1), (4096, 8, 3, 3, 3), (4096, 8, 3, 3, 8),
+                                           (4096, 8, 3, 3, 17), (65536, 24, 6, 6, 17)])
+def test_keyswitch_gather_equals_contiguous(n, M, K, alpha, B):
+    """run_gather on components at scattered device addresses equals run on
+    the same components in one contiguous buffer, bit for bit, over single
+    components, one lane, two lanes and the interleaved ModUp/MAC schedule,
+    at N = 2^12 (one-pass transforms) and N = 2^16 (two-pass)."""
+    hp = params.openfhe_like(n, M, K, alpha)
+    L = M
+    log_case(f"keyswitch gather n={n} L={L} K={K} batch={B}")
+    x, kb, ka, gb, ga, out, _, _ = _scattered_keyswitch(hp, n, L, B, 7100 + B)
+    w = L * n
+    for c in range(B):
+        assert np.array_equal(out[c][0], gb[c * w:(c + 1) * w]), f"B channel of component {c}"
+        assert np.array_equal(out[c][1], ga[c * w:(c + 1) * w]), f"A channel of component {c}"
+    if n == 4096:
+        wb, wa = o.key_switch(x[:w], n, hp.q, hp.q_psi, hp.M, hp.p, hp.p_psi, hp.alpha, kb, ka)
+        assert np.array_equal(out[0][0], wb) and np.array_equal(out[0][1], wa)
+
+
+def test_keyswitch_gather_rejections():
+    hp = params.openfhe_like(1024, 4, 2, 2)
+    ctx = hip.Context(hp.n, hp.q + hp.p, hp.q_psi + hp.p_psi)
+    ks = hip.KeySwitch(ctx, 4, hp.K, hp.alpha, 2)
+    kb, ka = _key(hp, 7)
+    ks.upload_key(kb, ka, hp.M, hp.q)
+    w = 4 * hp.n
+    bufs = [hip.DeviceBuffer(w) for _ in range(6)]
+    ws = hip.DeviceBuffer(ks.gather_workspace_words(2))
+    p = [b.ptr for b in bufs]
+    with pytest.raises(hip.ContractError, match="null component pointer"):
+        ks.run_gather([p[0], 0], p[2:4], p[4:6], ws.ptr)
+    with pytest.raises(hip.ContractError, match="capacity"):
+        ks.run_gather(p[0:3], p[0:3], p[3:6], ws.ptr)
+    with pytest.raises(ValueError):
+        ks.run_gather(p[0:2], p[2:3], p[4:6], ws.ptr)
+
+
 @pytest.mark.parametrize("evaluation", [True, False])
 def test_batched_modup_moddown_l24_k6_alpha6(evaluation):
     """N=2^16, Q: 24 towers (60 + 50-bit), P: 6 towers (60-bit): batched ModUp

@@ -392,7 +392,28 @@ struct Server::Impl {
     for (const auto &r : batch) sets.push_back(param_set(r->params));
     auto in_w = [&](size_t i) { return in_words(*sets[i], batch[i]->op, batch[i]->level); };
     auto out_w = [&](size_t i) { return out_words(*sets[i], batch[i]->op, batch[i]->level); };
+    // Device-resident key switches and device-resident transforms of ring
+    // dimensions the mixed launch takes run on the callers' buffers through
+    // pointer tables: no gather into the arena, no scatter back, and the arena
+    // holds only the workspace.
+    const bool device_ks = h.device && h.op == Op::KeySwitch;
+    bool zero_copy = device_ks;
+    if (pooled && h.device) {
+      zero_copy = true;
+      for (const auto &ps : sets) zero_copy = zero_copy && mixable(*ps);
+    }
     auto footprint = [&](size_t b) {
+      if (device_ks) {
+        size_t bytes = 0;
+        ok(ntt_hip_keyswitch_gather_workspace_bytes(lp->ks, b, &bytes), "workspace");
+        return bytes;
+      }
+      if (zero_copy) {
+        size_t towers = 0, table = 0;
+        for (size_t i = 0; i < b; ++i) towers += batch[i]->level;
+        ok(ntt_hip_transform_mixed_workspace_bytes(towers, &table), "mixed workspace");
+        return (table + 255) / 256 * 256;
+      }
       size_t words = 0;
       for (size_t i = 0; i < b; ++i) words += in_w(i) + out_w(i) * (h.op == Op::KeySwitch ? 2 : 1);
       size_t bytes = words * 8;
@@ -454,7 +475,12 @@ struct Server::Impl {
       void *d_ws = d_out2 + ow2;
       uint64_t *h_in = static_cast<uint64_t *>(slot.pinned);
       const bool direct = h.device && b == 1 && !pooled;  // run on the caller's buffers
-      if (!h.device) {
+      if (zero_copy) {
+        d_ws = slot.device;  // the arena holds only the workspace
+        d_in = const_cast<uint64_t *>(h.d_in);  // a single key switch runs on the caller's buffers
+        d_out = h.d_out;
+        d_out2 = h.d_out2;
+      } else if (!h.device) {
         for (size_t i = 0; i < b; ++i) std::memcpy(h_in + in_at[i], batch[i]->in.data(), in_w(i) * 8);
         ok(ntt_hip_memcpy(opt.device, d_in, h_in, iw * 8, NTT_HIP_H2D, slot.stream), "h2d");
       } else if (direct) {
@@ -466,7 +492,30 @@ struct Server::Impl {
           ok(ntt_hip_memcpy(opt.device, d_in + in_at[i], batch[i]->d_in, in_w(i) * 8, NTT_HIP_D2D, slot.stream),
              "gather");
       }
-      if (pooled && pls.size() == 1) {
+      if (zero_copy && pooled) {
+        // one mixed launch per pass, every tower read and written in place at the caller's addresses
+        std::vector<ntt_hip_mixed_job> jobs;
+        for (size_t i = 0; i < b; ++i) {
+          const auto &pl = pls[batch[i]->n];
+          for (size_t t = 0; t < batch[i]->level; ++t) {
+            const auto f = pl->tower_of.find(sets[i]->q[t]);
+            if (f == pl->tower_of.end()) throw std::runtime_error("prime missing from the ring's pool");
+            const size_t at = t * batch[i]->n;
+            jobs.push_back({pl->ctx, batch[i]->d_out + at, batch[i]->d_in + at, f->second, 0});
+          }
+        }
+        ok(ntt_hip_transform_mixed(jobs.data(), jobs.size(), d_ws, h.op == Op::Inverse, slot.stream), "transform");
+      } else if (device_ks && b > 1) {
+        std::vector<const uint64_t *> ins(b);
+        std::vector<uint64_t *> obs(b), oas(b);
+        for (size_t i = 0; i < b; ++i) {
+          ins[i] = batch[i]->d_in;
+          obs[i] = batch[i]->d_out;
+          oas[i] = batch[i]->d_out2;
+        }
+        ok(ntt_hip_keyswitch_run_gather(lp->ks, ins.data(), b, obs.data(), oas.data(), d_ws, slot.stream),
+           "keyswitch");
+      } else if (pooled && pls.size() == 1) {
         // one job per tower of every request, each with its prime's pool tower
         const auto &pl = pls.begin()->second;
         std::vector<ntt_hip_tower_job> jobs;
@@ -513,7 +562,7 @@ struct Server::Impl {
       if (!h.device) {
         uint64_t *h_out = h_in + iw;
         ok(ntt_hip_memcpy(opt.device, h_out, d_out, (ow + ow2) * 8, NTT_HIP_D2H, slot.stream), "d2h");
-      } else if (!direct) {
+      } else if (!direct && !zero_copy) {
         for (size_t i = 0; i < b; ++i) {
           ok(ntt_hip_memcpy(opt.device, batch[i]->d_out, d_out + out_at[i], out_len[i] * 8, NTT_HIP_D2D, slot.stream),
              "scatter");

@@ -80,8 +80,8 @@ def test_admission_by_footprint_queues_instead_of_failing():
 
 def test_device_resident_requests_equal_host_requests():
     """Ciphertexts resident in HBM (submit_device): a mixed stream plus a burst of
-    one group (batches > 1 gather and scatter device to device) give the same
-    residues as the host path and the oracle."""
+    one group (key-switch batches > 1 run on the callers' buffers through
+    pointer tables) give the same residues as the host path and the oracle."""
     reqs, pools = workload.stream(24, rate_per_s=4000.0, seed=15, sizes=(8192,), levels=(3, 8))
     burst, _ = workload.stream(6, rate_per_s=4000.0, seed=16, sizes=(8192,), levels=(5, 5), dnums=(2,))
     for r in burst:
@@ -316,3 +316,32 @@ def test_transforms_of_different_ring_dimensions_share_launches():
         assert np.array_equal(a, out)
     srv.close()
     alone.close()
+
+
+def test_device_resident_transforms_of_different_ring_dimensions():
+    """Device-resident NTT and INTT requests of N = 2^13 .. 2^16 run in mixed
+    launches on the callers' buffers (no copies through the slot arena), some
+    in place (input buffer == output buffer): every result equals the oracle,
+    and some batch spans ring dimensions."""
+    sizes = (8192, 16384, 32768, 65536)
+    pools = {n: workload.PrimePool(n) for n in sizes}
+    srv = serve.Server(streams=1, max_batch=16, hbm_budget_bytes=8 << 30)
+    rng = np.random.default_rng(65)
+    reqs = []
+    for k in range(16):
+        n = sizes[k % 4]
+        hp = pools[n].param_set(2 + k % 4, 2)
+        kb, ka = workload.make_key(hp, 700 + k)
+        op = serve.FORWARD if k % 3 else serve.INVERSE
+        x = workload.seeded_residues(n, hp.q, rng)
+        d_in = hip.DeviceBuffer(x.size).upload(x)
+        d_out = d_in if k % 5 == 0 else hip.DeviceBuffer(x.size)
+        reqs.append((hp, op, srv.add_params(hp, kb, ka), x, d_in, d_out))
+    log_case("serve: device-resident transforms over four ring dimensions")
+    tickets = [srv.submit_device(pid, op, len(hp.q), d_in.ptr, d_out.ptr) for hp, op, pid, x, d_in, d_out in reqs]
+    seen = [srv.wait_device(t)[1] for t in tickets]
+    assert max(seen) >= 4, f"no batch spans ring dimensions: batch sizes {seen}"
+    for hp, op, pid, x, d_in, d_out in reqs:
+        want = o.ntt_towers(x, hp.n, hp.q, hp.q_psi, inverse=op == serve.INVERSE)
+        assert np.array_equal(d_out.download(), want), f"N={hp.n} L={len(hp.q)} op={op}"
+    srv.close()

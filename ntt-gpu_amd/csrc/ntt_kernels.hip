@@ -114,6 +114,15 @@ typedef const __attribute__((address_space(1))) tw2 *TwPtr;
 
 __device__ __forceinline__ TwPtr tw_table(const ulonglong2 *p) { return (TwPtr)(const void *)p; }
 
+// Entry k of a twiddle table (at most 2^16 entries) as a 32-bit byte offset
+// from the table's wave-uniform base: the load takes the scalar-base plus
+// vector-offset form, with no 64-bit address arithmetic per twiddle (forward
+// 59-bit transform 1% faster).
+__device__ __forceinline__ tw2 tw_at(TwPtr tw, uint32_t k) {
+  typedef const __attribute__((address_space(1))) char *BytePtr;
+  return *(TwPtr)((BytePtr)tw + (k << 4));
+}
+
 // Arithmetic modes (the A template argument):
 //   kLazy   q < 2^61: 64-bit wide-lazy (forward residues in [0, 8q), inverse in [0, 4q); ct8/gs8);
 //   kExact  2^61 <= q < 2^63: canonical throughout;
@@ -275,7 +284,7 @@ __device__ __forceinline__ void reg_stages(Val<A> (&v)[16], TwPtr tw, uint32_t m
     for (int k = 0; k < 16; ++k) {
       if (k & h) continue;
       const uint32_t idx = ml * row_factor + gb + (uint32_t)(k / (2 * h));
-      const tw2 w = tw[idx];
+      const tw2 w = tw_at(tw, idx);
       if constexpr (Inverse) gs_bfly<A>(v[k], v[k + h], w, m);
       else ct_bfly<A>(v[k], v[k + h], w, m);
     }
@@ -294,7 +303,7 @@ __device__ __forceinline__ void reg_stages_fwd(Val<A> (&v)[16], TwPtr tw, uint32
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       if (k & h) continue;
-      const tw2 w = tw[ml * row_factor + gb + (uint32_t)(k / (2 * h))];
+      const tw2 w = tw_at(tw, ml * row_factor + gb + (uint32_t)(k / (2 * h)));
       ct_planned<A, H, G0 + S>(v[k], v[k + h], w, m, neg);
     }
     reg_stages_fwd<A, H, G0, S + 1>(v, tw, ml_scale, row_factor, g_base, m, neg);
@@ -367,7 +376,7 @@ __device__ __forceinline__ void a256_body(uint64_t *data, const uint64_t *src, c
 #pragma unroll
       for (int k = 0; k < 16; ++k) {
         if (k & h) continue;
-        const tw2 w = tw[(8u / h) + (uint32_t)(k / (2 * h))];
+        const tw2 w = tw_at(tw, (8u / h) + (uint32_t)(k / (2 * h)));
         gs_bfly<A>(v[k], v[k + h], w, m);
       }
     }
@@ -459,12 +468,22 @@ __global__ __launch_bounds__(256) void ntt_pass_b(TowerMap map, const PrimeConst
 // wave, so every exchange stays inside one wave and needs no workgroup
 // barrier (LDS operations of one wave complete in issue order; the wavefront
 // fences below only keep the compiler from reordering them). With 8 residues
-// a 64-bit thread needs 62-72 VGPRs, 7-8 waves per SIMD, against 98 VGPRs and
-// 4 waves for 16 residues. Measured: +3.9% batched KeySwitch/s, inverse
-// transform 9% faster; 32-bit towers 7-8% faster in both directions.
-// Forward layouts of row index j: P1 j = u + 32k; P2 j = (u >> 2) << 5 | k << 2
-// | (u & 3); P3 j = u << 3 | k. Stages 0-2 in P1, 3-5 in P2, 6-7 in P3, then
-// back to P1 for the store (every global access a 256-byte run per row).
+// a 64-bit thread needs 56-67 VGPRs, 7-8 waves per SIMD, against 98 VGPRs and
+// 4 waves for 16 residues.
+//
+// Layouts of row element j (0..255) over lane u (0..31 within the row) and
+// register slot k (0..7):
+//   W  j = 64 (k >> 1) + 2u + (k & 1)         slot bits 2, 1, 0 <-> j bits 7, 6, 0
+//   M  j = (u >> 3) << 6 | k << 3 | (u & 7)   slot bits <-> j bits 5, 4, 3
+//   F  j = u << 3 | k                         slot bits <-> j bits 2, 1, 0
+// Global loads and stores use W: each lane moves two adjacent residues as one
+// 16-byte access (a wave instruction covers 512 bytes of each of its two
+// rows), half the memory instructions of one residue per lane. Forward:
+// stages of distance 128, 64 in W, 32, 16, 8 in M, 4, 2, 1 in F, back to W
+// for the store; the inverse runs the mirror (F, M, W) and stores from W.
+// Twiddle of the butterfly whose upper element is j: forward stage s
+// (distance 128 >> s) uses T[(rf << s) + (j >> (8 - s))], inverse stage e
+// (distance 2^e) T^-1[(rf << (7 - e)) + (j >> (e + 1))], rf = R + row.
 __device__ __forceinline__ void wave_sync8() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -472,59 +491,121 @@ __device__ __forceinline__ void wave_sync8() {
 }
 __device__ __forceinline__ int pad16(int j) { return j + (j >> 4); }
 
-// Row-pass stages S..SEnd-1 (row stage s is transform stage 8 + s) in the
-// three layouts of ntt_pass_b8_fwd: s = 0-2 in P1, 3-5 in P2, 6-7 in P3.
+enum { kLayW = 0, kLayM = 1, kLayF = 2 };
+
+// j = b8_lane<Lay>(u) | b8_slot<Lay>(k): the lane and slot parts are disjoint
+// bit fields, so j >> s = (lane part >> s) + (slot part >> s), the second a
+// compile-time constant (a twiddle's lane-dependent index is computed once
+// per stage and its slot offset lands in the load's immediate).
+template <int Lay>
+__device__ __forceinline__ int b8_lane(int u) {
+  u &= 31;
+  if constexpr (Lay == kLayW) return u << 1;
+  else if constexpr (Lay == kLayM) return ((u >> 3) << 6) | (u & 7);
+  else return u << 3;
+}
+template <int Lay>
+__host__ __device__ constexpr int b8_slot(int k) {
+  return Lay == kLayW ? ((k >> 1) << 6) | (k & 1) : Lay == kLayM ? k << 3 : k;
+}
+template <int Lay>
+__device__ __forceinline__ int b8_pos(int u, int k) { return b8_lane<Lay>(u) | b8_slot<Lay>(k); }
+
+// Slot distance of forward stage s (its layout: W for s < 2, M for s < 5, F).
+template <int S>
+struct B8Fwd {
+  static constexpr int lay = S < 2 ? kLayW : S < 5 ? kLayM : kLayF;
+  static constexpr int h = S < 2 ? 4 >> S : S < 5 ? 4 >> (S - 2) : 4 >> (S - 5);
+};
+// Slot distance of inverse stage e (F for e < 3, M for e < 6, W).
+template <int E>
+struct B8Inv {
+  static constexpr int lay = E < 3 ? kLayF : E < 6 ? kLayM : kLayW;
+  static constexpr int h = E < 3 ? 1 << E : E < 6 ? 1 << (E - 3) : 1 << (E - 5);
+};
+
+// W-layout row accesses: slots 2a and 2a + 1 of a lane are adjacent words,
+// which the compiler merges into one 16-byte access (global memory takes
+// accesses aligned to 8 bytes on this target).
+template <int A>
+__device__ __forceinline__ void b8_load_w(Val<A> (&v)[8], const uint64_t *row, int u) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k) v[k] = (Val<A>)row[b8_pos<kLayW>(u, k)];
+}
+
+__device__ __forceinline__ void b8_store_w(uint64_t *row, int u, const uint64_t (&o)[8]) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k) row[b8_pos<kLayW>(u, k)] = o[k];
+}
+
+// Move v from layout From to layout To through the row's LDS buffer.
+template <int From, int To, class V>
+__device__ __forceinline__ void b8_xchg(V (&v)[8], V *buf, int u) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k) buf[pad16(b8_pos<From>(u, k))] = v[k];
+  wave_sync8();
+#pragma unroll
+  for (int k = 0; k < 8; ++k) v[k] = buf[pad16(b8_pos<To>(u, k))];
+  wave_sync8();
+}
+
+// Forward row stages S..SEnd-1 (row stage s is transform stage 8 + s).
 template <int A, int H, int S, int SEnd>
 __device__ __forceinline__ void b8_fwd_stages(Val<A> (&v)[8], TwPtr tw, uint32_t rf, int u, const Lazy8 &m,
                                               const NegMultiples &neg) {
   if constexpr (S < SEnd) {
-    constexpr int h = S < 3 ? 4 >> S : S < 6 ? 4 >> (S - 3) : 2 >> (S - 6);
-    const uint32_t gb = S < 3 ? 0u : S < 6 ? (uint32_t)(u >> 2) << (S - 3) : (uint32_t)u << (S - 5);
-    constexpr int shift = S < 3 ? 3 - S : S < 6 ? 6 - S : 8 - S;
+    constexpr int h = B8Fwd<S>::h, lay = B8Fwd<S>::lay;
+    const uint32_t base = (rf << S) + (uint32_t)(b8_lane<lay>(u) >> (8 - S));
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       if (k & h) continue;
-      const tw2 w = tw[(1u << S) * rf + gb + (uint32_t)(k >> shift)];
+      const tw2 w = tw_at(tw, base + (uint32_t)(b8_slot<lay>(k) >> (8 - S)));
       ct_planned<A, H, 8 + S>(v[k], v[k + h], w, m, neg);
     }
     b8_fwd_stages<A, H, S + 1, SEnd>(v, tw, rf, u, m, neg);
   }
 }
 
-// Rows of one workgroup (8 of 256, row index r = 8 bx + thread / 32):
-// loads in P1 and all 8 stages; v ends in layout P3.
+// Inverse row stages E..EEnd-1 (distance 2^e).
+template <int A, int E, int EEnd>
+__device__ __forceinline__ void b8_inv_stages(Val<A> (&v)[8], TwPtr tw, uint32_t rf, int u, const Lazy8 &m) {
+  if constexpr (E < EEnd) {
+    constexpr int h = B8Inv<E>::h, lay = B8Inv<E>::lay;
+    const uint32_t base = (rf << (7 - E)) + (uint32_t)(b8_lane<lay>(u) >> (E + 1));
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (k & h) continue;
+      const tw2 w = tw_at(tw, base + (uint32_t)(b8_slot<lay>(k) >> (E + 1)));
+      gs_bfly<A>(v[k], v[k + h], w, m);
+    }
+    b8_inv_stages<A, E + 1, EEnd>(v, tw, rf, u, m);
+  }
+}
+
+// One row (row index rf - R) of a workgroup: loads in W and all 8 forward
+// stages; v ends in layout F.
 template <int A, int H>
 __device__ __forceinline__ void b8_fwd_core(Val<A> (&v)[8], const uint64_t *row, TwPtr tw, uint32_t rf, int u,
                                             const Lazy8 &m, const NegMultiples &neg, Val<A> *buf) {
-#pragma unroll
-  for (int k = 0; k < 8; ++k) v[k] = (Val<A>)row[u + 32 * k];
-  b8_fwd_stages<A, H, 0, 3>(v, tw, rf, u, m, neg);
-  const int p2 = ((u >> 2) << 5) | (u & 3);
-#pragma unroll
-  for (int k = 0; k < 8; ++k) buf[pad16(u + 32 * k)] = v[k];
-  wave_sync8();
-#pragma unroll
-  for (int k = 0; k < 8; ++k) v[k] = buf[pad16(p2 | (k << 2))];
-  wave_sync8();
-  b8_fwd_stages<A, H, 3, 6>(v, tw, rf, u, m, neg);
-#pragma unroll
-  for (int k = 0; k < 8; ++k) buf[pad16(p2 | (k << 2))] = v[k];
-  wave_sync8();
-#pragma unroll
-  for (int k = 0; k < 8; ++k) v[k] = buf[pad16((u << 3) | k)];
-  wave_sync8();
-  b8_fwd_stages<A, H, 6, 8>(v, tw, rf, u, m, neg);
+  b8_load_w<A>(v, row, u);
+  b8_fwd_stages<A, H, 0, 2>(v, tw, rf, u, m, neg);
+  b8_xchg<kLayW, kLayM>(v, buf, u);
+  b8_fwd_stages<A, H, 2, 5>(v, tw, rf, u, m, neg);
+  b8_xchg<kLayM, kLayF>(v, buf, u);
+  b8_fwd_stages<A, H, 5, 8>(v, tw, rf, u, m, neg);
 }
 
-// Canonical output of b8_fwd_core, back to P1 for a coalesced store.
+// Canonical output of b8_fwd_core, back to W for the store.
 template <int A, int H>
 __device__ __forceinline__ void b8_fwd_store(Val<A> (&v)[8], uint64_t *row, int u, const Lazy8 &m,
                                              const NegMultiples &neg, const PrimeConst &pc, Val<A> *buf) {
 #pragma unroll
-  for (int k = 0; k < 8; ++k) buf[pad16((u << 3) | k)] = canon_planned<A, H>(v[k], m, neg, pc);
+  for (int k = 0; k < 8; ++k) buf[pad16(b8_pos<kLayF>(u, k))] = canon_planned<A, H>(v[k], m, neg, pc);
   wave_sync8();
+  uint64_t o[8];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) row[u + 32 * k] = buf[pad16(u + 32 * k)];
+  for (int k = 0; k < 8; ++k) o[k] = buf[pad16(b8_pos<kLayW>(u, k))];
+  b8_store_w(row, u, o);
 }
 
 template <int A, bool Epi, int H = 0>
@@ -546,38 +627,31 @@ __global__ __launch_bounds__(256) void ntt_pass_b8_fwd(TowerMap map, const Prime
   const NegMultiples neg = neg_multiples(m);
   b8_fwd_core<A, H>(v, row, tw, rf, u, m, neg, buf);
   if constexpr (Epi) {
-#pragma unroll
-    for (int k = 0; k < 8; ++k) buf[pad16((u << 3) | k)] = v[k];
-    wave_sync8();
-#pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = buf[pad16(u + 32 * k)];
+    b8_xchg<kLayF, kLayW>(v, buf, u);
     const uint64_t *sub;
     bool acc;
     uint64_t *out = epi_dest(map, tk, tpoly, sub, acc) + (uint64_t)r * kRow;
     sub += (uint64_t)r * kRow;
     const ulonglong2 c = map.epi.scale[tk];
-    if (acc) {
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const uint64_t d = epi_diff<A>(sub[u + 32 * k], below8_planned<A, H>(v[k], m, neg, pc), m, pc.q);
-        out[u + 32 * k] = add_mod(out[u + 32 * k], shoup(d, c.x, c.y, pc.q), pc.q);
-      }
-      return;
-    }
+    uint64_t s[8], o[8];
+    b8_load_w<kLazy>(s, sub, u);  // 64-bit words whatever the arithmetic
+    if (acc) b8_load_w<kLazy>(o, out, u);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const uint64_t d = epi_diff<A>(sub[u + 32 * k], below8_planned<A, H>(v[k], m, neg, pc), m, pc.q);
-      out[u + 32 * k] = shoup(d, c.x, c.y, pc.q);
+      const uint64_t d = epi_diff<A>(s[k], below8_planned<A, H>(v[k], m, neg, pc), m, pc.q);
+      const uint64_t y = shoup(d, c.x, c.y, pc.q);
+      o[k] = acc ? add_mod(o[k], y, pc.q) : y;
     }
+    b8_store_w(out, u, o);
   } else {
     b8_fwd_store<A, H>(v, row, u, m, neg, pc, buf);
   }
 }
 
 // Inverse row pass, 8 residues per thread (first pass of an inverse
-// transform): load in P1, then P3 (distances 1, 2, 4), P2' with
-// j = (u >> 3) << 6 | k << 3 | (u & 7) (distances 8, 16, 32), P1 (64, 128).
-// One row per 32 threads (row 8 bx + thread / 32): reads src, writes data.
+// transform): load in W, stages of distance 1, 2, 4 in F, 8, 16, 32 in M,
+// 64, 128 in W, store from W. One row per 32 threads (row 8 bx + thread /
+// 32): reads src, writes data.
 template <int A>
 __device__ __forceinline__ void b8_inv_body(uint64_t *data, const uint64_t *src, const PrimeConst &pc, uint32_t R,
                                             uint32_t bx, Val<A> *lds) {
@@ -590,60 +664,17 @@ __device__ __forceinline__ void b8_inv_body(uint64_t *data, const uint64_t *src,
   const uint32_t rf = R + r;
   Val<A> *buf = lds + rl * kLdsRow;
   Val<A> v[8];
+  b8_load_w<A>(v, src_row, u);
+  b8_xchg<kLayW, kLayF>(v, buf, u);
+  b8_inv_stages<A, 0, 3>(v, tw, rf, u, m);
+  b8_xchg<kLayF, kLayM>(v, buf, u);
+  b8_inv_stages<A, 3, 6>(v, tw, rf, u, m);
+  b8_xchg<kLayM, kLayW>(v, buf, u);
+  b8_inv_stages<A, 6, 8>(v, tw, rf, u, m);
+  uint64_t o[8];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) v[k] = (Val<A>)src_row[u + 32 * k];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) buf[pad16(u + 32 * k)] = v[k];
-  wave_sync8();
-#pragma unroll
-  for (int k = 0; k < 8; ++k) v[k] = buf[pad16((u << 3) | k)];
-  wave_sync8();
-#pragma unroll
-  for (int e = 0; e < 3; ++e) {
-    const int h = 1 << e;
-    const uint32_t gb = (uint32_t)u << (2 - e);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      if (k & h) continue;
-      const tw2 w = tw[(1u << (7 - e)) * rf + gb + (uint32_t)(k >> (e + 1))];
-      gs_bfly<A>(v[k], v[k + h], w, m);
-    }
-  }
-  const int p2 = ((u >> 3) << 6) | (u & 7);
-#pragma unroll
-  for (int k = 0; k < 8; ++k) buf[pad16((u << 3) | k)] = v[k];
-  wave_sync8();
-#pragma unroll
-  for (int k = 0; k < 8; ++k) v[k] = buf[pad16(p2 | (k << 3))];
-  wave_sync8();
-#pragma unroll
-  for (int e = 3; e < 6; ++e) {
-    const int h = 1 << (e - 3);
-    const uint32_t gb = (uint32_t)(u >> 3) << (5 - e);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      if (k & h) continue;
-      const tw2 w = tw[(1u << (7 - e)) * rf + gb + (uint32_t)(k >> (e - 2))];
-      gs_bfly<A>(v[k], v[k + h], w, m);
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < 8; ++k) buf[pad16(p2 | (k << 3))] = v[k];
-  wave_sync8();
-#pragma unroll
-  for (int k = 0; k < 8; ++k) v[k] = buf[pad16(u + 32 * k)];
-#pragma unroll
-  for (int e = 6; e < 8; ++e) {
-    const int h = 1 << (e - 5);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      if (k & h) continue;
-      const tw2 w = tw[(1u << (7 - e)) * rf + (uint32_t)(k >> (e - 4))];
-      gs_bfly<A>(v[k], v[k + h], w, m);
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < 8; ++k) row[u + 32 * k] = v[k];
+  for (int k = 0; k < 8; ++k) o[k] = v[k];
+  b8_store_w(row, u, o);
 }
 
 template <int A>
@@ -691,9 +722,9 @@ __device__ __forceinline__ void lds_generic_stages(const uint64_t *src, const Pr
       Val<A> x = lds[col * L + j], y = lds[col * L + j + (1u << lg_d)];
       if constexpr (Inverse) {
         if (last) gs_bfly_last<A>(x, y, pc, m);
-        else gs_bfly<A>(x, y, tw[ml + g], m);
+        else gs_bfly<A>(x, y, tw_at(tw, ml + g), m);
       } else {
-        ct_bfly<A>(x, y, tw[ml + g], m);
+        ct_bfly<A>(x, y, tw_at(tw, ml + g), m);
       }
       lds[col * L + j] = x;
       lds[col * L + j + (1u << lg_d)] = y;

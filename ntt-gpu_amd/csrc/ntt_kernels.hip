@@ -123,6 +123,11 @@ __device__ __forceinline__ tw2 tw_at(TwPtr tw, uint32_t k) {
   return *(TwPtr)((BytePtr)tw + (k << 4));
 }
 
+// The first 256 entries of a twiddle table staged in LDS (the column pass's
+// lane-dependent twiddles, the same for every column block of a prime).
+typedef const __attribute__((address_space(3))) tw2 *TwLds;
+__device__ __forceinline__ tw2 tw_at(TwLds tw, uint32_t k) { return tw[k]; }
+
 // Arithmetic modes (the A template argument):
 //   kLazy   q < 2^61: 64-bit wide-lazy (forward residues in [0, 8q), inverse in [0, 4q); ct8/gs8);
 //   kExact  2^61 <= q < 2^63: canonical throughout;
@@ -272,8 +277,8 @@ __device__ __forceinline__ Val<A> below8_planned(Val<A> x, const Lazy8 &m, const
 // Four register stages on v[0..15]. H runs 8,4,2,1 (forward) or 1,2,4,8
 // (inverse). Twiddle index = ml * row_factor + group, ml = (8/h) * ml_scale,
 // group = g_base * (8/h) + k / (2h).
-template <bool Inverse, int A>
-__device__ __forceinline__ void reg_stages(Val<A> (&v)[16], TwPtr tw, uint32_t ml_scale, uint32_t row_factor,
+template <bool Inverse, int A, class Tw>
+__device__ __forceinline__ void reg_stages(Val<A> (&v)[16], Tw tw, uint32_t ml_scale, uint32_t row_factor,
                                            uint32_t g_base, const Lazy8 &m) {
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
@@ -293,8 +298,8 @@ __device__ __forceinline__ void reg_stages(Val<A> (&v)[16], TwPtr tw, uint32_t m
 
 // Four forward register stages G0..G0+3 of plan H (the forward form of
 // reg_stages; distances 8, 4, 2, 1).
-template <int A, int H, int G0, int S = 0>
-__device__ __forceinline__ void reg_stages_fwd(Val<A> (&v)[16], TwPtr tw, uint32_t ml_scale, uint32_t row_factor,
+template <int A, int H, int G0, int S = 0, class Tw>
+__device__ __forceinline__ void reg_stages_fwd(Val<A> (&v)[16], Tw tw, uint32_t ml_scale, uint32_t row_factor,
                                                uint32_t g_base, const Lazy8 &m, const NegMultiples &neg) {
   if constexpr (S < 4) {
     constexpr int h = 8 >> S;
@@ -349,25 +354,33 @@ __device__ __forceinline__ uint64_t *epi_dest(const TowerMap &m, uint32_t k, uin
 // forward input (== data unless the first pass reads elsewhere).
 template <bool Inverse, int A, int H>
 __device__ __forceinline__ void a256_body(uint64_t *data, const uint64_t *src, const PrimeConst &pc, uint32_t bx,
-                                          Val<A> *lds) {
+                                          Val<A> *lds, tw2 *twl) {
   const TwPtr tw = tw_table(Inverse ? pc.inv : pc.fwd);
   const Lazy8 m = lazy8_of(pc.q, pc.nq, pc.nq2, pc.nq4);
   const int t = threadIdx.x, cl = t & 15, u = t >> 4;
   const uint32_t c = bx * 16 + cl;  // column
+  const TwLds twp = (TwLds)(const void *)twl;
+  // The four lane-dependent stages (twiddles 16..255, 4 distinct per wave)
+  // read the table from LDS: one 16-byte load per thread instead of 15 per
+  // thread through the vector cache (the uniform stages keep scalar loads).
+  const tw2 mine = tw_at(tw, (uint32_t)t);
   Val<A> v[16];
   if constexpr (!Inverse) {
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = src[(uint64_t)(u + 16 * k) * kRow + c];
+    twl[t] = mine;  // ordered before its readers by the exchange's barriers
     const NegMultiples neg = neg_multiples(m);
     reg_stages_fwd<A, H, 0>(v, tw, 1, 1, 0, m, neg);
     xchg_cols_p1_p2(v, lds, cl, u);
-    reg_stages_fwd<A, H, 4>(v, tw, 16, 1, u, m, neg);
+    reg_stages_fwd<A, H, 4>(v, twp, 16, 1, u, m, neg);
 #pragma unroll
     for (int k = 0; k < 16; ++k) data[(uint64_t)(16 * u + k) * kRow + c] = v[k];
   } else {
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = data[(uint64_t)(16 * u + k) * kRow + c];
-    reg_stages<true, A>(v, tw, 16, 1, u, m);
+    twl[t] = mine;
+    __syncthreads();
+    reg_stages<true, A>(v, twp, 16, 1, u, m);
     xchg_cols_p2_p1(v, lds, cl, u);
     // stages h = 1, 2, 4 (ml = 8, 4, 2), then the scaled last stage (ml = 1)
 #pragma unroll
@@ -390,12 +403,13 @@ __device__ __forceinline__ void a256_body(uint64_t *data, const uint64_t *src, c
 template <bool Inverse, int A, int H = 0>
 __global__ __launch_bounds__(256) void ntt_pass_a256(TowerMap map, const PrimeConst *__restrict__ primes) {
   __shared__ Val<A> lds[16 * kHalfRow];
+  __shared__ tw2 twl[256];
   uint64_t *data;
   const uint64_t *src;
   uint32_t prime, tk, tpoly;
   tower_of(map, blockIdx.y, !Inverse, data, src, prime, tk, tpoly);
   const PrimeConst pc = primes[prime];
-  a256_body<Inverse, A, H>(data, src, pc, blockIdx.x, lds);
+  a256_body<Inverse, A, H>(data, src, pc, blockIdx.x, lds, twl);
 }
 
 // ---------------------------------------------------------------- pass B (rows of 256)
@@ -791,6 +805,7 @@ __device__ __forceinline__ uint32_t mixed_job_of(const MixedJob *jobs, uint32_t 
 template <bool Inverse, int A>
 __global__ __launch_bounds__(256) void ntt_mixed_pass_a(const MixedJob *jobs, uint32_t count, uint32_t items) {
   __shared__ Val<A> lds[4096];  // the generic column pass's block; the 2^16 one needs 16 * kHalfRow
+  __shared__ tw2 twl[256];       // the 2^16 column pass's staged twiddles
   typedef const __attribute__((address_space(4))) MixedJob *CJob;
   const CJob cj = (CJob)(const void *)jobs;
   for (uint32_t it = blockIdx.x; it < items; it += gridDim.x) {
@@ -800,7 +815,7 @@ __global__ __launch_bounds__(256) void ntt_mixed_pass_a(const MixedJob *jobs, ui
     const uint32_t log_n = cj[j].log_n, bx = it - cj[j].first;
     const PrimeConst pc = *cj[j].prime;
     if (log_n == 16) {
-      a256_body<Inverse, A, 0>(data, src, pc, bx, lds);
+      a256_body<Inverse, A, 0>(data, src, pc, bx, lds, twl);
     } else {
       const uint32_t log_r = log_n - 8, cols = 4096u >> log_r, col0 = bx * cols, L = 1u << log_r;
       lds_generic_stages<Inverse, A>(src, pc, log_r, kRow, cols, col0, Inverse, lds);
@@ -882,7 +897,7 @@ hipError_t launch_ntt_mixed(const MixedJob *jobs, uint32_t count, uint32_t block
 // workgroups per CU (160 KiB of LDS; 0 = no cap).
 static uint32_t lds_pad_for(uint32_t cap) {
   if (cap == 0) return 0;
-  constexpr uint32_t kLdsPerCu = 160 * 1024, kStatic = 17536;  // largest static LDS of the 2^16 passes
+  constexpr uint32_t kLdsPerCu = 160 * 1024, kStatic = 17536 + 4096;  // largest static LDS of the 2^16 passes
   const uint32_t per_wg = kLdsPerCu / (cap + 1) + 1;
   return per_wg > kStatic ? per_wg - kStatic : 0;
 }

@@ -563,36 +563,50 @@ __device__ __forceinline__ void b8_xchg(V (&v)[8], V *buf, int u) {
   wave_sync8();
 }
 
+// A row's twiddles of its first kB8Staged forward stages (the last ones of the
+// inverse) are shared by 4 to 32 lanes each: they are staged in 16 LDS
+// entries per row, entry 2^s - 1 + i holding T[(rf << s) + i] (stage s,
+// 2^s entries), loaded by lanes 0-14 of the row and read back by all.
+constexpr int kB8Staged = 4;
+__device__ __forceinline__ void b8_stage_tw(tw2 *trow, TwPtr tw, uint32_t rf, int u) {
+  if (u < (1 << kB8Staged) - 1) {
+    const uint32_t s = 31 - __clz(u + 1), i = (uint32_t)(u + 1) - (1u << s);
+    trow[u] = tw_at(tw, (rf << s) + i);
+  }
+}
+
 // Forward row stages S..SEnd-1 (row stage s is transform stage 8 + s).
 template <int A, int H, int S, int SEnd>
-__device__ __forceinline__ void b8_fwd_stages(Val<A> (&v)[8], TwPtr tw, uint32_t rf, int u, const Lazy8 &m,
+__device__ __forceinline__ void b8_fwd_stages(Val<A> (&v)[8], TwPtr tw, TwLds trow, uint32_t rf, int u, const Lazy8 &m,
                                               const NegMultiples &neg) {
   if constexpr (S < SEnd) {
     constexpr int h = B8Fwd<S>::h, lay = B8Fwd<S>::lay;
-    const uint32_t base = (rf << S) + (uint32_t)(b8_lane<lay>(u) >> (8 - S));
+    const uint32_t lane = (uint32_t)(b8_lane<lay>(u) >> (8 - S));
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       if (k & h) continue;
-      const tw2 w = tw_at(tw, base + (uint32_t)(b8_slot<lay>(k) >> (8 - S)));
+      const uint32_t off = lane + (uint32_t)(b8_slot<lay>(k) >> (8 - S));
+      const tw2 w = S < kB8Staged ? tw_at(trow, (1u << S) - 1 + off) : tw_at(tw, (rf << S) + off);
       ct_planned<A, H, 8 + S>(v[k], v[k + h], w, m, neg);
     }
-    b8_fwd_stages<A, H, S + 1, SEnd>(v, tw, rf, u, m, neg);
+    b8_fwd_stages<A, H, S + 1, SEnd>(v, tw, trow, rf, u, m, neg);
   }
 }
 
 // Inverse row stages E..EEnd-1 (distance 2^e).
 template <int A, int E, int EEnd>
-__device__ __forceinline__ void b8_inv_stages(Val<A> (&v)[8], TwPtr tw, uint32_t rf, int u, const Lazy8 &m) {
+__device__ __forceinline__ void b8_inv_stages(Val<A> (&v)[8], TwPtr tw, TwLds trow, uint32_t rf, int u, const Lazy8 &m) {
   if constexpr (E < EEnd) {
     constexpr int h = B8Inv<E>::h, lay = B8Inv<E>::lay;
-    const uint32_t base = (rf << (7 - E)) + (uint32_t)(b8_lane<lay>(u) >> (E + 1));
+    const uint32_t lane = (uint32_t)(b8_lane<lay>(u) >> (E + 1));
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       if (k & h) continue;
-      const tw2 w = tw_at(tw, base + (uint32_t)(b8_slot<lay>(k) >> (E + 1)));
+      const uint32_t off = lane + (uint32_t)(b8_slot<lay>(k) >> (E + 1));
+      const tw2 w = 7 - E < kB8Staged ? tw_at(trow, (1u << (7 - E)) - 1 + off) : tw_at(tw, (rf << (7 - E)) + off);
       gs_bfly<A>(v[k], v[k + h], w, m);
     }
-    b8_inv_stages<A, E + 1, EEnd>(v, tw, rf, u, m);
+    b8_inv_stages<A, E + 1, EEnd>(v, tw, trow, rf, u, m);
   }
 }
 
@@ -600,13 +614,16 @@ __device__ __forceinline__ void b8_inv_stages(Val<A> (&v)[8], TwPtr tw, uint32_t
 // stages; v ends in layout F.
 template <int A, int H>
 __device__ __forceinline__ void b8_fwd_core(Val<A> (&v)[8], const uint64_t *row, TwPtr tw, uint32_t rf, int u,
-                                            const Lazy8 &m, const NegMultiples &neg, Val<A> *buf) {
+                                            const Lazy8 &m, const NegMultiples &neg, Val<A> *buf, tw2 *trow) {
+  b8_stage_tw(trow, tw, rf, u);
   b8_load_w<A>(v, row, u);
-  b8_fwd_stages<A, H, 0, 2>(v, tw, rf, u, m, neg);
+  wave_sync8();
+  const TwLds tws = (TwLds)(const void *)trow;
+  b8_fwd_stages<A, H, 0, 2>(v, tw, tws, rf, u, m, neg);
   b8_xchg<kLayW, kLayM>(v, buf, u);
-  b8_fwd_stages<A, H, 2, 5>(v, tw, rf, u, m, neg);
+  b8_fwd_stages<A, H, 2, 5>(v, tw, tws, rf, u, m, neg);
   b8_xchg<kLayM, kLayF>(v, buf, u);
-  b8_fwd_stages<A, H, 5, 8>(v, tw, rf, u, m, neg);
+  b8_fwd_stages<A, H, 5, 8>(v, tw, tws, rf, u, m, neg);
 }
 
 // Canonical output of b8_fwd_core, back to W for the store.
@@ -625,6 +642,7 @@ __device__ __forceinline__ void b8_fwd_store(Val<A> (&v)[8], uint64_t *row, int 
 template <int A, bool Epi, int H = 0>
 __global__ __launch_bounds__(256) void ntt_pass_b8_fwd(TowerMap map, const PrimeConst *__restrict__ primes, uint32_t R) {
   __shared__ Val<A> lds[8 * kLdsRow];
+  __shared__ tw2 tws[8 * 16];
   uint64_t *data;
   const uint64_t *src;
   uint32_t prime, tk, tpoly;
@@ -639,7 +657,7 @@ __global__ __launch_bounds__(256) void ntt_pass_b8_fwd(TowerMap map, const Prime
   Val<A> *buf = lds + rl * kLdsRow;
   Val<A> v[8];
   const NegMultiples neg = neg_multiples(m);
-  b8_fwd_core<A, H>(v, row, tw, rf, u, m, neg, buf);
+  b8_fwd_core<A, H>(v, row, tw, rf, u, m, neg, buf, tws + rl * 16);
   if constexpr (Epi) {
     b8_xchg<kLayF, kLayW>(v, buf, u);
     const uint64_t *sub;
@@ -668,7 +686,7 @@ __global__ __launch_bounds__(256) void ntt_pass_b8_fwd(TowerMap map, const Prime
 // 32): reads src, writes data.
 template <int A>
 __device__ __forceinline__ void b8_inv_body(uint64_t *data, const uint64_t *src, const PrimeConst &pc, uint32_t R,
-                                            uint32_t bx, Val<A> *lds) {
+                                            uint32_t bx, Val<A> *lds, tw2 *tws) {
   const TwPtr tw = tw_table(pc.inv);
   const Lazy8 m = lazy8_of(pc.q, pc.nq, pc.nq2, pc.nq4);
   const int t = threadIdx.x, u = t & 31, rl = t >> 5;
@@ -677,14 +695,17 @@ __device__ __forceinline__ void b8_inv_body(uint64_t *data, const uint64_t *src,
   const uint64_t *src_row = src + (uint64_t)r * kRow;
   const uint32_t rf = R + r;
   Val<A> *buf = lds + rl * kLdsRow;
+  tw2 *trow = tws + rl * 16;
   Val<A> v[8];
+  b8_stage_tw(trow, tw, rf, u);  // read back after the exchanges' wave syncs
   b8_load_w<A>(v, src_row, u);
   b8_xchg<kLayW, kLayF>(v, buf, u);
-  b8_inv_stages<A, 0, 3>(v, tw, rf, u, m);
+  const TwLds twr = (TwLds)(const void *)trow;
+  b8_inv_stages<A, 0, 3>(v, tw, twr, rf, u, m);
   b8_xchg<kLayF, kLayM>(v, buf, u);
-  b8_inv_stages<A, 3, 6>(v, tw, rf, u, m);
+  b8_inv_stages<A, 3, 6>(v, tw, twr, rf, u, m);
   b8_xchg<kLayM, kLayW>(v, buf, u);
-  b8_inv_stages<A, 6, 8>(v, tw, rf, u, m);
+  b8_inv_stages<A, 6, 8>(v, tw, twr, rf, u, m);
   uint64_t o[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) o[k] = v[k];
@@ -694,12 +715,13 @@ __device__ __forceinline__ void b8_inv_body(uint64_t *data, const uint64_t *src,
 template <int A>
 __global__ __launch_bounds__(256) void ntt_pass_b8_inv(TowerMap map, const PrimeConst *__restrict__ primes, uint32_t R) {
   __shared__ Val<A> lds[8 * kLdsRow];
+  __shared__ tw2 tws[8 * 16];
   uint64_t *data;
   const uint64_t *src;
   uint32_t prime, tk, tpoly;
   tower_of(map, blockIdx.y, true, data, src, prime, tk, tpoly);
   const PrimeConst pc = primes[prime];
-  b8_inv_body<A>(data, src, pc, R, blockIdx.x, lds);
+  b8_inv_body<A>(data, src, pc, R, blockIdx.x, lds, tws);
 }
 
 // ---------------------------------------------------------------- generic LDS kernel
@@ -831,6 +853,7 @@ __global__ __launch_bounds__(256) void ntt_mixed_pass_a(const MixedJob *jobs, ui
 template <bool Inverse, int A>
 __global__ __launch_bounds__(256) void ntt_mixed_pass_b(const MixedJob *jobs, uint32_t count, uint32_t items) {
   __shared__ Val<A> lds[8 * kLdsRow];
+  __shared__ tw2 tws[8 * 16];
   typedef const __attribute__((address_space(4))) MixedJob *CJob;
   const CJob cj = (CJob)(const void *)jobs;
   for (uint32_t it = blockIdx.x; it < items; it += gridDim.x) {
@@ -839,7 +862,7 @@ __global__ __launch_bounds__(256) void ntt_mixed_pass_b(const MixedJob *jobs, ui
     const uint32_t R = 1u << (cj[j].log_n - 8), bx = it - 2 * cj[j].first;
     const PrimeConst pc = *cj[j].prime;
     if constexpr (Inverse) {
-      b8_inv_body<A>(data, cj[j].src, pc, R, bx, lds);  // inverse: first pass
+      b8_inv_body<A>(data, cj[j].src, pc, R, bx, lds, tws);  // inverse: first pass
     } else {
       const TwPtr tw = tw_table(pc.fwd);
       const Lazy8 m = lazy8_of(pc.q, pc.nq, pc.nq2, pc.nq4);
@@ -849,7 +872,7 @@ __global__ __launch_bounds__(256) void ntt_mixed_pass_b(const MixedJob *jobs, ui
       uint64_t *row = data + (uint64_t)r * kRow;
       Val<A> *buf = lds + rl * kLdsRow;
       Val<A> v[8];
-      b8_fwd_core<A, 0>(v, row, tw, R + r, u, m, neg, buf);
+      b8_fwd_core<A, 0>(v, row, tw, R + r, u, m, neg, buf, tws + rl * 16);
       b8_fwd_store<A, 0>(v, row, u, m, neg, pc, buf);
     }
     wave_sync8();  // a wave's exchange rows are its own; order them against its next item

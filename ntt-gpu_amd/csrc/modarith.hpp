@@ -238,6 +238,43 @@ __host__ __device__ constexpr FwdPlanData make_fwd_plan(int H) {
   return p;
 }
 
+// ---------------------------------------------------------------- inverse reduction schedule (N = 2^16)
+// An inverse (Gentleman-Sande) butterfly whose inputs are below b q computes
+// s = x + y < 2b q and d = x + b q - y in (0, 2b q); its product output
+// d w mod q lies in [0, 4q) for any 64-bit d (shoup4), so only the sums grow,
+// doubling per stage. With every q < 2^(61 - H) a sum below 2^(3 + H) q still
+// fits 64 bits: make_inv_plan(H) lets inputs grow to cap q (cap = 2^(2 + H),
+// at most 64) and then brings the stage's sums back below 2q with the
+// one-multiply quotient estimate (reduce2_est, valid below 2^7 q), 4, 3 or 2
+// estimates over the 16 stages for H = 2, 3 and 4-5 against a conditional
+// subtract at every stage. H = 0 (cap 4) keeps one conditional subtract of
+// 4q per stage. c_log[g]: stage g's inputs are below 2^c_log[g] q, the
+// multiple of q its d adds; red[g]: 0 none, 1 subtract 4q, 2 estimate. The
+// last stage (n^-1 folded in) reduces its outputs itself.
+struct InvPlanData {
+  int red[16];
+  int c_log[16];
+};
+__host__ __device__ constexpr InvPlanData make_inv_plan(int H) {
+  InvPlanData p{};
+  const int cap = H == 0 ? 4 : (4 << H) < 64 ? (4 << H) : 64;
+  int b = 1;  // canonical input
+  for (int g = 0; g < 16; ++g) {
+    int l = 0;
+    while ((1 << l) < b) ++l;
+    p.c_log[g] = l;
+    const int next = 2 * b > 4 ? 2 * b : 4;
+    if (g == 15 || next <= cap) {
+      p.red[g] = 0;
+      b = next;
+    } else {
+      p.red[g] = H == 0 ? 1 : 2;
+      b = 4;
+    }
+  }
+  return p;
+}
+
 // ---------------------------------------------------------------- Barrett
 // Variable x variable products a * b mod q (the pointwise product of two
 // residue arrays; the reference's candidate form is include/ntt/barrett.hpp:6-49)

@@ -206,6 +206,40 @@ template <int H>
 struct FwdPlan {
   static constexpr FwdPlanData d = make_fwd_plan(H);
 };
+template <int H>
+struct InvPlan {
+  static constexpr InvPlanData d = make_inv_plan(H);
+};
+
+// Inverse butterfly of stage g of plan H (wide-lazy towers; every other
+// arithmetic, and H = 0, keeps gs_bfly's per-stage reduction).
+template <int A, int H, int G>
+__device__ __forceinline__ void gs_planned(Val<A> &x, Val<A> &y, tw2 w, const Lazy8 &m, const PrimeConst &pc) {
+  if constexpr (A == kLazy && H > 0) {
+    constexpr int red = InvPlan<H>::d.red[G];
+    const uint64_t s = x + y;
+    const uint64_t d = x + (pc.q << InvPlan<H>::d.c_log[G]) - y;
+    if constexpr (red == 2) x = reduce2_est(s, pc.est_shift, pc.est_mul, m.nq);
+    else if constexpr (red == 1) x = sub_if_ge_neg(s, m.nq4);
+    else x = s;
+    y = shoup4(d, w.x, w.y, m);
+  } else {
+    gs_bfly<A>(x, y, w, m);
+  }
+}
+
+// Last inverse stage of plan H, n^-1 folded in, canonical outputs.
+template <int A, int H>
+__device__ __forceinline__ void gs_last_planned(Val<A> &x, Val<A> &y, const PrimeConst &pc, const Lazy8 &m) {
+  if constexpr (A == kLazy && H > 0) {
+    const uint64_t s = x + y;
+    const uint64_t d = x + (pc.q << InvPlan<H>::d.c_log[15]) - y;
+    x = reduce_once(shoup_lazy(s, pc.ninv, pc.ninv_shoup, pc.q), pc.q);
+    y = reduce_once(shoup_lazy(d, pc.last_w, pc.last_wp, pc.q), pc.q);
+  } else {
+    gs_bfly_last<A>(x, y, pc, m);
+  }
+}
 
 // 2^64 - 2^k q for k = 0..6 (k <= 2 precomputed per prime; larger ones as
 // shifts of 2^64 - 4q, opaque to the optimizer, which otherwise folds the add
@@ -296,6 +330,25 @@ __device__ __forceinline__ void reg_stages(Val<A> (&v)[16], Tw tw, uint32_t ml_s
   }
 }
 
+// Four inverse register stages G0..G0+3 of plan H (distances 1, 2, 4, 8; the
+// planned form of reg_stages<true>).
+template <int A, int H, int G0, int S = 0, class Tw>
+__device__ __forceinline__ void reg_stages_inv(Val<A> (&v)[16], Tw tw, uint32_t ml_scale, uint32_t row_factor,
+                                               uint32_t g_base, const Lazy8 &m, const PrimeConst &pc) {
+  if constexpr (S < 4) {
+    constexpr int h = 1 << S;
+    const uint32_t ml = (8u / h) * ml_scale;
+    const uint32_t gb = g_base * (8u / h);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (k & h) continue;
+      const tw2 w = tw_at(tw, ml * row_factor + gb + (uint32_t)(k / (2 * h)));
+      gs_planned<A, H, G0 + S>(v[k], v[k + h], w, m, pc);
+    }
+    reg_stages_inv<A, H, G0, S + 1>(v, tw, ml_scale, row_factor, g_base, m, pc);
+  }
+}
+
 // Four forward register stages G0..G0+3 of plan H (the forward form of
 // reg_stages; distances 8, 4, 2, 1).
 template <int A, int H, int G0, int S = 0, class Tw>
@@ -350,6 +403,22 @@ __device__ __forceinline__ uint64_t *epi_dest(const TowerMap &m, uint32_t k, uin
 }
 
 // ---------------------------------------------------------------- pass A, R = 256
+// Inverse column-pass stages 12 + S .. 14 (P1 layout, distances 1, 2, 4;
+// uniform twiddles through scalar loads).
+template <int A, int H, int S>
+__device__ __forceinline__ void a256_inv_tail(Val<A> (&v)[16], TwPtr tw, const Lazy8 &m, const PrimeConst &pc) {
+  if constexpr (S < 3) {
+    constexpr int h = 1 << S;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (k & h) continue;
+      const tw2 w = tw_at(tw, (8u / h) + (uint32_t)(k / (2 * h)));
+      gs_planned<A, H, 12 + S>(v[k], v[k + h], w, m, pc);
+    }
+    a256_inv_tail<A, H, S + 1>(v, tw, m, pc);
+  }
+}
+
 // Column block bx (16 columns of 256) of one tower: data in place, src the
 // forward input (== data unless the first pass reads elsewhere).
 template <bool Inverse, int A, int H>
@@ -380,21 +449,13 @@ __device__ __forceinline__ void a256_body(uint64_t *data, const uint64_t *src, c
     for (int k = 0; k < 16; ++k) v[k] = data[(uint64_t)(16 * u + k) * kRow + c];
     twl[t] = mine;
     __syncthreads();
-    reg_stages<true, A>(v, twp, 16, 1, u, m);
+    reg_stages_inv<A, H, 8>(v, twp, 16, 1, u, m, pc);
     xchg_cols_p2_p1(v, lds, cl, u);
-    // stages h = 1, 2, 4 (ml = 8, 4, 2), then the scaled last stage (ml = 1)
+    // transform stages 12-14 (h = 1, 2, 4; ml = 8, 4, 2), then the scaled
+    // last stage (ml = 1)
+    a256_inv_tail<A, H, 0>(v, tw, m, pc);
 #pragma unroll
-    for (int s = 0; s < 3; ++s) {
-      const int h = 1 << s;
-#pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        if (k & h) continue;
-        const tw2 w = tw_at(tw, (8u / h) + (uint32_t)(k / (2 * h)));
-        gs_bfly<A>(v[k], v[k + h], w, m);
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < 8; ++k) gs_bfly_last<A>(v[k], v[k + 8], pc, m);
+    for (int k = 0; k < 8; ++k) gs_last_planned<A, H>(v[k], v[k + 8], pc, m);
 #pragma unroll
     for (int k = 0; k < 16; ++k) data[(uint64_t)(u + 16 * k) * kRow + c] = v[k];
   }
@@ -598,8 +659,9 @@ __device__ __forceinline__ void b8_fwd_stages(Val<A> (&v)[8], TwPtr tw, TwLds tr
 }
 
 // Inverse row stages E..EEnd-1 (distance 2^e).
-template <int A, int E, int EEnd>
-__device__ __forceinline__ void b8_inv_stages(Val<A> (&v)[8], TwPtr tw, TwLds trow, uint32_t rf, int u, const Lazy8 &m) {
+template <int A, int H, int E, int EEnd>
+__device__ __forceinline__ void b8_inv_stages(Val<A> (&v)[8], TwPtr tw, TwLds trow, uint32_t rf, int u, const Lazy8 &m,
+                                              const PrimeConst &pc) {
   if constexpr (E < EEnd) {
     constexpr int h = B8Inv<E>::h, lay = B8Inv<E>::lay;
     const uint32_t lane = (uint32_t)(b8_lane<lay>(u) >> (E + 1));
@@ -608,9 +670,9 @@ __device__ __forceinline__ void b8_inv_stages(Val<A> (&v)[8], TwPtr tw, TwLds tr
       if (k & h) continue;
       const uint32_t off = lane + (uint32_t)(b8_slot<lay>(k) >> (E + 1));
       const tw2 w = 7 - E < kB8Staged ? tw_at(trow, (1u << (7 - E)) - 1 + off) : tw_at(tw, (rf << (7 - E)) + off);
-      gs_bfly<A>(v[k], v[k + h], w, m);
+      gs_planned<A, H, E>(v[k], v[k + h], w, m, pc);
     }
-    b8_inv_stages<A, E + 1, EEnd>(v, tw, trow, rf, u, m);
+    b8_inv_stages<A, H, E + 1, EEnd>(v, tw, trow, rf, u, m, pc);
   }
 }
 
@@ -688,7 +750,7 @@ __global__ __launch_bounds__(256) void ntt_pass_b8_fwd(TowerMap map, const Prime
 // transform): load in W, stages of distance 1, 2, 4 in F, 8, 16, 32 in M,
 // 64, 128 in W, store from W. One row per 32 threads (row 8 bx + thread /
 // 32): reads src, writes data.
-template <int A>
+template <int A, int H>
 __device__ __forceinline__ void b8_inv_body(uint64_t *data, const uint64_t *src, const PrimeConst &pc, uint32_t R,
                                             uint32_t bx, Val<A> *lds, tw2 *tws) {
   const TwPtr tw = tw_table(pc.inv);
@@ -705,18 +767,18 @@ __device__ __forceinline__ void b8_inv_body(uint64_t *data, const uint64_t *src,
   b8_load_w<A>(v, src_row, u);
   b8_xchg<kLayW, kLayF>(v, buf, u);
   const TwLds twr = (TwLds)(const void *)trow;
-  b8_inv_stages<A, 0, 3>(v, tw, twr, rf, u, m);
+  b8_inv_stages<A, H, 0, 3>(v, tw, twr, rf, u, m, pc);
   b8_xchg<kLayF, kLayM>(v, buf, u);
-  b8_inv_stages<A, 3, 6>(v, tw, twr, rf, u, m);
+  b8_inv_stages<A, H, 3, 6>(v, tw, twr, rf, u, m, pc);
   b8_xchg<kLayM, kLayW>(v, buf, u);
-  b8_inv_stages<A, 6, 8>(v, tw, twr, rf, u, m);
+  b8_inv_stages<A, H, 6, 8>(v, tw, twr, rf, u, m, pc);
   uint64_t o[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) o[k] = v[k];
   b8_store_w(row, u, o);
 }
 
-template <int A>
+template <int A, int H = 0>
 __global__ __launch_bounds__(256) void ntt_pass_b8_inv(TowerMap map, const PrimeConst *__restrict__ primes, uint32_t R) {
   __shared__ Val<A> lds[8 * kLdsRow];
   __shared__ tw2 tws[8 * (1 << kB8Staged)];
@@ -725,7 +787,7 @@ __global__ __launch_bounds__(256) void ntt_pass_b8_inv(TowerMap map, const Prime
   uint32_t prime, tk, tpoly;
   tower_of(map, blockIdx.y, true, data, src, prime, tk, tpoly);
   const PrimeConst pc = primes[prime];
-  b8_inv_body<A>(data, src, pc, R, blockIdx.x, lds, tws);
+  b8_inv_body<A, H>(data, src, pc, R, blockIdx.x, lds, tws);
 }
 
 // ---------------------------------------------------------------- generic LDS kernel
@@ -866,7 +928,7 @@ __global__ __launch_bounds__(256) void ntt_mixed_pass_b(const MixedJob *jobs, ui
     const uint32_t R = 1u << (cj[j].log_n - 8), bx = it - 2 * cj[j].first;
     const PrimeConst pc = *cj[j].prime;
     if constexpr (Inverse) {
-      b8_inv_body<A>(data, cj[j].src, pc, R, bx, lds, tws);  // inverse: first pass
+      b8_inv_body<A, 0>(data, cj[j].src, pc, R, bx, lds, tws);  // inverse: first pass
     } else {
       const TwPtr tw = tw_table(pc.fwd);
       const Lazy8 m = lazy8_of(pc.q, pc.nq, pc.nq2, pc.nq4);
@@ -941,7 +1003,7 @@ static hipError_t launch_t(const TowerMap &map, const PrimeConst *primes, uint32
   const uint32_t R = n / kRow, log_r = log_n - 8;
   auto pass_a = [&]() {
     if (R == 256) {
-      hipLaunchKernelGGL((ntt_pass_a256<Inverse, A, Inverse ? 0 : H>), dim3(kRow / 16, towers), dim3(256), pad,
+      hipLaunchKernelGGL((ntt_pass_a256<Inverse, A, H>), dim3(kRow / 16, towers), dim3(256), pad,
                          stream, map, primes);
     } else {
       const uint32_t cols = 4096 / R;
@@ -953,7 +1015,7 @@ static hipError_t launch_t(const TowerMap &map, const PrimeConst *primes, uint32
   auto pass_b = [&]() {
     if constexpr (Inverse && (A == kLazy || A == kNarrow)) {
       if (R == 256 && !row16) {
-        hipLaunchKernelGGL((ntt_pass_b8_inv<A>), dim3(R / 8, towers), dim3(256), pb, stream, map, primes, R);
+        hipLaunchKernelGGL((ntt_pass_b8_inv<A, H>), dim3(R / 8, towers), dim3(256), pb, stream, map, primes, R);
         return;
       }
     }
@@ -1001,7 +1063,19 @@ static hipError_t dispatch(const TowerMap &map, const PrimeConst *primes, uint32
   const uint32_t pad = lds_pad_for((uint32_t)(arith >> kWgCapShift) & 15u);
   if (row16 && h != 0) return hipErrorInvalidValue;
   if (mode == kLazy) {
-    if (inverse) return launch_t<true, kLazy>(m, primes, log_n, count, stream, row16, pad);
+    // both directions take the class H of the launch's widest modulus: the
+    // forward and inverse reduction schedules (make_fwd_plan, make_inv_plan)
+    if (inverse) {
+      switch (h) {
+        case 0: return launch_t<true, kLazy, 0>(m, primes, log_n, count, stream, row16, pad);
+        case 1: return launch_t<true, kLazy, 1>(m, primes, log_n, count, stream, row16, pad);
+        case 2: return launch_t<true, kLazy, 2>(m, primes, log_n, count, stream, row16, pad);
+        case 3: return launch_t<true, kLazy, 3>(m, primes, log_n, count, stream, row16, pad);
+        case 4: return launch_t<true, kLazy, 4>(m, primes, log_n, count, stream, row16, pad);
+        case 5: return launch_t<true, kLazy, 5>(m, primes, log_n, count, stream, row16, pad);
+        default: return hipErrorInvalidValue;
+      }
+    }
     switch (h) {
       case 0: return launch_t<false, kLazy, 0>(m, primes, log_n, count, stream, row16, pad);
       case 1: return launch_t<false, kLazy, 1>(m, primes, log_n, count, stream, row16, pad);

@@ -27,6 +27,10 @@ void check(bool ok, const std::string &why) {
 }
 uint64_t mulmod(uint64_t a, uint64_t b, uint64_t q) { return (uint64_t)((u128)a * b % q); }
 uint64_t shoup_companion(uint64_t w, uint64_t q) { return (uint64_t)(((u128)w << 64) / q); }
+uint64_t shoup_lazy_host(uint64_t x, uint64_t w, uint64_t wp, uint64_t q) {
+  return x * w - (uint64_t)(((u128)x * wp) >> 64) * q;
+}
+uint64_t reduce_once(uint64_t x, uint64_t q) { return x >= q ? x - q : x; }
 // reduce2_est's per-prime constants, as the library computes them
 void est_constants(uint64_t q, uint32_t &shift, uint32_t &mul) {
   const int bits = 64 - __builtin_clzll(q);
@@ -162,9 +166,93 @@ void planned_forward_transforms(std::mt19937_64 &rng) {
   }
 }
 
+// The inverse reduction schedule of every headroom class H (make_inv_plan),
+// run over a whole 2^16-point Gentleman-Sande transform with the largest NTT
+// prime of the class: d = x + 2^c q - y never wraps and stays below 2^64, the
+// sums and products stay below the plan's bound for the next stage, every
+// estimate's input is below 2^7 q, and the last stage (n^-1 folded in, its
+// outputs reduced by the exact Shoup product and one subtract) returns the
+// exact inverse transform.
+void planned_inverse_transforms(std::mt19937_64 &rng) {
+  const int n = 1 << 16, lg = 16;
+  const int estimates[6] = {0, 7, 4, 3, 2, 2};
+  for (int H = 0; H <= 5; ++H) {
+    const InvPlanData plan = make_inv_plan(H);
+    int est = 0, sub = 0;
+    for (int g = 0; g < 16; ++g) est += plan.red[g] == 2, sub += plan.red[g] == 1;
+    check(est == estimates[H] && (H == 0 ? sub == 14 : sub == 0), "reductions of inverse plan " + std::to_string(H));
+    uint64_t q = (uint64_t{1} << (61 - H)) - (uint64_t{1} << 17) + 1;
+    while (!is_prime(q)) q -= uint64_t{1} << 17;
+    uint64_t psi = 0;
+    for (uint64_t g = 2; !psi; ++g) {
+      const uint64_t z = powmod(g, (q - 1) / (2 * n), q);
+      if (powmod(z, n, q) == q - 1) psi = z;
+    }
+    const uint64_t psi_inv = powmod(psi, 2 * (uint64_t)n - 1, q);
+    std::vector<uint64_t> tw(n), twp(n);
+    for (uint32_t e = 0; e < (uint32_t)n; ++e) tw[brev(e, lg)] = powmod(psi_inv, e, q);
+    for (int k = 0; k < n; ++k) twp[k] = shoup_companion(tw[k], q);
+    const uint64_t ninv = powmod(n, q - 2, q), last_w = mulmod(tw[1], ninv, q);
+    const uint64_t ninv_p = shoup_companion(ninv, q), last_wp = shoup_companion(last_w, q);
+    uint32_t shift, mul;
+    est_constants(q, shift, mul);
+    const Lazy8 m = lazy8_of(q);
+    for (int pattern = 0; pattern < 2; ++pattern) {
+      std::vector<uint64_t> a(n), want(n);
+      for (int i = 0; i < n; ++i) a[i] = pattern == 0 ? q - 1 : rng() % q;
+      want = a;
+      for (int g = 0; g < lg; ++g) {
+        const int d = 1 << g, groups = n >> (g + 1);
+        const uint64_t c = q << plan.c_log[g];
+        const u128 in_bound = (u128)c;  // inputs below 2^c_log q
+        for (int grp = 0; grp < groups; ++grp) {
+          const uint64_t w = tw[groups + grp], wp = twp[groups + grp];
+          for (int j = 0; j < d; ++j) {
+            const int i = 2 * grp * d + j;
+            const uint64_t x = a[i], y = a[i + d];
+            check((u128)x < in_bound && (u128)y < in_bound, "inverse inputs within 2^c_log q");
+            const u128 s = (u128)x + y, dd = (u128)x + c - y;
+            check(s < ((u128)1 << 64) && dd < ((u128)1 << 64), "inverse sum and difference fit 64 bits");
+            if (g == lg - 1) {
+              // exact reference: (x + y) n^-1, (x - y) W n^-1
+              const uint64_t ex = mulmod((want[i] + want[i + d]) % q, ninv, q);
+              const uint64_t ey = mulmod((want[i] + q - want[i + d]) % q, last_w, q);
+              want[i] = ex;
+              want[i + d] = ey;
+              a[i] = reduce_once(shoup_lazy_host((uint64_t)s, ninv, ninv_p, q), q);
+              a[i + d] = reduce_once(shoup_lazy_host((uint64_t)dd, last_w, last_wp, q), q);
+              continue;
+            }
+            const uint64_t ex = (want[i] + want[i + d]) % q, ey = mulmod((want[i] + q - want[i + d]) % q, w, q);
+            want[i] = ex;
+            want[i + d] = ey;
+            uint64_t xs = (uint64_t)s;
+            if (plan.red[g] == 2) {
+              check(s < ((u128)q << 7), "estimate input below 2^7 q");
+              xs = reduce2_est(xs, shift, mul, m.nq);
+              check(xs < 2 * q, "estimate output below 2q");
+            } else if (plan.red[g] == 1) {
+              xs = sub_if_ge_neg(xs, m.nq4);
+            }
+            const uint64_t ys = shoup4((uint64_t)dd, w, wp, m);
+            check(ys < 4 * q, "inverse product below 4q");
+            check(xs % q == ex && ys % q == ey, "planned inverse butterfly congruent");
+            a[i] = xs;
+            a[i + d] = ys;
+          }
+        }
+      }
+      for (int i = 0; i < n; ++i)
+        if (a[i] != want[i]) check(false, "planned inverse equals the exact transform, H = " + std::to_string(H));
+      ++checks;
+    }
+  }
+}
+
 int main() {
   std::mt19937_64 rng(0x7251d9);
   planned_forward_transforms(rng);
+  planned_inverse_transforms(rng);
   // OpenFHE-shaped 60- and 50-bit primes, a prime just below 2^61 and tiny ones
   const uint64_t primes[] = {1152921504606584833ULL, 1125899906826241ULL, 2305843009213693951ULL,
                              1152921504598720513ULL, 562949953216513ULL,  65537ULL, 17ULL};

@@ -132,7 +132,9 @@ int ntt_hip_transform_batch(ntt_hip_context *ctx, uint64_t *data, const uint64_t
  * tower counts, levels or parameter sets that share the context's primes run
  * together. The host array is checked (tower indices) and staged into
  * `jobs_workspace` (device, count * sizeof(ntt_hip_tower_job) bytes) on
- * `stream`, which must stay untouched until the launch has run. */
+ * `stream`, which must stay untouched until the launch has run. Not for
+ * stream capture: the staging copy reads the caller's host array, which is
+ * only guaranteed during the call. */
 typedef struct ntt_hip_tower_job {
   uint64_t offset, src_offset; /* elements */
   uint32_t tower, reserved;
@@ -150,7 +152,8 @@ int ntt_hip_transform_jobs(ntt_hip_context *ctx, uint64_t *data, const uint64_t 
  * checked on the host and staged into `jobs_workspace` (device,
  * ntt_hip_transform_mixed_workspace_bytes(count) bytes) on `stream` (NULL:
  * the first job's context stream), which must stay untouched until the
- * launch has run. */
+ * launch has run. Not for stream capture (the table is staged from a host
+ * array that lives only during the call). */
 typedef struct ntt_hip_mixed_job {
   ntt_hip_context *ctx;
   uint64_t *data;
@@ -221,7 +224,10 @@ int ntt_hip_keyswitch_run(ntt_hip_keyswitch *ks, const uint64_t *in, size_t batc
  * pointer arrays are host arrays of device pointers, staged behind the
  * workspace on `stream`; the workspace holds
  * ntt_hip_keyswitch_gather_workspace_bytes(ks, batch) bytes. Outputs equal
- * ntt_hip_keyswitch_run on the same components gathered into one buffer. */
+ * ntt_hip_keyswitch_run on the same components gathered into one buffer.
+ * Not for stream capture (the pointer tables are staged from host arrays
+ * that live only during the call; ntt_hip_keyswitch_capture records the
+ * contiguous form). */
 int ntt_hip_keyswitch_gather_workspace_bytes(ntt_hip_keyswitch *ks, size_t batch, size_t *bytes);
 int ntt_hip_keyswitch_run_gather(ntt_hip_keyswitch *ks, const uint64_t *const *in, size_t batch,
                                  uint64_t *const *out_b, uint64_t *const *out_a, void *workspace, void *stream);

@@ -1833,7 +1833,9 @@ int ntt_hip_keyswitch_autotune(ntt_hip_keyswitch *ks, size_t batch, char *report
     DeviceBlob io(c.device, 3 * batch * comp * sizeof(uint64_t));
     DeviceBlob ws(c.device, batch * ks->per_component() * sizeof(uint64_t));
     uint64_t *in = (uint64_t *)io.ptr, *ob = in + batch * comp, *oa = ob + batch * comp;
-    HIP_CHECK(hipMemsetAsync(in, 0, batch * comp * sizeof(uint64_t), c.stream));
+    // uniform residues below every modulus (canonical in every tower)
+    const uint64_t min_q = *std::min_element(c.q.begin(), c.q.end());
+    hip_ok(launch_fill_uniform(in, batch * comp, 0x7251d9, min_q, c.stream), "tune operands");
     auto run_ms = [&] {
       return time_ms(c, [&] {
         const int st = ntt_hip_keyswitch_run(ks, in, batch, ob, oa, ws.ptr, c.stream);
@@ -1904,7 +1906,8 @@ int ntt_hip_autotune(ntt_hip_context *ctx, size_t towers, char *report, size_t c
     const size_t a_words = std::max(towers, cpolys * cs) * n, b_words = std::max<size_t>(1, cpolys * ct) * n;
     DeviceBlob scratch(c.device, (a_words + b_words) * sizeof(uint64_t));
     uint64_t *a = (uint64_t *)scratch.ptr, *b = a + a_words;
-    HIP_CHECK(hipMemsetAsync(a, 0, (a_words + b_words) * sizeof(uint64_t), c.stream));
+    const uint64_t min_q = *std::min_element(c.q.begin(), c.q.end());
+    hip_ok(launch_fill_uniform(a, a_words + b_words, 0x7251d9, min_q, c.stream), "tune operands");
     const TowerMap m = affine(a, nullptr, n, 1, 0, n, 0);
     // one forward and one inverse transform as a standalone transform runs
     // them (the context's chunking)

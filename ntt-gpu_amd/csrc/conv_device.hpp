@@ -99,4 +99,5 @@ hipError_t launch_rotate(const RotateLaunch &r, uint32_t towers, uint32_t polys,
 
 namespace nttamd {
 hipError_t launch_copy_words(uint64_t *dst, const uint64_t *src, uint64_t words, hipStream_t stream);
+hipError_t launch_fill_uniform(uint64_t *dst, uint64_t words, uint64_t seed, uint64_t bound, hipStream_t stream);
 }

@@ -412,4 +412,25 @@ hipError_t launch_copy_words(uint64_t *dst, const uint64_t *src, uint64_t words,
   hipLaunchKernelGGL(copy_words_kernel, dim3((uint32_t)blocks), dim3(256), 0, stream, dst, src, words);
   return hipGetLastError();
 }
+
+// Seeded uniform words below `bound` (splitmix64 of the index, scaled by a
+// 64x64 high product): the autotuners' operands. Timing on zero operands
+// measures a different device: the batched key switch runs 6% faster on
+// zeros than on uniform residues (the multipliers draw less power).
+__global__ __launch_bounds__(256) void fill_uniform_kernel(uint64_t *__restrict__ dst, uint64_t words, uint64_t seed,
+                                                           uint64_t bound) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < words; i += (uint64_t)gridDim.x * 256) {
+    uint64_t z = seed + (i + 1) * 0x9e3779b97f4a7c15ULL;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    z ^= z >> 31;
+    dst[i] = __umul64hi(z, bound);
+  }
+}
+
+hipError_t launch_fill_uniform(uint64_t *dst, uint64_t words, uint64_t seed, uint64_t bound, hipStream_t stream) {
+  const uint64_t blocks = std::min<uint64_t>((words + 255) / 256, 65536);
+  hipLaunchKernelGGL(fill_uniform_kernel, dim3((uint32_t)blocks), dim3(256), 0, stream, dst, words, seed, bound);
+  return hipGetLastError();
+}
 }  // namespace nttamd

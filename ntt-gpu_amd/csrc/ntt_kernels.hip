@@ -66,6 +66,18 @@ __device__ __forceinline__ void xchg_cols_p1_p2(V (&v)[16], V *lds, int cl, int 
   for (int k = 0; k < 16; ++k) v[k] = w[k];
 }
 
+// P1 -> P2 in one round through a full 16 x 256 buffer of 32-bit words (the
+// packed forward column pass): every thread writes its j = u + 16k, then
+// reads j = 16 u2 + k for its own u2.
+__device__ __forceinline__ void xchg_cols_full(uint32_t (&v)[16], uint32_t *lds, int cl, int u, int u2) {
+#pragma unroll
+  for (int k = 0; k < 16; ++k) lds[lds_idx(cl, u + 16 * k)] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = lds[lds_idx(cl, 16 * u2 + k)];
+  __syncthreads();
+}
+
 // P2 -> P1: waves with u / 8 == h write j = 16u + k, every thread reads its
 // k in [8h, 8h + 8).
 template <class V>
@@ -402,6 +414,20 @@ __device__ __forceinline__ uint64_t *epi_dest(const TowerMap &m, uint32_t k, uin
          (uint64_t)k * m.tower_stride;
 }
 
+// ---------------------------------------------------------------- packed 32-bit hand-off
+// Towers below 2^30 at N = 2^16 hand their values from one pass to the next
+// as 32-bit words packed into the first half of the tower: element (row r,
+// column c) at 32-bit index packed32(r, c), so the 128-byte segment of row
+// r < 128 and column block c / 16 holds rows r and r + 128 of those 16
+// columns. Every segment is written whole by one column-pass workgroup (its
+// own 16 columns, read by it before) and by one row-pass wave (the pair of
+// rows r, r + 128, which it reads before writing), so the packing stays in
+// place without races, and each pass moves 256 KiB of hand-off per tower
+// instead of 512 KiB.
+__device__ __forceinline__ uint32_t packed32(uint32_t r, uint32_t c) {
+  return ((r & 127u) << 9) | ((c >> 4) << 5) | ((r >> 7) << 4) | (c & 15u);
+}
+
 // ---------------------------------------------------------------- pass A, R = 256
 // Inverse column-pass stages 12 + S .. 14 (P1 layout, distances 1, 2, 4;
 // uniform twiddles through scalar loads).
@@ -421,9 +447,12 @@ __device__ __forceinline__ void a256_inv_tail(Val<A> (&v)[16], TwPtr tw, const L
 
 // Column block bx (16 columns of 256) of one tower: data in place, src the
 // forward input (== data unless the first pass reads elsewhere).
-template <bool Inverse, int A, int H>
+// Pk (32-bit towers only): the hand-off to or from the row pass is packed
+// (packed32) instead of one 64-bit word per residue.
+template <bool Inverse, int A, int H, bool Pk = false>
 __device__ __forceinline__ void a256_body(uint64_t *data, const uint64_t *src, const PrimeConst &pc, uint32_t bx,
                                           Val<A> *lds, tw2 *twl) {
+  static_assert(!Pk || A == kNarrow, "the packed hand-off holds 32-bit residues");
   const TwPtr tw = tw_table(Inverse ? pc.inv : pc.fwd);
   const Lazy8 m = lazy8_of(pc.q, pc.nq, pc.nq2, pc.nq4);
   const int t = threadIdx.x, cl = t & 15, u = t >> 4;
@@ -440,13 +469,31 @@ __device__ __forceinline__ void a256_body(uint64_t *data, const uint64_t *src, c
     twl[t] = mine;  // ordered before its readers by the exchange's barriers
     const NegMultiples neg = neg_multiples(m);
     reg_stages_fwd<A, H, 0>(v, tw, 1, 1, 0, m, neg);
-    xchg_cols_p1_p2(v, lds, cl, u);
-    reg_stages_fwd<A, H, 4>(v, twp, 16, 1, u, m, neg);
+    if constexpr (Pk) {
+      // second layout with the row groups u2 and u2 + 8 in one wave (u2 = wave
+      // + 4 x lane group), so that each packed store writes both halves of its
+      // 128-byte segments (rows r and r + 128) at once, whole lines
+      const int u2 = (t >> 6) + 4 * ((t >> 4) & 3);
+      xchg_cols_full(v, lds, cl, u, u2);
+      reg_stages_fwd<A, H, 4>(v, twp, 16, 1, u2, m, neg);
+      uint32_t *h = reinterpret_cast<uint32_t *>(data);
 #pragma unroll
-    for (int k = 0; k < 16; ++k) data[(uint64_t)(16 * u + k) * kRow + c] = v[k];
+      for (int k = 0; k < 16; ++k) h[packed32(16 * u2 + k, c)] = (uint32_t)v[k];
+    } else {
+      xchg_cols_p1_p2(v, lds, cl, u);
+      reg_stages_fwd<A, H, 4>(v, twp, 16, 1, u, m, neg);
+#pragma unroll
+      for (int k = 0; k < 16; ++k) data[(uint64_t)(16 * u + k) * kRow + c] = v[k];
+    }
   } else {
+    if constexpr (Pk) {
+      const uint32_t *h = reinterpret_cast<const uint32_t *>(data);
 #pragma unroll
-    for (int k = 0; k < 16; ++k) v[k] = data[(uint64_t)(16 * u + k) * kRow + c];
+      for (int k = 0; k < 16; ++k) v[k] = (Val<A>)h[packed32(16 * u + k, c)];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) v[k] = data[(uint64_t)(16 * u + k) * kRow + c];
+    }
     twl[t] = mine;
     __syncthreads();
     reg_stages_inv<A, H, 8>(v, twp, 16, 1, u, m, pc);
@@ -461,16 +508,16 @@ __device__ __forceinline__ void a256_body(uint64_t *data, const uint64_t *src, c
   }
 }
 
-template <bool Inverse, int A, int H = 0>
+template <bool Inverse, int A, int H = 0, bool Pk = false>
 __global__ __launch_bounds__(256) void ntt_pass_a256(TowerMap map, const PrimeConst *__restrict__ primes) {
-  __shared__ Val<A> lds[16 * kHalfRow];
+  __shared__ Val<A> lds[(Pk && !Inverse) ? 16 * kLdsRow : 16 * kHalfRow];
   __shared__ tw2 twl[256];
   uint64_t *data;
   const uint64_t *src;
   uint32_t prime, tk, tpoly;
   tower_of(map, blockIdx.y, !Inverse, data, src, prime, tk, tpoly);
   const PrimeConst pc = primes[prime];
-  a256_body<Inverse, A, H>(data, src, pc, blockIdx.x, lds, twl);
+  a256_body<Inverse, A, H, Pk>(data, src, pc, blockIdx.x, lds, twl);
 }
 
 // ---------------------------------------------------------------- pass B (rows of 256)
@@ -613,6 +660,29 @@ __device__ __forceinline__ void b8_store_w(uint64_t *row, int u, const uint64_t 
   for (int k = 0; k < 8; ++k) row[b8_pos<kLayW>(u, k)] = o[k];
 }
 
+// Row r of the packed hand-off in layout W (tower base `data`): slots 2a and
+// 2a + 1 are adjacent words of one 16-column block.
+__device__ __forceinline__ void b8_load_packed(uint32_t (&v)[8], const uint64_t *data, uint32_t r, int u) {
+  const uint32_t *h = reinterpret_cast<const uint32_t *>(data);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) v[k] = h[packed32(r, (uint32_t)b8_pos<kLayW>(u, k))];
+}
+__device__ __forceinline__ void b8_store_packed(uint64_t *data, uint32_t r, int u, const uint32_t (&v)[8]) {
+  uint32_t *h = reinterpret_cast<uint32_t *>(data);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) h[packed32(r, (uint32_t)b8_pos<kLayW>(u, k))] = v[k];
+}
+
+// Row of thread-row rl (0..7) in workgroup bx of a row pass: rows 8 bx .. 8 bx
+// + 7, or with the packed hand-off the row pairs (4 bx + w, 4 bx + w + 128) of
+// its four waves w (a wave's 64 lanes then cover both halves of the packed
+// segments of row 4 bx + w).
+template <bool Pk>
+__device__ __forceinline__ uint32_t b8_row_of(uint32_t bx, int rl) {
+  if constexpr (Pk) return bx * 4 + (uint32_t)(rl >> 1) + ((uint32_t)(rl & 1) << 7);
+  else return bx * 8 + (uint32_t)rl;
+}
+
 // Move v from layout From to layout To through the row's LDS buffer.
 template <int From, int To, class V>
 __device__ __forceinline__ void b8_xchg(V (&v)[8], V *buf, int u) {
@@ -678,11 +748,13 @@ __device__ __forceinline__ void b8_inv_stages(Val<A> (&v)[8], TwPtr tw, TwLds tr
 
 // One row (row index rf - R) of a workgroup: loads in W and all 8 forward
 // stages; v ends in layout F.
-template <int A, int H>
+template <int A, int H, bool Pk = false>
 __device__ __forceinline__ void b8_fwd_core(Val<A> (&v)[8], const uint64_t *row, TwPtr tw, uint32_t rf, int u,
-                                            const Lazy8 &m, const NegMultiples &neg, Val<A> *buf, tw2 *trow) {
+                                            const Lazy8 &m, const NegMultiples &neg, Val<A> *buf, tw2 *trow,
+                                            const uint64_t *tower = nullptr, uint32_t r = 0) {
   b8_stage_tw(trow, tw, rf, u);
-  b8_load_w<A>(v, row, u);
+  if constexpr (Pk) b8_load_packed(v, tower, r, u);
+  else b8_load_w<A>(v, row, u);
   wave_sync8();
   const TwLds tws = (TwLds)(const void *)trow;
   b8_fwd_stages<A, H, 0, 2>(v, tw, tws, rf, u, m, neg);
@@ -705,8 +777,9 @@ __device__ __forceinline__ void b8_fwd_store(Val<A> (&v)[8], uint64_t *row, int 
   b8_store_w(row, u, o);
 }
 
-template <int A, bool Epi, int H = 0>
+template <int A, bool Epi, int H = 0, bool Pk = false>
 __global__ __launch_bounds__(256) void ntt_pass_b8_fwd(TowerMap map, const PrimeConst *__restrict__ primes, uint32_t R) {
+  static_assert(!Pk || (A == kNarrow && !Epi), "the packed hand-off: 32-bit towers without an epilogue");
   __shared__ Val<A> lds[8 * kLdsRow];
   __shared__ tw2 tws[8 * (1 << kB8Staged)];
   uint64_t *data;
@@ -717,13 +790,13 @@ __global__ __launch_bounds__(256) void ntt_pass_b8_fwd(TowerMap map, const Prime
   const TwPtr tw = tw_table(pc.fwd);
   const Lazy8 m = lazy8_of(pc.q, pc.nq, pc.nq2, pc.nq4);
   const int t = threadIdx.x, u = t & 31, rl = t >> 5;
-  const uint32_t r = blockIdx.x * 8 + rl;
+  const uint32_t r = b8_row_of<Pk>(blockIdx.x, rl);
   uint64_t *row = data + (uint64_t)r * kRow;
   const uint32_t rf = R + r;
   Val<A> *buf = lds + rl * kLdsRow;
   Val<A> v[8];
   const NegMultiples neg = neg_multiples(m);
-  b8_fwd_core<A, H>(v, row, tw, rf, u, m, neg, buf, tws + rl * (1 << kB8Staged));
+  b8_fwd_core<A, H, Pk>(v, row, tw, rf, u, m, neg, buf, tws + rl * (1 << kB8Staged), data, r);
   if constexpr (Epi) {
     b8_xchg<kLayF, kLayW>(v, buf, u);
     const uint64_t *sub;
@@ -750,13 +823,14 @@ __global__ __launch_bounds__(256) void ntt_pass_b8_fwd(TowerMap map, const Prime
 // transform): load in W, stages of distance 1, 2, 4 in F, 8, 16, 32 in M,
 // 64, 128 in W, store from W. One row per 32 threads (row 8 bx + thread /
 // 32): reads src, writes data.
-template <int A, int H>
+template <int A, int H, bool Pk = false>
 __device__ __forceinline__ void b8_inv_body(uint64_t *data, const uint64_t *src, const PrimeConst &pc, uint32_t R,
                                             uint32_t bx, Val<A> *lds, tw2 *tws) {
+  static_assert(!Pk || A == kNarrow, "the packed hand-off holds 32-bit residues");
   const TwPtr tw = tw_table(pc.inv);
   const Lazy8 m = lazy8_of(pc.q, pc.nq, pc.nq2, pc.nq4);
   const int t = threadIdx.x, u = t & 31, rl = t >> 5;
-  const uint32_t r = bx * 8 + rl;
+  const uint32_t r = b8_row_of<Pk>(bx, rl);
   uint64_t *row = data + (uint64_t)r * kRow;
   const uint64_t *src_row = src + (uint64_t)r * kRow;
   const uint32_t rf = R + r;
@@ -772,13 +846,19 @@ __device__ __forceinline__ void b8_inv_body(uint64_t *data, const uint64_t *src,
   b8_inv_stages<A, H, 3, 6>(v, tw, twr, rf, u, m, pc);
   b8_xchg<kLayM, kLayW>(v, buf, u);
   b8_inv_stages<A, H, 6, 8>(v, tw, twr, rf, u, m, pc);
-  uint64_t o[8];
+  if constexpr (Pk) {
+    // the wave has read rows r and r +- 128 (all the packed segments of its
+    // pair) before any of its lanes writes them: the stores depend on every load
+    b8_store_packed(data, r, u, v);
+  } else {
+    uint64_t o[8];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) o[k] = v[k];
-  b8_store_w(row, u, o);
+    for (int k = 0; k < 8; ++k) o[k] = v[k];
+    b8_store_w(row, u, o);
+  }
 }
 
-template <int A, int H = 0>
+template <int A, int H = 0, bool Pk = false>
 __global__ __launch_bounds__(256) void ntt_pass_b8_inv(TowerMap map, const PrimeConst *__restrict__ primes, uint32_t R) {
   __shared__ Val<A> lds[8 * kLdsRow];
   __shared__ tw2 tws[8 * (1 << kB8Staged)];
@@ -787,7 +867,7 @@ __global__ __launch_bounds__(256) void ntt_pass_b8_inv(TowerMap map, const Prime
   uint32_t prime, tk, tpoly;
   tower_of(map, blockIdx.y, true, data, src, prime, tk, tpoly);
   const PrimeConst pc = primes[prime];
-  b8_inv_body<A, H>(data, src, pc, R, blockIdx.x, lds, tws);
+  b8_inv_body<A, H, Pk>(data, src, pc, R, blockIdx.x, lds, tws);
 }
 
 // ---------------------------------------------------------------- generic LDS kernel
@@ -1001,8 +1081,19 @@ static hipError_t launch_t(const TowerMap &map, const PrimeConst *primes, uint32
     return hipGetLastError();
   }
   const uint32_t R = n / kRow, log_r = log_n - 8;
+  // 32-bit towers at N = 2^16 with the 8-residue row pass hand their values
+  // over packed (packed32), unless the forward's row pass carries the ModDown
+  // epilogue; both passes of the launch agree on the form.
+  const bool packed = A == kNarrow && R == 256 && !row16 && (Inverse || !map.epi.sub);
   auto pass_a = [&]() {
     if (R == 256) {
+      if constexpr (A == kNarrow) {
+        if (packed) {
+          hipLaunchKernelGGL((ntt_pass_a256<Inverse, A, H, true>), dim3(kRow / 16, towers), dim3(256), pad, stream,
+                             map, primes);
+          return;
+        }
+      }
       hipLaunchKernelGGL((ntt_pass_a256<Inverse, A, H>), dim3(kRow / 16, towers), dim3(256), pad,
                          stream, map, primes);
     } else {
@@ -1015,12 +1106,26 @@ static hipError_t launch_t(const TowerMap &map, const PrimeConst *primes, uint32
   auto pass_b = [&]() {
     if constexpr (Inverse && (A == kLazy || A == kNarrow)) {
       if (R == 256 && !row16) {
+        if constexpr (A == kNarrow) {
+          if (packed) {
+            hipLaunchKernelGGL((ntt_pass_b8_inv<A, H, true>), dim3(R / 8, towers), dim3(256), pb, stream, map, primes,
+                               R);
+            return;
+          }
+        }
         hipLaunchKernelGGL((ntt_pass_b8_inv<A, H>), dim3(R / 8, towers), dim3(256), pb, stream, map, primes, R);
         return;
       }
     }
     if constexpr (!Inverse && (A == kLazy || A == kNarrow)) {
       if (R == 256 && !row16) {
+        if constexpr (A == kNarrow) {
+          if (packed) {
+            hipLaunchKernelGGL((ntt_pass_b8_fwd<A, false, H, true>), dim3(R / 8, towers), dim3(256), pb, stream, map,
+                               primes, R);
+            return;
+          }
+        }
         if (map.epi.sub)
           hipLaunchKernelGGL((ntt_pass_b8_fwd<A, true, H>), dim3(R / 8, towers), dim3(256), pb, stream, map, primes,
                              R);

@@ -175,6 +175,46 @@ def test_narrow_primes(n):
         assert np.array_equal(_gpu_round(ctx, x, 0, T, True), o.ntt_towers(x, n, qs, rs, inverse=True))
 
 
+def test_narrow_packed_handoff_out_of_place_and_strided():
+    """32-bit towers at N = 2^16 hand their values between the passes packed in
+    the first half of each tower: out of place in both directions (the source
+    stays untouched), and a batch of polynomials whose stride leaves a gap
+    tower after each one (the gaps keep their contents), all against the
+    oracle."""
+    n = N16
+    qs = [params.last_prime(30, 2 * n)] + params.find_primes(n, 2, 30)
+    rs = [params.find_psi(q, n) for q in qs]
+    ctx = hip.Context(n, qs, rs)
+    T = len(qs)
+    x = uniform(n, qs, 31)
+    want = o.ntt_towers(x, n, qs, rs)
+    log_case("ntt narrow packed hand-off, out of place")
+    src, dst, back = hip.DeviceBuffer(x.size).upload(x), hip.DeviceBuffer(x.size), hip.DeviceBuffer(x.size)
+    ctx.transform(dst.ptr, 0, T, False, src_ptr=src.ptr)
+    ctx.synchronize()
+    assert np.array_equal(dst.download(), want)
+    assert np.array_equal(src.download(), x)
+    ctx.transform(back.ptr, 0, T, True, src_ptr=dst.ptr)
+    ctx.synchronize()
+    assert np.array_equal(back.download(), x)
+    assert np.array_equal(dst.download(), want)
+    log_case("ntt narrow packed hand-off, strided batch with gap towers")
+    P, stride = 4, (T + 1) * n
+    polys = [uniform(n, qs, 40 + p) for p in range(P)]
+    gap = np.arange(n, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    buf = np.concatenate([np.concatenate([polys[p], gap]) for p in range(P)])
+    d = hip.DeviceBuffer(buf.size).upload(buf)
+    ctx.transform_batch(d.ptr, P, stride, 0, T, False)
+    ctx.synchronize()
+    got = d.download()
+    for p in range(P):
+        assert np.array_equal(got[p * stride:p * stride + T * n], o.ntt_towers(polys[p], n, qs, rs)), f"poly {p}"
+        assert np.array_equal(got[p * stride + T * n:(p + 1) * stride], gap), f"gap after poly {p}"
+    ctx.transform_batch(d.ptr, P, stride, 0, T, True)
+    ctx.synchronize()
+    assert np.array_equal(d.download(), buf)
+
+
 def test_pointwise_every_width():
     """Barrett variable x variable products (lazy form below 2^62, exact form
     up to 2^63) on primes of 30, 50, 60, 61.9, 62.5 and ~63 bits: all-(q-1),

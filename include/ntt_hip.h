@@ -108,9 +108,22 @@ int ntt_hip_get_tuning(ntt_hip_context *ctx, uint32_t *ntt_chunk, uint32_t *conv
 int ntt_hip_set_ntt_shape(ntt_hip_context *ctx, int fwd_headroom, uint32_t row_residues, uint32_t wg_per_cu);
 /* The same search timed on `batch` fused key switches of the plan (its key
  * uploaded), over the context's NTT shape and conversion width and the
- * plan's concurrent lanes and MAC/transform interleave group; allocates its
- * own buffers for the duration of the call. */
+ * plan's concurrent lanes, ModUp/MAC fusion and MAC/transform interleave
+ * group; allocates its own buffers for the duration of the call. */
 int ntt_hip_keyswitch_autotune(ntt_hip_keyswitch *ks, size_t batch, char *report, size_t capacity);
+/* The plan's key-switch shape: lanes 1..8 (concurrent sub-batches),
+ * mac_group (towers per ModUp/MAC interleave group, 0 = all at once) and
+ * fuse_mac: 0 = the ModUp's last NTT pass writes the digits and the key inner
+ * product reads them back, 1 = that pass and the inner product run as one
+ * kernel with 128-bit sums, 2 = the same with 64-bit sums (defaults: lanes
+ * 2, mac_group 0, fuse_mac 2). The fusion applies at N = 2^16 with moduli
+ * below 2^61, the 8-residue row pass and sums that fit (128-bit: digits * q
+ * < 2^64; 64-bit: 2 q times the digit count rounded up to a power of two <=
+ * 2^64); otherwise the unfused path runs. Every shape is bit-exact. `fused`
+ * reports whether the fused kernel applies to the plan as set. */
+int ntt_hip_keyswitch_set_shape(ntt_hip_keyswitch *ks, uint32_t lanes, uint32_t mac_group, uint32_t fuse_mac);
+int ntt_hip_keyswitch_get_shape(ntt_hip_keyswitch *ks, uint32_t *lanes, uint32_t *mac_group, uint32_t *fuse_mac,
+                                int *fused);
 int ntt_hip_get_ntt_shape(ntt_hip_context *ctx, int *fwd_headroom, uint32_t *row_residues, uint32_t *wg_per_cu);
 
 /* ---- Driver seam operations (include/ntt/resident_driver.hpp:13-57) on

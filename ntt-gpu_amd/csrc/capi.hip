@@ -417,10 +417,22 @@ struct ntt_hip_keyswitch {
   size_t jobs_per_poly = 0;
   std::vector<uint32_t> tower_job_begin;          // [T + 1]: jobs of tower t are [begin[t], begin[t + 1])
   // ModUp complement transforms and the key inner product interleaved by
-  // groups of this many towers (0: one transform launch, then one MAC), so
-  // the MAC reads a group's digits while they are still in the Infinity Cache.
+  // groups of this many towers (0: all towers at once), so the MAC reads a
+  // group's digits while they are still in the Infinity Cache. Measured at
+  // N = 2^16, L = 24, dnum = 4, batch 64: groups of 2 made the unfused path
+  // 1.25% faster and the fused one 0.5% slower.
   uint32_t mac_group = kDefaultMacGroup;
-  static constexpr uint32_t kDefaultMacGroup = 2;
+  static constexpr uint32_t kDefaultMacGroup = 0;
+  // ModUp's row pass fused with the key inner product (launch_modup_mac_rows)
+  // where it applies: 0 off (row pass, then the MAC reads the digits back),
+  // 1 with 128-bit sums, 2 with 64-bit sums of half-reduced products. Same
+  // workload: 8605 KeySwitch/s with 64-bit sums, 7812 unfused, 7559 with
+  // 128-bit sums (their 64 accumulator registers spill at 4 waves per SIMD).
+  uint32_t fuse_mac = kDefaultFuseMac;
+  static constexpr uint32_t kDefaultFuseMac = 2;
+  // log2 of the 64-bit sums' digit bound (beta <= 2^mac_sum_log, 2^(mac_sum_log
+  // + 1) max q <= 2^64), or -1 when 64-bit sums do not fit.
+  int mac_sum_log = -1;
   std::unique_ptr<DeviceBlob> key;                // beta*(L+K)*n * 2 (b then a), Montgomery form
   bool key_loaded = false;
   bool mac_per_term = false;
@@ -1049,6 +1061,10 @@ int ntt_hip_keyswitch_create(ntt_hip_keyswitch **out, ntt_hip_context *ctx, size
     u128 bound = 0;
     for (size_t i = 0; i < T; ++i) bound = std::max<u128>(bound, (u128)ctx->q[i] * ks->beta);
     ks->mac_per_term = bound >= ((u128)1 << 64);
+    int sum_log = 0;
+    while ((1u << sum_log) < ks->beta) ++sum_log;
+    const uint64_t max_q = *std::max_element(ctx->q.begin(), ctx->q.end());
+    ks->mac_sum_log = sum_log <= 6 && ((u128)max_q << (sum_log + 1)) <= ((u128)1 << 64) ? sum_log : -1;
     ks->key = std::make_unique<DeviceBlob>(ctx->device, 2 * ks->beta * T * n * sizeof(uint64_t));
     DeviceGuard g(ctx->device);
     HIP_CHECK(hipMemcpy(ks->tower_prime->ptr, tp.data(), T * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -1239,6 +1255,63 @@ void ks_mac(const ntt_hip_keyswitch &key, const ntt_hip_keyswitch &plan, KsIn in
   hip_ok(launch_mac_fused(m, key.mac_per_term, (uint32_t)B, s), "mac");
 }
 
+// The fused ModUp row pass + MAC covers N = 2^16 contexts in the wide-lazy
+// arithmetic with the 8-residue row pass, and sums that fit its accumulators.
+bool fused_mac_applies(const ntt_hip_keyswitch &plan) {
+  const ntt_hip_context &c = *plan.ctx;
+  const int mode = c.ntt_mode();
+  if (plan.fuse_mac == 0 || c.log_n != 16 || (mode & kArithMask) != kLazy || (mode & kRow16Bit)) return false;
+  return plan.fuse_mac == 1 ? !plan.mac_per_term : plan.mac_sum_log >= 0;
+}
+
+// ModUp with its row pass fused into the key inner product of `key` into z:
+// per group of towers, the column pass of their complement transforms, then
+// one launch that finishes them in registers and accumulates both channels.
+void ks_modup_mac_fused(const ntt_hip_keyswitch &key, const ntt_hip_keyswitch &plan, KsIn in, size_t B,
+                        const KsBuffers &w, uint64_t *z, uint32_t G, hipStream_t s) {
+  const ntt_hip_context &c = *plan.ctx;
+  const uint32_t T = plan.L + plan.K;
+  const size_t n = c.n;
+  ks_modup_convert(plan, in, B, w, s);
+  const TowerMap cols = complement_jobs(plan, w);
+  ModUpMacRows mm{};
+  mm.digits = w.digits;
+  mm.own = in.base;
+  mm.own_table = in.table;
+  mm.own_poly_stride = plan.L * n;
+  mm.key_b = (const uint64_t *)key.key->ptr;
+  mm.key_a = mm.key_b + (size_t)plan.beta * T * n;
+  mm.out_b = z;
+  mm.out_a = z + B * plan.z_words();
+  mm.tower_prime = (const uint32_t *)plan.tower_prime->ptr;
+  mm.digit_poly_stride = plan.digit_words();
+  mm.out_poly_stride = plan.z_words();
+  mm.towers = T;
+  mm.alpha = plan.alpha;
+  mm.active_q = plan.L;
+  mm.digits_count = plan.beta;
+  mm.polys = (uint32_t)B;
+  mm.wide = plan.fuse_mac == 1;
+  mm.sum_log = plan.mac_sum_log < 0 ? 0 : (uint32_t)plan.mac_sum_log;
+  for (uint32_t g0 = 0; g0 < T; g0 += G) {
+    const uint32_t g1 = std::min(T, g0 + G);
+    // runs of towers of one headroom class: both passes of a tower take its class
+    for (uint32_t t0 = g0; t0 < g1;) {
+      const int mode = c.ntt_mode_for(t0, 1);
+      uint32_t t1 = t0 + 1;
+      while (t1 < g1 && c.ntt_mode_for(t1, 1) == mode) ++t1;
+      const uint32_t k0 = plan.tower_job_begin[t0], k1 = plan.tower_job_begin[t1];
+      hip_ok(launch_ntt_columns_span(cols, c.dprimes(), (uint32_t)B, k0 * (uint32_t)B, (k1 - k0) * (uint32_t)B, mode,
+                                     s),
+             "modup column pass");
+      mm.first_tower = t0;
+      mm.launch_towers = t1 - t0;
+      hip_ok(launch_modup_mac_rows(mm, c.dprimes(), mode, s), "modup row pass + mac");
+      t0 = t1;
+    }
+  }
+}
+
 // ModUp and the key inner product of `key` into z, interleaved by tower
 // groups (mac_group) when the plan asks for it.
 constexpr size_t kMinGroupBatch = 8;
@@ -1246,6 +1319,10 @@ void ks_modup_mac(const ntt_hip_keyswitch &key, const ntt_hip_keyswitch &plan, K
                   const KsBuffers &w, uint64_t *z, hipStream_t s) {
   const ntt_hip_context &c = *plan.ctx;
   const uint32_t T = plan.L + plan.K, G = plan.mac_group;
+  if (fused_mac_applies(plan)) {
+    ks_modup_mac_fused(key, plan, in, B, w, z, G == 0 || G >= T || B < kMinGroupBatch ? T : G, s);
+    return;
+  }
   if (G == 0 || G >= T || B < kMinGroupBatch) {  // small batches: fewer, larger launches
     ks_modup(plan, in, B, w, s);
     ks_mac(key, plan, in, B, w, z, s);
@@ -1819,10 +1896,35 @@ int ntt_hip_get_ntt_shape(ntt_hip_context *ctx, int *fwd_headroom, uint32_t *row
   });
 }
 
+int ntt_hip_keyswitch_set_shape(ntt_hip_keyswitch *ks, uint32_t lanes, uint32_t mac_group, uint32_t fuse_mac) {
+  return guarded([&] {
+    require(ks != nullptr, "null key switch plan");
+    require(lanes >= 1 && lanes <= 8, "lanes must be 1..8");
+    require(fuse_mac <= 2, "fuse_mac must be 0 (off), 1 (128-bit sums) or 2 (64-bit sums)");
+    ks->lanes = lanes;
+    ks->mac_group = mac_group;
+    ks->fuse_mac = fuse_mac;
+    return NTT_HIP_OK;
+  });
+}
+
+int ntt_hip_keyswitch_get_shape(ntt_hip_keyswitch *ks, uint32_t *lanes, uint32_t *mac_group, uint32_t *fuse_mac,
+                                int *fused) {
+  return guarded([&] {
+    require(ks && lanes && mac_group && fuse_mac && fused, "null argument");
+    *lanes = ks->lanes;
+    *mac_group = ks->mac_group;
+    *fuse_mac = ks->fuse_mac;
+    *fused = fused_mac_applies(*ks) ? 1 : 0;
+    return NTT_HIP_OK;
+  });
+}
+
 // The key switch's own tuner: the same coordinate search with alternating
-// pairs, timed on the batched key switch itself (zero inputs: the arithmetic
-// is data-independent), over the context's NTT shape and chunk, the plan's
-// lanes and MAC/transform interleave group, and the conversion width.
+// pairs, timed on the batched key switch itself (seeded uniform residues: the
+// clock under load depends on the data), over the context's NTT shape and
+// chunk, the plan's lanes, ModUp/MAC fusion and interleave group, and the
+// conversion width.
 int ntt_hip_keyswitch_autotune(ntt_hip_keyswitch *ks, size_t batch, char *report, size_t capacity) {
   return guarded([&] {
     require(ks && ks->key_loaded, "autotune needs a plan with its key uploaded");
@@ -1873,7 +1975,11 @@ int ntt_hip_keyswitch_autotune(ntt_hip_keyswitch *ks, size_t batch, char *report
     }
     const uint32_t lanes[4] = {ntt_hip_keyswitch::kDefaultLanes, 1, 3, 4};
     sweep("lanes", 4, [&](size_t i) { ks->lanes = lanes[i]; }, 0);
-    const uint32_t groups[5] = {ntt_hip_keyswitch::kDefaultMacGroup, 0, 1, 4, 8};
+    const uint32_t fuse[3] = {ntt_hip_keyswitch::kDefaultFuseMac, ntt_hip_keyswitch::kDefaultFuseMac == 0 ? 1u : 0u,
+                              ntt_hip_keyswitch::kDefaultFuseMac == 2 ? 1u : 2u};
+    sweep("fuse_mac", 3, [&](size_t i) { ks->fuse_mac = fuse[i]; }, 0);
+    const uint32_t groups[5] = {ntt_hip_keyswitch::kDefaultMacGroup, ntt_hip_keyswitch::kDefaultMacGroup == 2 ? 0u : 2u,
+                                1, 4, 8};
     sweep("mac_group", 5, [&](size_t i) { ks->mac_group = groups[i]; }, 0);
     const uint32_t targets[6] = {0, 2, 4, 8, 12, 24};
     sweep("conv_targets", 6, [&](size_t i) { c.conv_targets = targets[i]; }, 0);
@@ -1881,7 +1987,8 @@ int ntt_hip_keyswitch_autotune(ntt_hip_keyswitch *ks, size_t batch, char *report
     rep += ",\"tuned_ms\":" + std::to_string(run_ms()) + "}";
     rep += ",\"chosen\":{\"fwd_headroom\":" + std::to_string(c.fwd_headroom) +
            ",\"row_residues\":" + std::to_string(c.row16 ? 16 : 8) + ",\"wg_per_cu\":" + std::to_string(c.wg_cap) +
-           ",\"lanes\":" + std::to_string(ks->lanes) + ",\"mac_group\":" + std::to_string(ks->mac_group) +
+           ",\"lanes\":" + std::to_string(ks->lanes) + ",\"fuse_mac\":" + std::to_string(ks->fuse_mac) +
+           ",\"mac_group\":" + std::to_string(ks->mac_group) +
            ",\"conv_targets\":" + std::to_string(c.conv_targets) + "},\"batch\":" + std::to_string(batch) + "}";
     if (report && capacity) std::snprintf(report, capacity, "%s", rep.c_str());
     return NTT_HIP_OK;

@@ -66,11 +66,18 @@ __host__ __device__ __forceinline__ U128 mul_wide63(uint64_t a, uint64_t b) {
   return {(u << 32) | (uint32_t)p00, hi};
 }
 
+// (S + m q) / 2^64 with m = lo * qinv_neg mod 2^64, which makes the sum a
+// multiple of 2^64 (its low words carry exactly when lo != 0): congruent to
+// S * 2^-64 mod q and below S / 2^64 + q, so below 2q for S < q * 2^64.
+__host__ __device__ __forceinline__ uint64_t mont_half(U128 s, uint64_t q, uint64_t qinv_neg) {
+  const uint64_t m = s.lo * qinv_neg;
+  return s.hi + mul_wide63(m, q).hi + (s.lo != 0 ? 1u : 0u);
+}
+
 // Montgomery reduction: S = hi:lo with S < q * 2^64 -> S * 2^-64 mod q, canonical.
 // qinv_neg = -q^-1 mod 2^64; q < 2^63 (every prime of this library).
 __host__ __device__ __forceinline__ uint64_t mont_reduce(U128 s, uint64_t q, uint64_t qinv_neg) {
-  const uint64_t m = s.lo * qinv_neg;
-  const uint64_t t = s.hi + mul_wide63(m, q).hi + (s.lo != 0 ? 1u : 0u);
+  const uint64_t t = mont_half(s, q, qinv_neg);
   return t >= q ? t - q : t;
 }
 

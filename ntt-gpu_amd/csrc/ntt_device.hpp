@@ -125,4 +125,35 @@ hipError_t launch_ntt_mixed(const MixedJob *jobs, uint32_t count, uint32_t block
 hipError_t launch_ntt_span(const TowerMap &map, const PrimeConst *primes, uint32_t log_n, uint32_t polys, uint32_t y0,
                            uint32_t count, bool inverse, int arith, hipStream_t stream);
 
+// The column pass alone of forward transforms at N = 2^16 in the wide-lazy
+// arithmetic (arith kLazy with its headroom class, 8-residue row pass shape),
+// launch indices [y0, y0 + count) of `map` over `polys` polynomials: the
+// towers are left in the hand-off form that launch_modup_mac_rows reads.
+hipError_t launch_ntt_columns_span(const TowerMap &map, const PrimeConst *primes, uint32_t polys, uint32_t y0,
+                                   uint32_t count, int arith, hipStream_t stream);
+
+// ModUp's row pass fused with the key inner product, N = 2^16, wide-lazy
+// towers. For towers [first_tower, first_tower + launch_towers) of `polys`
+// polynomials: digit d's value of tower t is the row pass (in registers) of
+// its complement tower in `digits`, whose column pass has run, or for the
+// digit that owns t (t < active_q, d = t / alpha) the input tower itself;
+//   out_b[p][t] = sum_d X_d[t] * key_b[d][t],  out_a likewise,
+// keys in Montgomery form. wide: sums kept in 128 bits and reduced once (the
+// caller guarantees digits_count * q < 2^64); otherwise each product is
+// reduced to [0, 2q) into a 64-bit sum (digits_count <= 2^sum_log and
+// 2^(sum_log + 1) q <= 2^64). The digit buffers are only read.
+struct ModUpMacRows {
+  const uint64_t *digits;  // (poly, digit, tower, i)
+  const uint64_t *own;     // (poly, tower, i): evaluation-form input
+  const uint64_t *const *own_table;  // when set: polynomial p's input towers at own_table[p]
+  const uint64_t *key_b, *key_a;     // (digit, tower, i)
+  uint64_t *out_b, *out_a;           // (poly, tower, i)
+  const uint32_t *tower_prime;       // [towers]
+  uint64_t own_poly_stride, digit_poly_stride, out_poly_stride;
+  uint32_t towers, alpha, active_q, digits_count;
+  uint32_t first_tower, launch_towers, polys;
+  uint32_t wide, sum_log;
+};
+hipError_t launch_modup_mac_rows(const ModUpMacRows &m, const PrimeConst *primes, int arith, hipStream_t stream);
+
 }  // namespace nttamd

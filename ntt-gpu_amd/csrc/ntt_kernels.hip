@@ -819,6 +819,99 @@ __global__ __launch_bounds__(256) void ntt_pass_b8_fwd(TowerMap map, const Prime
   }
 }
 
+// ModUp's forward row pass fused with the key inner product (ModUpMacRows):
+// a workgroup takes 8 rows of tower t of one polynomial and runs, digit by
+// digit, the row pass of the digit's complement tower in registers (or loads
+// the owning digit's input rows), multiplies by both channels' key rows and
+// adds into 128-bit sums. The complement towers' finished transforms are
+// never written out and read back, and the sums are written once.
+// Wide: 128-bit sums, one Montgomery reduction per output (digits_count * q
+// < 2^64); otherwise each product is reduced to [0, 2q) (mont_half) into a
+// 64-bit sum below 2^(sum_log + 1) q, made canonical by sum_log + 1
+// conditional subtracts (2^(sum_log + 1) q <= 2^64): half the accumulator
+// registers for about twice the multiply instructions.
+template <int H, bool Wide>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void ntt_pass_b8_fwd_mac(
+    ModUpMacRows mm, const PrimeConst *__restrict__ primes) {
+  __shared__ uint64_t lds[8 * kLdsRow];
+  __shared__ tw2 tws[8 * (1 << kB8Staged)];
+  const uint32_t k = blockIdx.y / mm.polys, poly = blockIdx.y - k * mm.polys;
+  const uint32_t tower = mm.first_tower + k;
+  const PrimeConst pc = primes[mm.tower_prime[tower]];
+  const TwPtr tw = tw_table(pc.fwd);
+  const Lazy8 m = lazy8_of(pc.q, pc.nq, pc.nq2, pc.nq4);
+  const NegMultiples neg = neg_multiples(m);
+  const int t = threadIdx.x, u = t & 31, rl = t >> 5;
+  const uint32_t r = blockIdx.x * 8 + (uint32_t)rl;
+  const uint64_t at = ((uint64_t)tower << 16) + (uint64_t)r * kRow;  // row r of tower t (N = 2^16)
+  uint64_t *buf = lds + rl * kLdsRow;
+  tw2 *trow = tws + rl * (1 << kB8Staged);
+  const uint32_t own_digit = tower < mm.active_q ? tower / mm.alpha : 0xffffffffu;
+  const uint64_t *own = (mm.own_table ? mm.own_table[poly] : mm.own + (uint64_t)poly * mm.own_poly_stride) + at;
+  const uint64_t *dg = mm.digits + (uint64_t)poly * mm.digit_poly_stride + at;
+  const uint64_t step = (uint64_t)mm.towers << 16;  // digit stride of the digit buffer and the key
+  U128 sb[8], sa[8];
+  uint64_t cb[8], ca[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    sb[j] = sa[j] = U128{0, 0};
+    cb[j] = ca[j] = 0;
+  }
+#pragma unroll 1
+  for (uint32_t d = 0; d < mm.digits_count; ++d) {
+    uint64_t x[8];
+    if (d == own_digit) {
+      b8_load_w<kLazy>(x, own, u);
+    } else {
+      b8_fwd_core<kLazy, H>(x, dg + d * step, tw, kRow + r, u, m, neg, buf, trow);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) buf[pad16(b8_pos<kLayF>(u, j))] = canon_planned<kLazy, H>(x[j], m, neg, pc);
+      wave_sync8();
+#pragma unroll
+      for (int j = 0; j < 8; ++j) x[j] = buf[pad16(b8_pos<kLayW>(u, j))];
+      wave_sync8();  // the next digit's exchanges rewrite buf
+    }
+    uint64_t kb[8], ka[8];
+    b8_load_w<kLazy>(kb, mm.key_b + d * step + at, u);
+    b8_load_w<kLazy>(ka, mm.key_a + d * step + at, u);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if constexpr (Wide) {
+        add_wide(sb[j], mul_wide63(x[j], kb[j]));
+        add_wide(sa[j], mul_wide63(x[j], ka[j]));
+      } else {
+        cb[j] += mont_half(mul_wide63(x[j], kb[j]), pc.q, pc.qinv_neg);
+        ca[j] += mont_half(mul_wide63(x[j], ka[j]), pc.q, pc.qinv_neg);
+      }
+    }
+  }
+  uint64_t ob[8], oa[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    if constexpr (Wide) {
+      ob[j] = mont_reduce(sb[j], pc.q, pc.qinv_neg);
+      oa[j] = mont_reduce(sa[j], pc.q, pc.qinv_neg);
+    } else {
+      ob[j] = cb[j];
+      oa[j] = ca[j];
+    }
+  }
+  if constexpr (!Wide) {
+    // nq << k = -(2^k q) mod 2^64: subtract 2^k q, k = sum_log .. 0, where it fits
+    for (int k = (int)mm.sum_log; k >= 0; --k) {
+      const uint64_t nc = m.nq << k;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        ob[j] = sub_if_ge_neg(ob[j], nc);
+        oa[j] = sub_if_ge_neg(oa[j], nc);
+      }
+    }
+  }
+  const uint64_t o = (uint64_t)poly * mm.out_poly_stride + at;
+  b8_store_w(mm.out_b + o, u, ob);
+  b8_store_w(mm.out_a + o, u, oa);
+}
+
 // Inverse row pass, 8 residues per thread (first pass of an inverse
 // transform): load in W, stages of distance 1, 2, 4 in F, 8, 16, 32 in M,
 // 64, 128 in W, store from W. One row per 32 threads (row 8 bx + thread /
@@ -1209,6 +1302,50 @@ hipError_t launch_ntt_span(const TowerMap &map, const PrimeConst *primes, uint32
   m.polys = polys;
   m.y_offset = map.y_offset + y0;
   return dispatch(m, primes, log_n, count, inverse, arith, stream);
+}
+
+hipError_t launch_ntt_columns_span(const TowerMap &map, const PrimeConst *primes, uint32_t polys, uint32_t y0,
+                                   uint32_t count, int arith, hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  if (polys == 0 || (arith & kArithMask) != kLazy || (arith & kRow16Bit)) return hipErrorInvalidValue;
+  TowerMap m = map;
+  m.polys = polys;
+  m.y_offset = map.y_offset + y0;
+  const uint64_t y_end = (uint64_t)m.y_offset + count;
+  m.polys_mul = y_end * m.polys <= (uint64_t{1} << 32) ? ((uint64_t{1} << 32) + m.polys - 1) / m.polys : 0;
+  const uint32_t pad = lds_pad_for((uint32_t)(arith >> kWgCapShift) & 15u);
+  const dim3 grid(kRow / 16, count);
+  switch ((arith >> kArithShift) & 7) {
+    case 0: hipLaunchKernelGGL((ntt_pass_a256<false, kLazy, 0>), grid, dim3(256), pad, stream, m, primes); break;
+    case 1: hipLaunchKernelGGL((ntt_pass_a256<false, kLazy, 1>), grid, dim3(256), pad, stream, m, primes); break;
+    case 2: hipLaunchKernelGGL((ntt_pass_a256<false, kLazy, 2>), grid, dim3(256), pad, stream, m, primes); break;
+    case 3: hipLaunchKernelGGL((ntt_pass_a256<false, kLazy, 3>), grid, dim3(256), pad, stream, m, primes); break;
+    case 4: hipLaunchKernelGGL((ntt_pass_a256<false, kLazy, 4>), grid, dim3(256), pad, stream, m, primes); break;
+    case 5: hipLaunchKernelGGL((ntt_pass_a256<false, kLazy, 5>), grid, dim3(256), pad, stream, m, primes); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_modup_mac_rows(const ModUpMacRows &mm, const PrimeConst *primes, int arith, hipStream_t stream) {
+  if (mm.launch_towers == 0) return hipSuccess;
+  if (mm.polys == 0 || mm.digits_count == 0 || mm.alpha == 0 || (arith & kArithMask) != kLazy ||
+      (arith & kRow16Bit) || (uint64_t)mm.launch_towers * mm.polys > 65535 ||
+      mm.first_tower + mm.launch_towers > mm.towers ||
+      (!mm.wide && (mm.sum_log > 6 || mm.digits_count > (1u << mm.sum_log))))
+    return hipErrorInvalidValue;
+  const dim3 grid(kRow / 8, mm.launch_towers * mm.polys);
+  const int h = (arith >> kArithShift) & 7;
+  if (h > kMaxHeadroom) return hipErrorInvalidValue;
+  typedef void (*Kernel)(ModUpMacRows, const PrimeConst *);
+  static constexpr Kernel wide[] = {ntt_pass_b8_fwd_mac<0, true>, ntt_pass_b8_fwd_mac<1, true>,
+                                    ntt_pass_b8_fwd_mac<2, true>, ntt_pass_b8_fwd_mac<3, true>,
+                                    ntt_pass_b8_fwd_mac<4, true>, ntt_pass_b8_fwd_mac<5, true>};
+  static constexpr Kernel half[] = {ntt_pass_b8_fwd_mac<0, false>, ntt_pass_b8_fwd_mac<1, false>,
+                                    ntt_pass_b8_fwd_mac<2, false>, ntt_pass_b8_fwd_mac<3, false>,
+                                    ntt_pass_b8_fwd_mac<4, false>, ntt_pass_b8_fwd_mac<5, false>};
+  hipLaunchKernelGGL((mm.wide ? wide : half)[h], grid, dim3(256), 0, stream, mm, primes);
+  return hipGetLastError();
 }
 
 // `chunk` > 0 splits the towers into groups run pass after pass (A then B for

@@ -72,6 +72,9 @@ SIGNATURES = {
     "ntt_hip_set_ntt_shape": [VP, INT, ctypes.c_uint32, ctypes.c_uint32],
     "ntt_hip_context_device_bytes": [VP, ctypes.POINTER(SZ)],
     "ntt_hip_keyswitch_autotune": [VP, SZ, ctypes.c_char_p, SZ],
+    "ntt_hip_keyswitch_set_shape": [VP, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32],
+    "ntt_hip_keyswitch_get_shape": [VP, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+                                    ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(INT)],
     "ntt_hip_keyswitch_device_bytes": [VP, ctypes.POINTER(SZ)],
     "ntt_hip_transform_jobs": [VP, VP, VP, VP, SZ, VP, INT, VP],
     "ntt_hip_transform_mixed_workspace_bytes": [SZ, ctypes.POINTER(SZ)],
@@ -500,6 +503,20 @@ def set_tuning(ctx, ntt_chunk, conv_targets):
 
 def set_ntt_shape(ctx, fwd_headroom=-1, row_residues=8, wg_per_cu=0):
     check(lib().ntt_hip_set_ntt_shape(ctx.handle, fwd_headroom, row_residues, wg_per_cu), "set_ntt_shape")
+
+
+def set_keyswitch_shape(ks, lanes=2, mac_group=0, fuse_mac=2):
+    """Key-switch plan shape: concurrent lanes, ModUp/MAC interleave group,
+    ModUp row pass fused with the key inner product (0 off, 1 128-bit sums,
+    2 64-bit sums)."""
+    check(lib().ntt_hip_keyswitch_set_shape(ks.handle, lanes, mac_group, fuse_mac), "keyswitch_set_shape")
+
+
+def get_keyswitch_shape(ks):
+    la, g, f, fused = ctypes.c_uint32(0), ctypes.c_uint32(0), ctypes.c_uint32(0), INT(0)
+    check(lib().ntt_hip_keyswitch_get_shape(ks.handle, ctypes.byref(la), ctypes.byref(g), ctypes.byref(f),
+                                            ctypes.byref(fused)), "keyswitch_get_shape")
+    return {"lanes": la.value, "mac_group": g.value, "fuse_mac": f.value, "fused": bool(fused.value)}
 
 
 def get_ntt_shape(ctx):

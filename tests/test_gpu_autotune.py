@@ -102,19 +102,23 @@ def test_tuned_choice_is_exact_and_not_slower():
         hip.set_tuning(ctx, tuning["ntt_chunk"], tuning["conv_targets"])
         ctx.transform_batch(data.ptr, towers, N16, 0, 1, False, stream=st.handle)
         e0.record(st.handle)
-        for _ in range(2):
+        for _ in range(4):
             ctx.transform_batch(data.ptr, towers, N16, 0, 1, False, stream=st.handle)
             ctx.transform_batch(data.ptr, towers, N16, 0, 1, True, stream=st.handle)
         e1.record(st.handle)
         e1.synchronize()
         return e0.elapsed_ms(e1)
 
+    # alternating pairs, compared by the median of their ratios: single
+    # samples of a few milliseconds spread by up to 20% with the clock, and
+    # pairing cancels the drift between samples
     default = ({"fwd_headroom": -1, "row_residues": 8, "wg_per_cu": 0}, {"ntt_chunk": 0, "conv_targets": 0})
     td, tt = [], []
-    for _ in range(4):
+    for _ in range(8):
         td.append(timed(*default))
         tt.append(timed(*tuned))
-    assert statistics.median(tt) <= 1.05 * statistics.median(td), (rep, td, tt)
+    ratios = [t / d for t, d in zip(tt, td)]
+    assert statistics.median(ratios) <= 1.05, (rep, td, tt)
     # the tuned shape is bit-exact
     x = uniform(N16, [q] * 2, 99)
     buf = hip.DeviceBuffer(x.size).upload(x)
@@ -137,7 +141,7 @@ def test_keyswitch_autotune_is_exact_and_not_slower():
     ks.upload_key(kb, ka, hp.M, hp.q)
     log_case("keyswitch autotune L=24 batch 8")
     rep = ks.autotune(B)
-    assert {"lanes", "mac_group", "conv_targets"} <= set(rep["chosen"])
+    assert {"lanes", "fuse_mac", "mac_group", "conv_targets"} <= set(rep["chosen"])
     x = np.concatenate([uniform(N16, hp.q, 90 + c) for c in range(B)])
     comp = 24 * N16
     d_in = hip.DeviceBuffer(x.size).upload(x)
@@ -166,7 +170,8 @@ def test_keyswitch_autotune_is_exact_and_not_slower():
         return e0.elapsed_ms(e1)
 
     td, tt = [], []
-    for _ in range(4):
+    for _ in range(8):
         td.append(timed(ks2))
         tt.append(timed(ks))
-    assert statistics.median(tt) <= 1.05 * statistics.median(td), (rep, td, tt)
+    ratios = [t / d for t, d in zip(tt, td)]
+    assert statistics.median(ratios) <= 1.05, (rep, td, tt)

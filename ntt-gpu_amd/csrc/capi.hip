@@ -175,22 +175,26 @@ struct ntt_hip_context {
   int fwd_headroom = -1;       // -1: from the moduli; 0..kMaxHeadroom: at most that
   bool row16 = false;          // 16 residues per thread in the N = 2^16 row pass
   uint32_t wg_cap = 0;         // workgroups per CU for the N = 2^16 passes (0 = no cap)
+  // A modulus below 2^24 in the basis: the quotient estimate that headroom
+  // classes above 0 use to reduce (reduce2_est) needs 24-bit moduli, so
+  // every launch of such a context runs class 0 (a reduction per stage).
+  bool small_q = false;
   int ntt_mode() const {
     int a = arith;
-    if ((a & kArithMask) == kLazy && fwd_headroom >= 0) {
-      const int h = std::min(fwd_headroom, (a >> kArithShift) & 7);
+    if ((a & kArithMask) == kLazy && (fwd_headroom >= 0 || small_q)) {
+      const int h = small_q ? 0 : std::min(fwd_headroom, (a >> kArithShift) & 7);
       a = kLazy | (h << kArithShift);
     }
     return nttamd::ntt_mode(a, row16, wg_cap);
   }
-  // The mode of a forward launch over towers [first, first + count) only:
-  // the headroom class of their own largest modulus (at most the tuned
+  // The mode of a launch over towers [first, first + count) only: the
+  // headroom class of their own largest modulus (at most the tuned
   // fwd_headroom), so narrower towers skip reductions a wider tower of the
-  // basis would need. Inverse launches ignore the class.
+  // basis would need, forward and inverse.
   int ntt_mode_for_q(uint64_t max_q) const {
     int a = arith;
     if ((a & kArithMask) == kLazy) {
-      int h = headroom_for(max_q);
+      int h = small_q ? 0 : headroom_for(max_q);
       if (fwd_headroom >= 0) h = std::min(h, fwd_headroom);
       a = kLazy | (h << kArithShift);
     }
@@ -672,6 +676,7 @@ int ntt_hip_context_create(ntt_hip_context **out, int device, uint64_t n, const 
     c->psi.assign(psi, psi + towers);
     // 32-bit butterflies when every q < 2^30; wide-lazy (residues below 8q < 2^64) when every q < 2^61
     c->arith = arith_for(*std::max_element(c->q.begin(), c->q.end()));
+    c->small_q = *std::min_element(c->q.begin(), c->q.end()) < (uint64_t{1} << 24);
     // Host tables: fwd[k] = {psi^bitrev(k), Shoup}, inv[k] = {psi^-bitrev(k), Shoup}
     // (reference transform_stage.hpp:19-30), one thread per group of towers.
     std::vector<ulonglong2> host(towers * 2 * (size_t)n);

@@ -115,6 +115,25 @@ def test_fused_primes_below_2_61():
             assert np.array_equal(gb[c * w:(c + 1) * w], wb) and np.array_equal(ga[c * w:(c + 1) * w], wa), (fuse, c)
 
 
+def test_fused_with_a_prime_below_2_24():
+    """A Q basis with a 23-bit prime beside 60- and 50-bit ones (the context
+    runs headroom class 0), three one-tower digits, fused 64-bit sums: equal
+    to the oracle."""
+    m = 2 * N16
+    q = [params.last_prime(60, m), params.last_prime(50, m), params.last_prime(23, m)]
+    hp = params.HybridParams(n=N16, q=q, p=[params.previous_prime(q[0], m)], alpha=1)
+    L, B = 3, 2
+    ctx, ks, kb, ka = _plan(hp, L, B, 9850)
+    assert hip.get_keyswitch_shape(ks)["fused"]
+    x = np.concatenate([uniform(N16, hp.q, 9860 + c) for c in range(B)])
+    log_case(f"modup+mac n=65536 q={q} alpha=1")
+    gb, ga = _run(ctx, ks, x, L, B)
+    w = L * N16
+    for c in range(B):
+        wb, wa = _oracle(hp, L, x[c * w:(c + 1) * w], kb, ka)
+        assert np.array_equal(gb[c * w:(c + 1) * w], wb) and np.array_equal(ga[c * w:(c + 1) * w], wa), c
+
+
 def test_fused_gather_equals_contiguous_unfused():
     """run_gather (components at scattered addresses: the fused kernel reads
     each digit's own towers through the pointer table) with the fused kernel

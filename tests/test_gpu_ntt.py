@@ -175,6 +175,28 @@ def test_narrow_primes(n):
         assert np.array_equal(_gpu_round(ctx, x, 0, T, True), o.ntt_towers(x, n, qs, rs, inverse=True))
 
 
+def test_primes_below_2_24_in_a_wide_context():
+    """Primes below 2^24 beside 50- and 60-bit primes at N = 2^16: such a
+    context runs headroom class 0 (the quotient estimate of the higher
+    classes needs moduli of 24 bits or more), on its own towers and on the
+    small towers alone; forward and inverse equal the oracle, all-(q-1) and
+    seeded inputs."""
+    m = 2 * N16
+    qs = [params.last_prime(60, m), params.last_prime(50, m), params.last_prime(23, m), params.last_prime(20, m)]
+    rs = [params.find_psi(q, N16) for q in qs]
+    ctx = hip.Context(N16, qs, rs)
+    log_case(f"ntt n=65536 primes below 2^24 beside wide ones q={qs}")
+    T = len(qs)
+    for x in (np.concatenate([np.full(N16, q - 1, np.uint64) for q in qs]), uniform(N16, qs, 31)):
+        f = _gpu_round(ctx, x, 0, T, False)
+        assert np.array_equal(f, o.ntt_towers(x, N16, qs, rs))
+        assert np.array_equal(_gpu_round(ctx, f, 0, T, True), x)
+        assert np.array_equal(_gpu_round(ctx, x, 0, T, True), o.ntt_towers(x, N16, qs, rs, inverse=True))
+        tail = x[2 * N16:]
+        assert np.array_equal(_gpu_round(ctx, tail, 2, 2, False), o.ntt_towers(tail, N16, qs[2:], rs[2:]))
+        assert np.array_equal(_gpu_round(ctx, tail, 2, 2, True), o.ntt_towers(tail, N16, qs[2:], rs[2:], inverse=True))
+
+
 def test_narrow_packed_handoff_out_of_place_and_strided():
     """32-bit towers at N = 2^16 hand their values between the passes packed in
     the first half of each tower: out of place in both directions (the source

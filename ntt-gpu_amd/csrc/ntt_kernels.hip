@@ -864,8 +864,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void n
       b8_load_w<kLazy>(x, own, u);
     } else {
       b8_fwd_core<kLazy, H>(x, dg + d * step, tw, kRow + r, u, m, neg, buf, trow);
+      // 128-bit sums need canonical values (digits_count * q * q < q * 2^64);
+      // a half-reduced product is below x K / 2^64 + q < 2q for any 64-bit
+      // x and K < q, so the 64-bit sums take the row pass's lazy outputs
 #pragma unroll
-      for (int j = 0; j < 8; ++j) buf[pad16(b8_pos<kLayF>(u, j))] = canon_planned<kLazy, H>(x[j], m, neg, pc);
+      for (int j = 0; j < 8; ++j)
+        buf[pad16(b8_pos<kLayF>(u, j))] = Wide ? canon_planned<kLazy, H>(x[j], m, neg, pc) : x[j];
       wave_sync8();
 #pragma unroll
       for (int j = 0; j < 8; ++j) x[j] = buf[pad16(b8_pos<kLayW>(u, j))];

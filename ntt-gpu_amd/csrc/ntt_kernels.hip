@@ -829,7 +829,10 @@ __global__ __launch_bounds__(256) void ntt_pass_b8_fwd(TowerMap map, const Prime
 // < 2^64); otherwise each product is reduced to [0, 2q) (mont_half) into a
 // 64-bit sum below 2^(sum_log + 1) q, made canonical by sum_log + 1
 // conditional subtracts (2^(sum_log + 1) q <= 2^64): half the accumulator
-// registers for about twice the multiply instructions.
+// registers for about twice the multiply instructions. Measured and not
+// taken (both spill at 4 waves per SIMD): sums of two digits' products per
+// reduction (100+ registers spilled) and the quotient estimate for the
+// final reduction (4 spilled, 0.6% slower).
 template <int H, bool Wide>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void ntt_pass_b8_fwd_mac(
     ModUpMacRows mm, const PrimeConst *__restrict__ primes) {

@@ -627,6 +627,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--roofline-only", action="store_true")
+    ap.add_argument("--keyswitch-only", action="store_true",
+                    help="for counter runs: only the batched key switch (warm-up and timed graph replays)")
     ap.add_argument("--roofline-bits", type=int, default=59)
     ap.add_argument("--serve-rates", type=str, default="200,1000")
     ap.add_argument("--serve-device-rates", type=str, default="1000,5000,20000")
@@ -709,6 +711,10 @@ def main():
     gpu_a = d_a.download(comp, 0)
 
     progress(f"batched done: {batched_kss:.1f} KS/s")
+    if args.keyswitch_only:
+        print(json.dumps({"batched_keyswitch_per_s": round(batched_kss, 2), "batch": B, "steps": args.steps,
+                          "warmup": args.warmup, "autotune": ks_tuning}), flush=True)
+        return
     # ---- plain: one component per submission, per-stage launches, sync per request
     ws1 = hip.DeviceBuffer(ks.workspace_words(1))
     for _ in range(2):

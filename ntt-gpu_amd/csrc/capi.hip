@@ -1317,8 +1317,9 @@ void ks_modup_mac_fused(const ntt_hip_keyswitch &key, const ntt_hip_keyswitch &p
   }
 }
 
-// ModUp and the key inner product of `key` into z, interleaved by tower
-// groups (mac_group) when the plan asks for it.
+// ModUp and the key inner product of `key` into z: the fused row pass where
+// it applies, otherwise interleaved by tower groups (mac_group) when the
+// plan asks for it.
 constexpr size_t kMinGroupBatch = 8;
 void ks_modup_mac(const ntt_hip_keyswitch &key, const ntt_hip_keyswitch &plan, KsIn in, size_t B,
                   const KsBuffers &w, uint64_t *z, hipStream_t s) {

@@ -31,6 +31,7 @@ BUTTERFLIES_PER_TOWER = (N // 2) * 16  # one direction of a 2^16-point transform
 # HBM bytes of the roofline launch measured by rocprofv3 --pmc FETCH_SIZE /
 # WRITE_SIZE (per dispatch, calibrated on a copy kernel of known size)
 HBM_COUNTERS = os.path.join(ROOT, "profiles", "ntt_n65536_hbm_counters.json")
+HBM_COUNTERS_30 = os.path.join(ROOT, "profiles", "ntt_n65536_hbm_counters_30bit.json")
 VALU_PROBE = os.path.join(ROOT, "ntt-gpu_amd", "bin", "valu_probe")
 SESSION_BENCH = os.path.join(ROOT, "ntt-gpu_amd", "bin", "session_keyswitch_bench")
 
@@ -128,7 +129,7 @@ def ntt_roofline(towers, reps, stream, bits=59, autotune=True):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": measured_traffic(towers) if bits > 30 else None,
+        "traffic": measured_traffic(towers, bits),
         "kernel": f"forward NTT (ntt_pass_a256 + ntt_pass_b8_fwd), N=65536, {bits}-bit prime",
         "towers_per_launch": towers,
         "launch_ms": round(ms, 4),
@@ -138,12 +139,13 @@ def ntt_roofline(towers, reps, stream, bits=59, autotune=True):
     }
 
 
-def measured_traffic(towers):
+def measured_traffic(towers, bits=59):
     """HBM bytes of one forward roofline launch over `towers` towers, from the
-    committed counter summary (FETCH_SIZE and WRITE_SIZE per dispatch of both
-    passes, each scaled by the copy-kernel calibration), per tower x towers."""
+    committed counter summary of the prime's width (FETCH_SIZE and WRITE_SIZE
+    per dispatch of both passes, each scaled by the copy-kernel calibration;
+    30-bit towers hand off packed 32-bit words), per tower x towers."""
     try:
-        with open(HBM_COUNTERS) as f:
+        with open(HBM_COUNTERS_30 if bits <= 30 else HBM_COUNTERS) as f:
             c = json.load(f)
     except (OSError, ValueError):
         return None
